@@ -1,0 +1,284 @@
+"""cudadecon_amd - MI355X (gfx950) build of cudaDecon's RL-Biggs-Andrews path.
+
+The product is ``libcudaDecon.so`` in this directory: a C-ABI library whose
+exported symbols are those of the reference library (src/linearDecon.h:243-317)
+plus the ``cudaDecon_*`` extensions declared in ``include/cudaDecon.h``.  This
+module binds them with ctypes the way pycudadecon binds the reference
+library, and adds thin numpy conveniences.
+
+The library is loaded lazily; when it is missing every call raises
+``RuntimeError`` (there is no CPU fallback).  Build it with
+``python -c "import __graft_entry__ as g; g.build()"`` from the repository
+root, or ``make -C cudadecon_amd/csrc``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "lib", "lib_path", "RLEngine", "rl_init", "rl_decon", "rl_cleanup", "deskew", "affine", "affine_ra",
+    "fft_r2c", "fft_c2r", "filter_volume", "output_shape", "last_error",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libcudaDecon.so")
+
+
+_f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u16 = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+# Exported symbols and their ctypes signatures (include/cudaDecon.h).
+SIGNATURES = {
+    # reference API
+    "RL_interface_init": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int, C.c_bool, C.c_char_p]),
+    "RL_interface": (C.c_int, [_u16, C.c_int, C.c_int, C.c_int, _f, C.c_void_p, C.c_float, C.c_bool, C.c_bool,
+                               C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_bool, C.c_bool]),
+    "Deskew_interface": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, _f, C.c_int,
+                                   C.c_int, C.c_float]),
+    "Affine_interface": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f, _f]),
+    "Affine_interface_RA": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, _f, _f]),
+    "camcor_interface_init": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "camcor_interface": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "RL_cleanup": (None, []),
+    "cuda_reset": (None, []),
+    "get_output_nx": (C.c_uint, []),
+    "get_output_ny": (C.c_uint, []),
+    "get_output_nz": (C.c_uint, []),
+    "get_version": (C.c_char_p, []),
+    # extensions
+    "cudaDecon_last_error": (C.c_char_p, []),
+    "cudaDecon_fft_supported": (C.c_int, [C.c_int]),
+    "cudaDecon_find_optimal_dimension": (C.c_uint, [C.c_uint, C.c_int]),
+    "cudaDecon_get_session_dims": (C.c_uint, [C.c_int]),
+    "cudaDecon_fft_r2c": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f]),
+    "cudaDecon_fft_c2r": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f]),
+    "cudaDecon_filter": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int, C.c_float,
+                                   C.c_float, C.c_float, C.c_int]),
+    "cudaDecon_rl_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int, C.c_float,
+                                         C.c_float, C.c_float]),
+    "cudaDecon_rl_set_raw": (C.c_int, [C.c_void_p, _f]),
+    "cudaDecon_rl_set_seed": (C.c_int, [C.c_void_p, _f]),
+    "cudaDecon_rl_iterate": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "cudaDecon_rl_get": (C.c_int, [C.c_void_p, _f]),
+    "cudaDecon_rl_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "cudaDecon_rl_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cudaDecon_rl_destroy": (None, [C.c_void_p]),
+    "cudaDecon_device_sync": (C.c_int, []),
+}
+
+PASS_NAMES = ["x_r2c", "y_fwd", "z_filter", "y_inv", "x_lrcore", "x_update", "lambda"]
+
+
+def lib():
+    """Load libcudaDecon.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; the HIP extension is required (no CPU fallback)")
+        L = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    return lib().cudaDecon_last_error().decode()
+
+
+def _check(rc, what):
+    if not rc:
+        raise RuntimeError(f"{what} failed: {last_error()}")
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---------------------------------------------------------------------------
+# reference-style session API (pycudadecon's RL_interface usage)
+# ---------------------------------------------------------------------------
+def rl_init(rawdata_shape, otfpath, dzdata=0.5, dxdata=0.1, dzpsf=0.1, dxpsf=0.1, deskew=0.0, rotate=0.0,
+            width=0, skewed_decon=False):
+    """RL_interface_init for a (nz, ny, nx) stack; returns nothing, raises on failure."""
+    nz, ny, nx = rawdata_shape
+    rc = lib().RL_interface_init(int(nx), int(ny), int(nz), float(dxdata), float(dzdata), float(dxpsf),
+                                 float(dzpsf), float(deskew), float(rotate), int(width), bool(skewed_decon),
+                                 os.fsencode(otfpath))
+    _check(rc, "RL_interface_init")
+
+
+def output_shape():
+    L = lib()
+    return (int(L.get_output_nz()), int(L.get_output_ny()), int(L.get_output_nx()))
+
+
+def rl_decon(im, background=100.0, n_iters=10, shift=0, save_deskewed=False, rescale=False, napodize=0,
+             nz_blend=0, pad_val=0.0, dup_reversed_z=False, skewed_decon=False):
+    """RL_interface on a uint16 (nz, ny, nx) stack; returns result (and the
+    deskewed raw when save_deskewed)."""
+    im = np.ascontiguousarray(im, dtype=np.uint16)
+    nz, ny, nx = im.shape
+    out = np.empty(output_shape(), np.float32)
+    L = lib()
+    desk = None
+    dptr = None
+    if save_deskewed:
+        desk = np.empty(deskewed_shape(), np.float32)
+        dptr = desk.ctypes.data
+    rc = L.RL_interface(im, nx, ny, nz, out, dptr, float(background), bool(rescale), bool(save_deskewed),
+                        int(n_iters), int(shift), int(napodize), int(nz_blend), float(pad_val),
+                        bool(dup_reversed_z), bool(skewed_decon))
+    _check(rc, "RL_interface")
+    return (out, desk) if save_deskewed else out
+
+
+def deskewed_shape():
+    """(nz, ny, nx) of the deskewed raw stack of the current session."""
+    L = lib()
+    return (int(L.cudaDecon_get_session_dims(0)), int(L.get_output_ny()), int(L.cudaDecon_get_session_dims(1)))
+
+
+def rl_cleanup():
+    lib().RL_cleanup()
+
+
+def deskew(im, angle, dz, dx, width=0, shift=0, pad_val=0.0, out_width=None):
+    """Deskew_interface on a float (nz, ny, nx) stack.  out_width must be
+    given when width == 0 (the reference sizes it from its session globals)."""
+    im = _f32(im)
+    nz, ny, nx = im.shape
+    w = int(width) if width else int(out_width)
+    out = np.empty((nz, ny, w), np.float32)
+    _check(lib().Deskew_interface(im, nx, ny, nz, float(dz), float(dx), float(angle), out, int(width), int(shift),
+                                  float(pad_val)), "Deskew_interface")
+    return out
+
+
+def affine(im, mat):
+    im = _f32(im)
+    nz, ny, nx = im.shape
+    out = np.empty_like(im)
+    _check(lib().Affine_interface(im, nx, ny, nz, out, _f32(np.asarray(mat).reshape(16))), "Affine_interface")
+    return out
+
+
+def affine_ra(im, mat, dx, dy, dz):
+    im = _f32(im)
+    nz, ny, nx = im.shape
+    out = np.empty_like(im)
+    _check(lib().Affine_interface_RA(im, nx, ny, nz, float(dx), float(dy), float(dz), out,
+                                     _f32(np.asarray(mat).reshape(16))), "Affine_interface_RA")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# extensions
+# ---------------------------------------------------------------------------
+def fft_r2c(v):
+    v = _f32(v)
+    nz, ny, nx = v.shape
+    out = np.empty((nz, ny, nx // 2 + 1), np.complex64)
+    _check(lib().cudaDecon_fft_r2c(v, nx, ny, nz, out.view(np.float32).reshape(-1)), "cudaDecon_fft_r2c")
+    return out
+
+
+def fft_c2r(s, nx):
+    s = np.ascontiguousarray(s, dtype=np.complex64)
+    nz, ny, _ = s.shape
+    out = np.empty((nz, ny, nx), np.float32)
+    _check(lib().cudaDecon_fft_c2r(s.view(np.float32).reshape(-1), nx, ny, nz, out), "cudaDecon_fft_c2r")
+    return out
+
+
+def otf_image(otf_table: np.ndarray) -> np.ndarray:
+    """OTF image as stored in its TIFF: an RA table (nr, nzotf) complex64 is
+    width 2*nzotf x height nr; a 3-D table (nzotf, nyotf, nxotf) complex64 is
+    width 2*nxotf x height nyotf x depth nzotf."""
+    t = np.ascontiguousarray(otf_table, dtype=np.complex64)
+    img = t.view(np.float32)
+    if t.ndim == 2:
+        return img.reshape(1, t.shape[0], 2 * t.shape[1])
+    return img.reshape(t.shape[0], t.shape[1], 2 * t.shape[2])
+
+
+def filter_volume(v, otf_table, kscale, conj=False):
+    """One filterGPU call: C2R(H . R2C(v / N)) with H interpolated from otf_table."""
+    v = _f32(v).copy()
+    nz, ny, nx = v.shape
+    img = otf_image(otf_table)
+    d, h, w = img.shape
+    _check(lib().cudaDecon_filter(v, nx, ny, nz, np.ascontiguousarray(img).reshape(-1), w, h, d,
+                                  float(kscale[0]), float(kscale[1]), float(kscale[2]), int(conj)),
+           "cudaDecon_filter")
+    return v
+
+
+class RLEngine:
+    """RL-Biggs-Andrews on HBM-resident state (cudaDecon_rl_* handle)."""
+
+    def __init__(self, shape, otf_table, kscale):
+        nz, ny, nx = shape
+        self.shape = (nz, ny, nx)
+        img = otf_image(otf_table)
+        d, h, w = img.shape
+        self._img = np.ascontiguousarray(img).reshape(-1)
+        self.h = lib().cudaDecon_rl_create(nx, ny, nz, self._img, w, h, d, float(kscale[0]), float(kscale[1]),
+                                           float(kscale[2]))
+        if not self.h:
+            raise RuntimeError(f"cudaDecon_rl_create failed: {last_error()}")
+
+    def set_raw(self, raw):
+        _check(lib().cudaDecon_rl_set_raw(self.h, _f32(raw)), "cudaDecon_rl_set_raw")
+
+    def set_seed(self, seed):
+        _check(lib().cudaDecon_rl_set_seed(self.h, _f32(seed)), "cudaDecon_rl_set_seed")
+
+    def iterate(self, n, k0=0, seeded=False):
+        """Run n iterations; returns (lambdas, device ms)."""
+        lam = np.zeros(max(int(n), 1), np.float64)
+        ms = np.zeros(1, np.float64)
+        _check(lib().cudaDecon_rl_iterate(self.h, int(n), int(k0), int(seeded), lam.ctypes.data, ms.ctypes.data),
+               "cudaDecon_rl_iterate")
+        return lam[:n], float(ms[0])
+
+    def get(self):
+        out = np.empty(self.shape, np.float32)
+        _check(lib().cudaDecon_rl_get(self.h, out), "cudaDecon_rl_get")
+        return out
+
+    def timing(self, enable=-1):
+        ms = np.zeros(7, np.float64)
+        calls = np.zeros(7, np.int64)
+        _check(lib().cudaDecon_rl_timing(self.h, int(enable), ms.ctypes.data, calls.ctypes.data),
+               "cudaDecon_rl_timing")
+        return dict(zip(PASS_NAMES, ms)), dict(zip(PASS_NAMES, calls))
+
+    def geometry(self):
+        v = (C.c_long * 3)()
+        _check(lib().cudaDecon_rl_geometry(self.h, C.byref(v, 0), C.byref(v, 8), C.byref(v, 16)),
+               "cudaDecon_rl_geometry")
+        return int(v[0]), int(v[1]), int(v[2])
+
+    def close(self):
+        if self.h:
+            lib().cudaDecon_rl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,595 @@
+// extern "C" boundary of libcudaDecon.so (declarations in include/cudaDecon.h).
+//
+// Session state mirrors the reference library's process globals
+// (src/RL-Biggs-Andrews.cpp:45-47, 459-466).  Every entry point catches all
+// exceptions and returns 0 (or a neutral value) with the message kept for
+// cudaDecon_last_error().
+#include "../../include/cudaDecon.h"
+
+#include "decon.hpp"
+#include "rl_kernels.hpp"
+#include "tiff.hpp"
+
+#include <cfloat>
+#include <cmath>
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+
+namespace cdn {
+namespace {
+
+thread_local std::string g_err;
+
+// findOptimalDimension / notGoodDimension (src/RL-Biggs-Andrews.cpp:18-43);
+// n <= 1 is returned as is (the reference recurses forever at 1).
+bool not_good(unsigned n) {
+  while (true) {
+    if (n <= 1) return false;
+    if (n == 2 || n == 3 || n == 5 || n == 7) return false;
+    if (n % 2 == 0) { n /= 2; continue; }
+    if (n % 3 == 0) { n /= 3; continue; }
+    if (n % 5 == 0) { n /= 5; continue; }
+    if (n % 7 == 0) { n /= 7; continue; }
+    return true;
+  }
+}
+unsigned find_optimal_dimension(unsigned n, int step = -1) {
+  if (n <= 1) return n;
+  unsigned o = n;
+  while (not_good(o)) o += step;
+  return o;
+}
+
+struct Session {
+  hipStream_t st = nullptr;
+  unsigned output_nx = 0, output_ny = 0, output_nz = 0;
+  bool crop = false;
+  double deskewFactor = 0.0;
+  unsigned deskewedXdim = 0;
+  bool rot_on = false;
+  float rot[4] = {0, 0, 0, 0};
+  float dr = 0, dz = 0;  // dz after the |sin| factor when deskewing
+  unsigned fft_nx = 0;   // x length of the iteration grid
+  OtfTable otf;
+  RLEngine eng;
+  DevBuf u16, bufA, bufB, red_part, red_acc, red_mean;
+  ~Session() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+std::unique_ptr<Session> g_s;
+// bleach-correction statics (src/RL-Biggs-Andrews.cpp:46-47)
+double g_intensity0 = 0.0;
+bool g_first = true;
+// output_nz survives RL_cleanup in the reference and feeds Deskew_interface
+// (src/RL-Biggs-Andrews.cpp:459,741)
+unsigned g_output_nz = 0;
+
+hipStream_t make_stream() {
+  hipStream_t s;
+  CDN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+// k-scales for an FFT grid of x-length fft_nx (src/RL-Biggs-Andrews.cpp:634-642)
+void kscales(const Session& S, unsigned fft_nx, float ks[3]) {
+  const float dkx = (float)(1.0 / (S.dr * (float)fft_nx));
+  const float dky = (float)(1.0 / (S.dr * (float)S.output_ny));
+  const float dkz = (float)(1.0 / (S.dz * (float)S.output_nz));
+  ks[0] = dkx / S.otf.dkx;
+  ks[1] = dky / S.otf.dky;
+  ks[2] = dkz / S.otf.dkz;
+}
+
+// rotated sizes (src/RL-Biggs-Andrews.cpp:410-411), float arithmetic
+void rotated_dims(const float* p, unsigned nx, unsigned nz, int* nx_o, int* nz_o) {
+  *nz_o = (int)((float)nz * p[3] / p[0]);
+  *nx_o = (int)((float)nx * p[3] + (float)nz * p[2] * p[2] / std::fabs(p[1]));
+}
+
+template <class F> int guard(F&& f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+  } catch (...) {
+    g_err = "unknown error";
+  }
+  return 0;
+}
+
+}  // namespace
+}  // namespace cdn
+
+using namespace cdn;
+
+struct cudaDecon_rl {
+  hipStream_t st = nullptr;
+  RLEngine eng;
+  int k_next = 0;
+  ~cudaDecon_rl() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" {
+
+const char* cudaDecon_last_error(void) { return g_err.c_str(); }
+
+const char* get_version(void) { return "0.7.0"; }
+
+int cudaDecon_fft_supported(int n) { return fft_length_supported(n) ? 1 : 0; }
+
+unsigned cudaDecon_find_optimal_dimension(unsigned n, int step) { return find_optimal_dimension(n, step); }
+
+int RL_interface_init(int nx, int ny, int nz, float dr, float dz, float dr_psf, float dz_psf, float deskewAngle,
+                      float rotationAngle, int outputWidth, bool bSkewedDecon, char* OTF_file_name) {
+  return guard([&]() -> int {
+    if (nx <= 0 || ny <= 0 || nz <= 0) fail("RL_interface_init: non-positive dimensions");
+    if (!OTF_file_name) fail("RL_interface_init: OTF file name is null");
+    auto S = std::make_unique<Session>();
+    S->output_ny = find_optimal_dimension(ny);
+    S->output_nz = find_optimal_dimension(nz);
+    const bool has_deskew = std::fabs(deskewAngle) > 0.0;
+    // the CLI also optimises the width for skewed decon
+    // (src/linearDecon.cpp:643); the library path leaves it unoptimised and
+    // then plans an FFT of the deskewed width over the raw width
+    S->output_nx = (has_deskew && !bSkewedDecon) ? (unsigned)nx : find_optimal_dimension(nx);
+    S->crop = (S->output_nx != (unsigned)nx) || (S->output_ny != (unsigned)ny) || (S->output_nz != (unsigned)nz);
+
+    const TiffImage img = tiff_read(OTF_file_name);
+    if (bSkewedDecon) dz_psf *= std::fabs(std::sin(deskewAngle * M_PI / 180.));
+    S->otf.set_from_image(img.data.data(), img.width, img.height, img.depth, dr_psf, dz_psf);
+
+    S->deskewFactor = 0.;
+    S->deskewedXdim = S->output_nx;
+    if (has_deskew) {
+      if (deskewAngle < 0) deskewAngle += 180.;
+      S->deskewFactor = std::cos(deskewAngle * M_PI / 180.) * dz / dr;
+      if (outputWidth == 0)
+        S->deskewedXdim = (unsigned)(S->deskewedXdim +
+                                     std::floor(S->output_nz * dz * std::fabs(std::cos(deskewAngle * M_PI / 180.)) / dr));
+      else
+        S->deskewedXdim = (unsigned)outputWidth;
+      S->deskewedXdim = find_optimal_dimension(S->deskewedXdim);
+      dz *= std::fabs(std::sin(deskewAngle * M_PI / 180.));
+    }
+    if (std::fabs(rotationAngle) > 0.0) {
+      rotationAngle *= M_PI / 180;
+      const float stretch = dr / dz;
+      S->rot_on = true;
+      S->rot[0] = std::cos(rotationAngle) * stretch;
+      S->rot[1] = std::sin(rotationAngle) * stretch;
+      S->rot[2] = -std::sin(rotationAngle);
+      S->rot[3] = std::cos(rotationAngle);
+      if (S->rot[1] == 0.f || S->rot[0] == 0.f) fail("RL_interface_init: degenerate rotation angle");
+    }
+    S->dr = dr;
+    S->dz = dz;
+    const unsigned fft_nx = bSkewedDecon ? S->output_nx : S->deskewedXdim;
+    for (unsigned n : {fft_nx, S->output_ny, S->output_nz})
+      if (!fft_length_supported((int)n))
+        fail("RL_interface_init: FFT length " + std::to_string(n) + " is not compiled into libcudaDecon");
+    S->fft_nx = fft_nx;
+    S->st = make_stream();
+    g_output_nz = S->output_nz;
+    g_s = std::move(S);
+    return 1;
+  });
+}
+
+unsigned get_output_nx(void) {
+  if (!g_s) return 0;
+  if (g_s->rot_on) {
+    int nxo, nzo;
+    rotated_dims(g_s->rot, g_s->deskewedXdim, g_s->output_nz, &nxo, &nzo);
+    return (unsigned)nxo;
+  }
+  return g_s->deskewedXdim;
+}
+unsigned get_output_ny(void) { return g_s ? g_s->output_ny : 0; }
+
+unsigned cudaDecon_get_session_dims(int which) {
+  if (!g_s) return 0;
+  switch (which) {
+    case 0: return g_s->output_nz;
+    case 1: return g_s->deskewedXdim;
+    case 2: return g_s->output_nx;
+    case 3: return g_s->fft_nx;
+    default: return 0;
+  }
+}
+unsigned get_output_nz(void) {
+  if (!g_s) return 0;
+  if (g_s->rot_on) {
+    int nxo, nzo;
+    rotated_dims(g_s->rot, g_s->deskewedXdim, g_s->output_nz, &nxo, &nzo);
+    return (unsigned)nzo;
+  }
+  return g_s->output_nz;
+}
+
+int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, float* result,
+                 float* raw_deskewed_result, float background, bool bDoRescale, bool bSaveDeskewedRaw, int nIters,
+                 int extraShift, int napodize, int nZblend, float padVal, bool bDupRevStack, bool bSkewedDecon) {
+  // bDupRevStack has no effect on the library path's result: its FFT plans and
+  // kernel sizes cover only the first nz slices (src/RL-Biggs-Andrews.cpp:193-205
+  // against :622-642), so the mirrored half is never filtered.
+  (void)bDupRevStack;
+  return guard([&]() -> int {
+    if (!g_s) fail("RL_interface called before RL_interface_init");
+    Session& S = *g_s;
+    if (!raw_data || !result) fail("RL_interface: null buffer");
+    if ((unsigned)nx < S.output_nx || (unsigned)ny < S.output_ny || (unsigned)nz < S.output_nz)
+      fail("RL_interface: stack smaller than the initialised size");
+    // the reference indexes outside the stack for these (apodize and z-blend
+    // loops run napodize / nZblend planes from each edge)
+    if (napodize < 0 || (unsigned)napodize > std::min(S.output_ny, S.output_nx))
+      fail("RL_interface: napodize exceeds the stack's x or y size");
+    if (nZblend < 0 || (unsigned)nZblend > S.output_nz)
+      fail("RL_interface: nZblend exceeds the stack's z size");
+    const hipStream_t st = S.st;
+    const size_t nin = (size_t)nx * ny * nz;
+    S.u16.ensure(nin * sizeof(uint16_t));
+    CDN_HIP(hipMemcpyAsync(S.u16.p, raw_data, nin * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    unsigned cx = S.output_nx, cy = S.output_ny, cz = S.output_nz;
+    auto vox = [&]() { return (size_t)cx * cy * cz; };
+    S.bufA.ensure(vox() * sizeof(float));
+    float* X = S.bufA.as<float>();
+    u16_to_f32_crop(st, S.u16.as<uint16_t>(), nx, ny, nz, X, (int)cx, (int)cy, (int)cz);
+    // scratch buffer that is not X
+    auto other = [&](size_t bytes) -> float* {
+      DevBuf& b = (X == S.bufA.as<float>()) ? S.bufB : S.bufA;
+      b.ensure(bytes);
+      return b.as<float>();
+    };
+    const bool deskew_on = std::fabs(S.deskewFactor) > 0.0;
+    if (nIters > 0 || (bSaveDeskewedRaw && deskew_on) || S.rot_on) {
+      apodize(st, X, (int)cx, (int)cy, (int)cz, napodize);
+      bgsub(st, X, (long)vox(), background);
+      S.red_part.ensure(sizeof(double) * 2048);
+      S.red_acc.ensure(sizeof(double) * 4);
+      S.red_mean.ensure(sizeof(float));
+      const double inten = mean_above_background(st, X, (long)vox(), S.red_part.as<double>(), S.red_acc.as<double>(),
+                                                 S.red_mean.as<float>());
+      if (bDoRescale) {
+        if (g_first) {
+          g_intensity0 = inten;
+          g_first = false;
+        } else {
+          rescale(st, X, (long)vox(), (float)(g_intensity0 / inten));
+        }
+      }
+      if ((!bSkewedDecon || (bSaveDeskewedRaw && nIters == 0)) && deskew_on) {
+        float* D = other(sizeof(float) * (size_t)S.deskewedXdim * cy * cz);
+        deskew(st, X, (int)cx, (int)cy, (int)cz, S.deskewFactor, D, (int)S.deskewedXdim, extraShift, padVal);
+        X = D;
+        cx = S.deskewedXdim;
+        if (bSaveDeskewedRaw) {
+          if (!raw_deskewed_result) fail("RL_interface: raw_deskewed_result is null");
+          CDN_HIP(hipMemcpyAsync(raw_deskewed_result, X, vox() * sizeof(float), hipMemcpyDeviceToHost, st));
+          if (nIters == 0) {
+            // the reference returns uninitialised memory in `result` here
+            CDN_HIP(hipMemcpyAsync(result, X, vox() * sizeof(float), hipMemcpyDeviceToHost, st));
+            CDN_HIP(hipStreamSynchronize(st));
+            return 1;
+          }
+        }
+      }
+      zblend(st, X, (int)cx, (int)cy, (int)cz, nZblend);
+      if (nIters > 0) {
+        float ks[3];
+        kscales(S, cx, ks);
+        S.eng.setup((int)cx, (int)cy, (int)cz, S.otf, ks, st);
+        S.eng.load_raw(X, true, st);
+        S.eng.iterate(nIters, 0, false, st, nullptr);
+        X = S.eng.result();
+      }
+    }
+    if (bSkewedDecon && deskew_on) {
+      float* D = other(sizeof(float) * (size_t)S.deskewedXdim * cy * cz);
+      deskew(st, X, (int)cx, (int)cy, (int)cz, S.deskewFactor, D, (int)S.deskewedXdim, extraShift, 0.f);
+      X = D;
+      cx = S.deskewedXdim;
+    }
+    if (S.rot_on) {
+      int nxo, nzo;
+      rotated_dims(S.rot, cx, cz, &nxo, &nzo);
+      if (nxo <= 0 || nzo <= 0) fail("RL_interface: rotation gives an empty result");
+      float* R = other(sizeof(float) * (size_t)nxo * cy * nzo);
+      rotate(st, X, (int)cx, (int)cy, (int)cz, S.rot, R, nxo, nzo);
+      X = R;
+      cx = (unsigned)nxo;
+      cz = (unsigned)nzo;
+    }
+    CDN_HIP(hipMemcpyAsync(result, X, vox() * sizeof(float), hipMemcpyDeviceToHost, st));
+    CDN_HIP(hipStreamSynchronize(st));
+    return 1;
+  });
+}
+
+void RL_cleanup(void) {
+  // src/RL-Biggs-Andrews.cpp:707-715
+  g_intensity0 = 0.;
+  g_first = true;
+  g_s.reset();
+}
+
+void cuda_reset(void) {
+  g_s.reset();
+  (void)hipDeviceReset();
+}
+
+int Deskew_interface(const float* const raw_data, int nx, int ny, int nz, float dz, float dr, float deskewAngle,
+                     float* const result, int outputWidth, int extraShift, float padVal) {
+  return guard([&]() -> int {
+    if (!raw_data || !result) fail("Deskew_interface: null buffer");
+    double factor = 0.;
+    unsigned xdim = (unsigned)nx;
+    if (std::fabs(deskewAngle) > 0.0) {
+      if (deskewAngle < 0) deskewAngle += 180.;
+      factor = std::cos(deskewAngle * M_PI / 180.) * dz / dr;
+      if (outputWidth == 0)
+        xdim = (unsigned)(xdim + std::floor(g_output_nz * dz * std::fabs(std::cos(deskewAngle * M_PI / 180.)) / dr) / 4.);
+      else
+        xdim = (unsigned)outputWidth;
+    }
+    hipStream_t st = make_stream();
+    DevBuf din, dout;
+    try {
+      din.ensure(sizeof(float) * (size_t)nx * ny * nz);
+      dout.ensure(sizeof(float) * (size_t)xdim * ny * nz);
+      CDN_HIP(hipMemcpyAsync(din.p, raw_data, sizeof(float) * (size_t)nx * ny * nz, hipMemcpyHostToDevice, st));
+      deskew(st, din.as<float>(), nx, ny, nz, factor, dout.as<float>(), (int)xdim, extraShift, padVal);
+      CDN_HIP(hipMemcpyAsync(result, dout.p, sizeof(float) * (size_t)xdim * ny * nz, hipMemcpyDeviceToHost, st));
+      CDN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    (void)hipStreamDestroy(st);
+    return 1;
+  });
+}
+
+static int affine_common(const float* raw_data, int nx, int ny, int nz, float* result, const float* affMat, int ra,
+                         float dx, float dy, float dz) {
+  return guard([&]() -> int {
+    if (!raw_data || !result || !affMat) fail("Affine_interface: null buffer");
+    const size_t n = (size_t)nx * ny * nz;
+    hipStream_t st = make_stream();
+    DevBuf din, dout;
+    try {
+      din.ensure(sizeof(float) * n);
+      dout.ensure(sizeof(float) * n);
+      CDN_HIP(hipMemcpyAsync(din.p, raw_data, sizeof(float) * n, hipMemcpyHostToDevice, st));
+      affine(st, din.as<float>(), nx, ny, nz, dout.as<float>(), affMat, ra, dx, dy, dz);
+      CDN_HIP(hipMemcpyAsync(result, dout.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+      CDN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    (void)hipStreamDestroy(st);
+    return 1;
+  });
+}
+
+int Affine_interface(const float* const raw_data, int nx, int ny, int nz, float* const result, const float* affMat) {
+  return affine_common(raw_data, nx, ny, nz, result, affMat, 0, 1.f, 1.f, 1.f);
+}
+
+int Affine_interface_RA(const float* const raw_data, int nx, int ny, int nz, float dx, float dy, float dz,
+                        float* const result, const float* affMat) {
+  return affine_common(raw_data, nx, ny, nz, result, affMat, 1, dx, dy, dz);
+}
+
+int camcor_interface_init(int, int, int, const float* const) {
+  g_err = "camcor_interface_init: camera correction is not part of this library";
+  return 0;
+}
+int camcor_interface(const unsigned short* const, int, int, int, unsigned short* const) {
+  g_err = "camcor_interface: camera correction is not part of this library";
+  return 0;
+}
+
+int cudaDecon_device_sync(void) {
+  return guard([]() -> int {
+    CDN_HIP(hipDeviceSynchronize());
+    return 1;
+  });
+}
+
+// ---- standalone FFT / filter -------------------------------------------------
+int cudaDecon_fft_r2c(const float* in, int nx, int ny, int nz, float* out) {
+  return guard([&]() -> int {
+    hipStream_t st = make_stream();
+    Fft3 F;
+    DevBuf dv, ds;
+    try {
+      F.setup(nx, ny, nz, st);
+      dv.ensure(sizeof(float) * F.nvox);
+      ds.ensure(sizeof(float2) * F.nspec);
+      CDN_HIP(hipMemcpyAsync(dv.p, in, sizeof(float) * F.nvox, hipMemcpyHostToDevice, st));
+      F.r2c(st, dv.as<float>(), ds.as<float2>());
+      CDN_HIP(hipMemcpy2DAsync(out, sizeof(float2) * F.nxh, ds.p, sizeof(float2) * F.pitch, sizeof(float2) * F.nxh,
+                               (size_t)ny * nz, hipMemcpyDeviceToHost, st));
+      CDN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    (void)hipStreamDestroy(st);
+    return 1;
+  });
+}
+
+int cudaDecon_fft_c2r(const float* in, int nx, int ny, int nz, float* out) {
+  return guard([&]() -> int {
+    hipStream_t st = make_stream();
+    Fft3 F;
+    DevBuf dv, ds;
+    try {
+      F.setup(nx, ny, nz, st);
+      dv.ensure(sizeof(float) * F.nvox);
+      ds.ensure(sizeof(float2) * F.nspec);
+      CDN_HIP(hipMemcpy2DAsync(ds.p, sizeof(float2) * F.pitch, in, sizeof(float2) * F.nxh, sizeof(float2) * F.nxh,
+                               (size_t)ny * nz, hipMemcpyHostToDevice, st));
+      F.c2r(st, ds.as<float2>(), dv.as<float>());
+      CDN_HIP(hipMemcpyAsync(out, dv.p, sizeof(float) * F.nvox, hipMemcpyDeviceToHost, st));
+      CDN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    (void)hipStreamDestroy(st);
+    return 1;
+  });
+}
+
+int cudaDecon_filter(float* vol, int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
+                     int otf_depth, float kxscale, float kyscale, float kzscale, int conj) {
+  return guard([&]() -> int {
+    hipStream_t st = make_stream();
+    try {
+      Fft3 F;
+      F.setup(nx, ny, nz, st);
+      OtfTable T;
+      T.set_from_image(otf, otf_width, otf_height, otf_depth, 1.f, 1.f);
+      OtfDev D;
+      const float ks[3] = {kxscale, kyscale, kzscale};
+      D.setup(T, ks, nz, st);
+      DevBuf dv, ds;
+      dv.ensure(sizeof(float) * F.nvox);
+      ds.ensure(sizeof(float2) * F.nspec);
+      CDN_HIP(hipMemcpyAsync(dv.p, vol, sizeof(float) * F.nvox, hipMemcpyHostToDevice, st));
+      float2* S = ds.as<float2>();
+      launch_x_r2c(nx, st, dv.as<float>(), nullptr, nullptr, nullptr, XIN_PLAIN, S, (long)ny * nz, F.pitch,
+                   F.twx.as<float2>());
+      F.col(st, S, 1, false);
+      launch_z_filter(nz, st, S, F.nxh, (long)ny * F.pitch, F.pitch, ny, conj, D.params(T, 1.0 / (double)F.nvox),
+                      F.twz.as<float2>());
+      F.col(st, S, 1, true);
+      launch_x_c2r(nx, st, S, dv.as<float>(), (long)ny * nz, F.pitch, F.twx.as<float2>());
+      CDN_HIP(hipMemcpyAsync(vol, dv.p, sizeof(float) * F.nvox, hipMemcpyDeviceToHost, st));
+      CDN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    (void)hipStreamDestroy(st);
+    return 1;
+  });
+}
+
+// ---- RL engine handle -----------------------------------------------------
+cudaDecon_rl* cudaDecon_rl_create(int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
+                                  int otf_depth, float kxscale, float kyscale, float kzscale) {
+  cudaDecon_rl* out = nullptr;
+  guard([&]() -> int {
+    auto h = std::make_unique<cudaDecon_rl>();
+    h->st = make_stream();
+    OtfTable T;
+    T.set_from_image(otf, otf_width, otf_height, otf_depth, 1.f, 1.f);
+    const float ks[3] = {kxscale, kyscale, kzscale};
+    h->eng.setup(nx, ny, nz, T, ks, h->st);
+    out = h.release();
+    return 1;
+  });
+  return out;
+}
+
+int cudaDecon_rl_set_raw(cudaDecon_rl* h, const float* raw) {
+  return guard([&]() -> int {
+    if (!h || !raw) fail("cudaDecon_rl_set_raw: null argument");
+    h->eng.load_raw(raw, false, h->st);
+    CDN_HIP(hipStreamSynchronize(h->st));
+    h->k_next = 0;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_set_seed(cudaDecon_rl* h, const float* seed) {
+  return guard([&]() -> int {
+    if (!h || !seed) fail("cudaDecon_rl_set_seed: null argument");
+    h->eng.load_seed(seed, false, h->st);
+    CDN_HIP(hipStreamSynchronize(h->st));
+    return 1;
+  });
+}
+
+int cudaDecon_rl_iterate(cudaDecon_rl* h, int n, int k0, int seeded, double* lambdas, double* elapsed_ms) {
+  return guard([&]() -> int {
+    if (!h) fail("cudaDecon_rl_iterate: null handle");
+    if (k0 != 0 && k0 != h->k_next) fail("cudaDecon_rl_iterate: k0 must be 0 or continue the previous call");
+    if (k0 == 0) h->eng.reset(h->st);
+    hipEvent_t e0, e1;
+    CDN_HIP(hipEventCreate(&e0));
+    CDN_HIP(hipEventCreate(&e1));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    CDN_HIP(hipEventRecord(e0, h->st));
+    h->eng.iterate(n, k0, seeded != 0, h->st, nullptr);
+    CDN_HIP(hipEventRecord(e1, h->st));
+    CDN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CDN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    if (lambdas) {
+      // the engine's lambda slots of this call: k < 2 carry no factor
+      std::vector<double> hl(4 * (size_t)std::max(n, 1));
+      CDN_HIP(hipMemcpy(hl.data(), h->eng.lam.p, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
+      for (int j = 0; j < n; ++j) lambdas[j] = (k0 + j >= 2) ? hl[4 * (size_t)j] : 0.0;
+    }
+    h->k_next = k0 + n;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_get(cudaDecon_rl* h, float* X) {
+  return guard([&]() -> int {
+    if (!h || !X) fail("cudaDecon_rl_get: null argument");
+    CDN_HIP(hipMemcpyAsync(X, h->eng.result(), sizeof(float) * h->eng.fft.nvox, hipMemcpyDeviceToHost, h->st));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    return 1;
+  });
+}
+
+int cudaDecon_rl_timing(cudaDecon_rl* h, int enable, double* pass_ms, long* pass_calls) {
+  return guard([&]() -> int {
+    if (!h) fail("cudaDecon_rl_timing: null handle");
+    if (pass_ms)
+      for (int i = 0; i < PASS_COUNT; ++i) pass_ms[i] = h->eng.pass_ms[i];
+    if (pass_calls)
+      for (int i = 0; i < PASS_COUNT; ++i) pass_calls[i] = h->eng.pass_calls[i];
+    if (enable >= 0) {
+      const bool on = enable != 0;
+      if (on && !h->eng.timing)
+        for (int i = 0; i < PASS_COUNT; ++i) {
+          h->eng.pass_ms[i] = 0;
+          h->eng.pass_calls[i] = 0;
+        }
+      h->eng.timing = on;
+    }
+    return 1;
+  });
+}
+
+int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pitch) {
+  return guard([&]() -> int {
+    if (!h) fail("cudaDecon_rl_geometry: null handle");
+    if (nvox) *nvox = h->eng.fft.nvox;
+    if (nspec_alg) *nspec_alg = (long)h->eng.fft.nz * h->eng.fft.ny * h->eng.fft.nxh;
+    if (pitch) *pitch = h->eng.fft.pitch;
+    return 1;
+  });
+}
+
+void cudaDecon_rl_destroy(cudaDecon_rl* h) { delete h; }
+
+}  // extern "C"

@@ -1,0 +1,267 @@
+// Host orchestration of the RL-Biggs-Andrews iteration (restates
+// src/RL-Biggs-Andrews.cpp:211-367 as 9 fused passes + a device-side
+// acceleration factor; no per-iteration host synchronisation).
+#include "decon.hpp"
+
+#include "rl_kernels.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace cdn {
+
+void DevBuf::ensure(size_t b) {
+  if (b <= bytes && p) return;
+  release();
+  if (b == 0) return;
+  CDN_HIP(hipMalloc(&p, b));
+  bytes = b;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Fft3
+// ---------------------------------------------------------------------------
+static void upload_twiddles(DevBuf& b, int n, hipStream_t st) {
+  const int cnt = fft_twiddle_count(n);
+  std::vector<float2> h(cnt);
+  fft_twiddles(n, h.data());
+  b.ensure(sizeof(float2) * cnt);
+  CDN_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(float2) * cnt, hipMemcpyHostToDevice, st));
+  CDN_HIP(hipStreamSynchronize(st));
+}
+
+void Fft3::setup(int x, int y, int z, hipStream_t st) {
+  for (int n : {x, y, z})
+    if (!fft_length_supported(n))
+      fail("FFT length " + std::to_string(n) + " is not among the lengths compiled into libcudaDecon");
+  nx = x;
+  ny = y;
+  nz = z;
+  nxh = nx / 2 + 1;
+  pitch = (nxh + 15) / 16 * 16;
+  nvox = (long)nx * ny * nz;
+  nspec = (long)nz * ny * pitch;
+  upload_twiddles(twx, nx, st);
+  upload_twiddles(twy, ny, st);
+  upload_twiddles(twz, nz, st);
+}
+
+void Fft3::col(hipStream_t st, float2* S, int axis, bool inverse) const {
+  if (axis == 1)
+    launch_col_fft(ny, inverse, st, S, nxh, pitch, (long)ny * pitch, nz, twy.as<float2>());
+  else
+    launch_col_fft(nz, inverse, st, S, nxh, (long)ny * pitch, pitch, ny, twz.as<float2>());
+}
+
+void Fft3::r2c(hipStream_t st, const float* in, float2* S) const {
+  launch_x_r2c(nx, st, in, nullptr, nullptr, nullptr, XIN_PLAIN, S, (long)ny * nz, pitch, twx.as<float2>());
+  col(st, S, 1, false);
+  col(st, S, 2, false);
+}
+
+void Fft3::c2r(hipStream_t st, float2* S, float* out) const {
+  col(st, S, 2, true);
+  col(st, S, 1, true);
+  launch_x_c2r(nx, st, S, out, (long)ny * nz, pitch, twx.as<float2>());
+}
+
+// ---------------------------------------------------------------------------
+// OTF
+// ---------------------------------------------------------------------------
+void OtfTable::set_from_image(const float* img, int width, int height, int depth, float dr_psf, float dz_psf) {
+  // determine_OTF_dimensions (src/RL-Biggs-Andrews.cpp:506-529), float math
+  if (depth > 1) {
+    nxotf = width / 2;
+    nyotf = height;
+    nzotf = depth;
+  } else {
+    nxotf = height;
+    nyotf = 1;
+    nzotf = width / 2;
+  }
+  if (nxotf < 2 || nzotf < 1) fail("OTF table is too small");
+  dkx = 1.0f / ((float)((nxotf - 1) * 2) * dr_psf);
+  dky = (nyotf > 1) ? 1.0f / ((float)nyotf * dr_psf) : dkx;
+  dkz = 1.0f / ((float)nzotf * dz_psf);
+  data.assign(img, img + (size_t)2 * nxotf * nyotf * nzotf);
+}
+
+void OtfDev::setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st) {
+  ra = (t.nyotf == 1);
+  ks[0] = kscale[0];
+  ks[1] = kscale[1];
+  ks[2] = kscale[2];
+  nz_for = nz;
+  if (ra) {
+    // z-interpolated radial table: tz[r][kz] = T[r][iz](1-az) + T[r][iz1] az
+    // for the grid's signed kz, as in dev_otfinterpolate's RA branch
+    // (src/RLgpuImpl.cu:234-272); zero where kzindex is outside the table.
+    const int nr = t.nxotf, nzo = t.nzotf;
+    std::vector<float2> h((size_t)nr * nz);
+    for (int kz = 0; kz < nz; ++kz) {
+      const int kzi = kz > (nz >> 1) ? kz - nz : kz;
+      const float kzs = (float)kzi * ks[2];
+      const float kzindex = kzs < 0.f ? kzs + (float)nzo : kzs;
+      const bool ok = kzindex < (float)nzo && kzindex >= 0.f;
+      const int iz = ok ? (int)std::floor(kzindex) : 0;
+      const float az = kzindex - (float)iz;
+      const int iz1 = (iz == nzo - 1) ? 0 : iz + 1;
+      for (int r = 0; r < nr; ++r) {
+        float2 v = make_float2(0.f, 0.f);
+        if (ok) {
+          const float* a = &t.data[2 * ((size_t)r * nzo + iz)];
+          const float* b = &t.data[2 * ((size_t)r * nzo + iz1)];
+          v.x = a[0] * (1 - az) + b[0] * az;
+          v.y = a[1] * (1 - az) + b[1] * az;
+        }
+        h[(size_t)r * nz + kz] = v;
+      }
+    }
+    tz.ensure(sizeof(float2) * h.size());
+    CDN_HIP(hipMemcpyAsync(tz.p, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, st));
+  } else {
+    raw.ensure(sizeof(float) * t.data.size());
+    CDN_HIP(hipMemcpyAsync(raw.p, t.data.data(), sizeof(float) * t.data.size(), hipMemcpyHostToDevice, st));
+  }
+  CDN_HIP(hipStreamSynchronize(st));
+}
+
+OtfParams OtfDev::params(const OtfTable& t, double inv_n) const {
+  OtfParams p{};
+  p.ra = ra;
+  p.ra_tab.tz = tz.as<float2>();
+  p.ra_tab.nr = t.nxotf;
+  p.tab3.raw = raw.as<float2>();
+  p.tab3.nxotf = t.nxotf;
+  p.tab3.nyotf = t.nyotf;
+  p.tab3.nzotf = t.nzotf;
+  p.kxscale = ks[0];
+  p.kyscale = ks[1];
+  p.kzscale = ks[2];
+  p.inv_n = (float)inv_n;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// RL engine
+// ---------------------------------------------------------------------------
+void RLEngine::setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st) {
+  if (!fft.matches(nx, ny, nz)) fft.setup(nx, ny, nz, st);
+  table = t;
+  otf.setup(table, kscale, nz, st);
+  const size_t vb = sizeof(float) * (size_t)fft.nvox;
+  raw.ensure(vb);
+  xa.ensure(vb);
+  xb.ensure(vb);
+  y.ensure(vb);
+  g.ensure(vb);
+  spec.ensure(sizeof(float2) * (size_t)fft.nspec);
+  nb_part = x_update_blocks(nx, (long)ny * nz);
+  part.ensure(sizeof(double) * 2 * (size_t)nb_part);
+  cur = xa.as<float>();
+  prev = xb.as<float>();
+}
+
+void RLEngine::load_raw(const float* in, bool on_device, hipStream_t st) {
+  const size_t vb = sizeof(float) * (size_t)fft.nvox;
+  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  CDN_HIP(hipMemcpyAsync(raw.p, in, vb, k, st));
+  CDN_HIP(hipMemcpyAsync(xa.p, raw.p, vb, hipMemcpyDeviceToDevice, st));
+  cur = xa.as<float>();
+  prev = xb.as<float>();
+}
+
+void RLEngine::reset(hipStream_t st) {
+  CDN_HIP(hipMemcpyAsync(xa.p, raw.p, sizeof(float) * (size_t)fft.nvox, hipMemcpyDeviceToDevice, st));
+  cur = xa.as<float>();
+  prev = xb.as<float>();
+}
+
+void RLEngine::load_seed(const float* in, bool on_device, hipStream_t st) {
+  const size_t vb = sizeof(float) * (size_t)fft.nvox;
+  CDN_HIP(hipMemcpyAsync(y.p, in, vb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+}
+
+void RLEngine::iterate(int n, int k0, bool seeded, hipStream_t st, double* lambdas) {
+  if (n <= 0) return;
+  const long rows = (long)fft.ny * fft.nz;
+  const float eps = FLT_EPSILON;  // src/RL-Biggs-Andrews.cpp:274
+  const OtfParams op = otf.params(table, 1.0 / (double)fft.nvox);
+  lam.ensure(sizeof(double) * 4 * (size_t)n);
+  float2* S = spec.as<float2>();
+  const float2* twx = fft.twx.as<float2>();
+  const float2* twz = fft.twz.as<float2>();
+  double* lamd = lam.as<double>();
+
+  std::vector<hipEvent_t> ev;
+  std::vector<int> evpass;
+  auto mark = [&](int pass) {
+    if (!timing) return;
+    hipEvent_t e;
+    CDN_HIP(hipEventCreate(&e));
+    CDN_HIP(hipEventRecord(e, st));
+    ev.push_back(e);
+    evpass.push_back(pass);
+  };
+  mark(-1);
+  for (int j = 0; j < n; ++j) {
+    const int k = k0 + j;
+    double* lamk = lamd + 4 * j;
+    int mode;
+    const float* yv;
+    if (k >= 2) {
+      accel_factor(st, part.as<double>(), nb_part, (double)eps, lamk);
+      mark(PASS_LAMBDA);
+      mode = XIN_PRED;
+      yv = y.as<float>();
+    } else if (k == 0 && seeded) {
+      mode = XIN_SEED;
+      yv = y.as<float>();
+    } else {
+      mode = XIN_PLAIN;
+      yv = cur;
+    }
+    launch_x_r2c(fft.nx, st, cur, prev, y.as<float>(), lamk, mode, S, rows, fft.pitch, twx);
+    mark(PASS_XR2C);
+    for (int f = 0; f < 2; ++f) {
+      fft.col(st, S, 1, false);
+      mark(PASS_YFWD);
+      launch_z_filter(fft.nz, st, S, fft.nxh, (long)fft.ny * fft.pitch, fft.pitch, fft.ny, f, op, twz);
+      mark(PASS_ZFILT);
+      fft.col(st, S, 1, true);
+      mark(PASS_YINV);
+      if (f == 0) {
+        launch_x_lrcore(fft.nx, st, S, raw.as<float>(), rows, fft.pitch, twx, eps);
+        mark(PASS_XLR);
+      }
+    }
+    launch_x_update(fft.nx, st, S, yv, prev, g.as<float>(), part.as<double>(), k >= 1, rows, fft.pitch, twx);
+    mark(PASS_XUPD);
+    std::swap(cur, prev);
+  }
+  if (timing || lambdas) CDN_HIP(hipStreamSynchronize(st));
+  if (timing) {
+    for (size_t i = 1; i < ev.size(); ++i) {
+      float ms = 0.f;
+      CDN_HIP(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+      pass_ms[evpass[i]] += ms;
+      pass_calls[evpass[i]] += 1;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  if (lambdas) {
+    std::vector<double> h(4 * (size_t)n);
+    CDN_HIP(hipMemcpy(h.data(), lam.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; ++j) lambdas[j] = (k0 + j >= 2) ? h[4 * (size_t)j] : 0.0;
+  }
+}
+
+}  // namespace cdn

@@ -1,0 +1,89 @@
+// Host orchestration of the RL-Biggs-Andrews path on one GPU.
+#pragma once
+
+#include "common.hpp"
+#include "fft_kernels.hpp"
+
+#include <string>
+#include <vector>
+
+namespace cdn {
+
+// Owned device allocation that only grows (kept across calls; freed by the
+// owner's destructor or release()).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void ensure(size_t b);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// 3-D R2C / C2R plan for one (nx, ny, nz) grid.
+struct Fft3 {
+  int nx = 0, ny = 0, nz = 0, nxh = 0;
+  long pitch = 0;          // half-spectrum row pitch (complex elements)
+  long nvox = 0;           // nx ny nz
+  long nspec = 0;          // nz ny pitch (allocated complex elements)
+  DevBuf twx, twy, twz;
+  bool matches(int x, int y, int z) const { return nx == x && ny == y && nz == z; }
+  void setup(int x, int y, int z, hipStream_t st);
+  void r2c(hipStream_t st, const float* in, float2* S) const;   // unnormalised
+  void c2r(hipStream_t st, float2* S, float* out) const;        // unnormalised, consumes S
+  void col(hipStream_t st, float2* S, int axis, bool inverse) const;  // axis 1 = y, 2 = z
+};
+
+// Raw OTF as loaded (complex64 interleaved) and determine_OTF_dimensions
+// (src/RL-Biggs-Andrews.cpp:506-529).
+struct OtfTable {
+  std::vector<float> data;
+  int nxotf = 0, nyotf = 0, nzotf = 0;
+  float dkx = 0, dky = 0, dkz = 0;
+  void set_from_image(const float* img, int width, int height, int depth, float dr_psf, float dz_psf);
+};
+
+struct OtfDev {
+  int ra = 1, nz_for = -1;
+  float ks[3] = {0, 0, 0};
+  DevBuf raw, tz;
+  void setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st);
+  OtfParams params(const OtfTable& t, double inv_n) const;
+};
+
+enum Pass { PASS_XR2C = 0, PASS_YFWD, PASS_ZFILT, PASS_YINV, PASS_XLR, PASS_XUPD, PASS_LAMBDA, PASS_COUNT };
+
+// RL-BA engine: all state resident in HBM; the acceleration factor stays on
+// the device so an iteration issues no host synchronisation.
+struct RLEngine {
+  Fft3 fft;
+  OtfDev otf;
+  OtfTable table;
+  DevBuf raw, xa, xb, y, g, spec, part, lam;
+  float* cur = nullptr;   // X_k after the last iteration
+  float* prev = nullptr;  // X_{k-1}
+  int nb_part = 0;
+  // per-pass timing (ms, summed) when enabled
+  bool timing = false;
+  double pass_ms[PASS_COUNT] = {};
+  long pass_calls[PASS_COUNT] = {};
+
+  void setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st);
+  // Stage the (preprocessed) stack: R := in, X_0 := in.  `in` is a device
+  // pointer when on_device, else host.
+  void load_raw(const float* in, bool on_device, hipStream_t st);
+  void load_seed(const float* in, bool on_device, hipStream_t st);
+  // X_0 := R again (restart from the staged stack)
+  void reset(hipStream_t st);
+  // Run iterations k = k0 .. k0+n-1 continuing the resident state (k0 > 0
+  // requires the previous call to have ended at k0-1); seeded => Y_0 from the
+  // seed buffer.  lambdas (optional, host, n entries) receives the factor
+  // used at each k (0 for k < 2).
+  void iterate(int n, int k0, bool seeded, hipStream_t st, double* lambdas);
+  float* result() const { return cur; }
+};
+
+}  // namespace cdn

@@ -1,0 +1,255 @@
+// Device FFT core: radix-2/3/4/5/7/8/16 butterflies and an LDS-resident
+// Stockham autosort transform, fully specialised at compile time on the
+// length N (all 2,3,5,7-smooth lengths the library instantiates).
+//
+// Convention: forward X[k] = sum_n x[n] exp(-2 pi i n k / N); inverse uses
+// exp(+...) and is unnormalised (the 1/N of the reference's filterGPU,
+// src/RLgpuImpl.cu:176-177, is folded into the OTF multiply).
+//
+// Stockham stage s with radix R and Lp = product of earlier radices, for each
+// butterfly j in [0, N/R), k = j mod Lp:
+//   y_t = a[j + t N/R] * exp(-+2 pi i t k / (Lp R)),  Y_q = DFT_R(y)_q,
+//   b[(j - k) R + k + q Lp] = Y_q.
+// Reads and writes of one stage are separated by a workgroup barrier, so one
+// LDS buffer is used in place; each thread keeps its butterfly outputs in
+// registers across the barrier.
+#pragma once
+
+#include "common.hpp"
+
+namespace cdn {
+
+// ---------------------------------------------------------------------------
+// In-register DFTs.  INV selects the sign of the exponent (+ for inverse).
+// ---------------------------------------------------------------------------
+template <int R, bool INV> struct Dft;
+
+template <bool INV> struct Dft<2, INV> {
+  static __device__ __forceinline__ void run(float2* x) {
+    float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  }
+};
+
+template <bool INV> struct Dft<4, INV> {
+  static __device__ __forceinline__ void run(float2* x) {
+    float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+    float2 c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+    // forward: X1 = b - i d, X3 = b + i d
+    float2 jd = INV ? cmuli<1>(d) : cmuli<-1>(d);
+    x[0] = cadd(a, c);
+    x[2] = csub(a, c);
+    x[1] = cadd(b, jd);
+    x[3] = csub(b, jd);
+  }
+};
+
+template <bool INV> struct Dft<8, INV> {
+  static __device__ __forceinline__ void run(float2* x) {
+    constexpr float h = 0.7071067811865476f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      a[t] = cadd(x[t], x[t + 4]);
+      b[t] = csub(x[t], x[t + 4]);
+    }
+    // b_t *= W8^{+-t}
+    if (!INV) {
+      b[1] = make_float2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+      b[2] = cmuli<-1>(b[2]);
+      b[3] = make_float2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
+    } else {
+      b[1] = make_float2(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y));
+      b[2] = cmuli<1>(b[2]);
+      b[3] = make_float2(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y));
+    }
+    Dft<4, INV>::run(a);
+    Dft<4, INV>::run(b);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      x[2 * q] = a[q];
+      x[2 * q + 1] = b[q];
+    }
+  }
+};
+
+template <bool INV> struct Dft<16, INV> {
+  static __device__ __forceinline__ void run(float2* x) {
+    // 16 = 2 x 8, decimation in frequency: a_t = x_t + x_{t+8}, b_t = (x_t - x_{t+8}) W16^t
+    constexpr float c1 = 0.9238795325112867f, s1 = 0.3826834323650898f, h = 0.7071067811865476f;
+    const float tw[8][2] = {{1.f, 0.f}, {c1, s1}, {h, h}, {s1, c1}, {0.f, 1.f}, {-s1, c1}, {-h, h}, {-c1, s1}};
+    float2 a[8], b[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      a[t] = cadd(x[t], x[t + 8]);
+      float2 d = csub(x[t], x[t + 8]);
+      // W16^t = cos - i sin (forward), cos + i sin (inverse)
+      float2 w = make_float2(tw[t][0], INV ? tw[t][1] : -tw[t][1]);
+      b[t] = (t == 0) ? d : cmul(d, w);
+    }
+    Dft<8, INV>::run(a);
+    Dft<8, INV>::run(b);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      x[2 * q] = a[q];
+      x[2 * q + 1] = b[q];
+    }
+  }
+};
+
+// Odd prime R: X_q = x0 + sum_t a_t cos(2pi tq/R) -+ i sum_t b_t sin(2pi tq/R),
+// a_t = x_t + x_{R-t}, b_t = x_t - x_{R-t}, t = 1..(R-1)/2.
+template <int R> struct OddConst;
+template <> struct OddConst<3> {
+  static constexpr float c[1] = {-0.5f};
+  static constexpr float s[1] = {0.8660254037844387f};
+};
+template <> struct OddConst<5> {
+  static constexpr float c[2] = {0.30901699437494745f, -0.8090169943749473f};
+  static constexpr float s[2] = {0.9510565162951535f, 0.5877852522924732f};
+};
+template <> struct OddConst<7> {
+  static constexpr float c[3] = {0.6234898018587336f, -0.22252093395631434f, -0.900968867902419f};
+  static constexpr float s[3] = {0.7818314824680298f, 0.9749279121818236f, 0.43388373911755823f};
+};
+
+template <int R, bool INV> struct DftOdd {
+  static __device__ __forceinline__ void run(float2* x) {
+    constexpr int H = (R - 1) / 2;
+    float2 a[H], b[H];
+#pragma unroll
+    for (int t = 1; t <= H; ++t) {
+      a[t - 1] = cadd(x[t], x[R - t]);
+      b[t - 1] = csub(x[t], x[R - t]);
+    }
+    float2 x0 = x[0];
+    float2 s0 = x0;
+#pragma unroll
+    for (int t = 0; t < H; ++t) s0 = cadd(s0, a[t]);
+    x[0] = s0;
+#pragma unroll
+    for (int q = 1; q <= H; ++q) {
+      float2 re = x0, im = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int t = 1; t <= H; ++t) {
+        // cos(2 pi t q / R) and sin(2 pi t q / R) via symmetric index m = tq mod R
+        const int m = (t * q) % R;
+        const int mm = m <= H ? m : R - m;  // cos index
+        const float cs = OddConst<R>::c[mm - 1];
+        const float sn = m <= H ? OddConst<R>::s[mm - 1] : -OddConst<R>::s[mm - 1];
+        re = make_float2(re.x + cs * a[t - 1].x, re.y + cs * a[t - 1].y);
+        im = make_float2(im.x + sn * b[t - 1].x, im.y + sn * b[t - 1].y);
+      }
+      // forward: X_q = re - i im, X_{R-q} = re + i im
+      float2 iim = cmuli<1>(im);
+      if (!INV) {
+        x[q] = csub(re, iim);
+        x[R - q] = cadd(re, iim);
+      } else {
+        x[q] = cadd(re, iim);
+        x[R - q] = csub(re, iim);
+      }
+    }
+  }
+};
+template <bool INV> struct Dft<3, INV> : DftOdd<3, INV> {};
+template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
+template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
+
+// ---------------------------------------------------------------------------
+// Compile-time radix plan.
+// ---------------------------------------------------------------------------
+struct Plan {
+  int ns = 0;
+  int r[24] = {};
+  int lp[24] = {};
+  int twoff[24] = {};
+  int twsize = 0;
+  int lcm = 1;
+  bool ok = false;
+};
+
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
+constexpr Plan make_plan(int N) {
+  Plan P{};
+  int n = N;
+  int ns = 0;
+  // radix 16 while it leaves at least a factor 8 (keeps radices large)
+  while (n % 16 == 0 && (n / 16) % 8 == 0) { P.r[ns++] = 16; n /= 16; }
+  while (n % 8 == 0) { P.r[ns++] = 8; n /= 8; }
+  if (n % 4 == 0) { P.r[ns++] = 4; n /= 4; }
+  if (n % 2 == 0) { P.r[ns++] = 2; n /= 2; }
+  while (n % 3 == 0) { P.r[ns++] = 3; n /= 3; }
+  while (n % 5 == 0) { P.r[ns++] = 5; n /= 5; }
+  while (n % 7 == 0) { P.r[ns++] = 7; n /= 7; }
+  P.ns = ns;
+  P.ok = (n == 1);
+  int lp = 1, off = 0, l = 1;
+  for (int s = 0; s < ns; ++s) {
+    P.lp[s] = lp;
+    P.twoff[s] = off;
+    if (lp > 1) off += lp * (P.r[s] - 1);
+    lp *= P.r[s];
+    l = l / gcd_c(l, P.r[s]) * P.r[s];
+  }
+  P.twsize = off;
+  P.lcm = l;
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// One Stockham stage over a line held in LDS at element stride SE.
+// Threads p in [0, TPT) of the line share the N/R butterflies cyclically.
+// tw: this stage's twiddles, tw[k (R-1) + t - 1] = exp(-2 pi i t k / (Lp R)).
+// ---------------------------------------------------------------------------
+template <int N, int R, int LP, int TPT, int SE, bool INV>
+__device__ __forceinline__ void stockham_stage(float2* line, int p, const float2* __restrict__ tw) {
+  constexpr int B = N / R;
+  constexpr int U = (B + TPT - 1) / TPT;
+  constexpr bool EXACT = (B % TPT) == 0;
+  float2 v[U][R];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = p + u * TPT;
+    if (EXACT || j < B) {
+      float2 x[R];
+#pragma unroll
+      for (int t = 0; t < R; ++t) x[t] = line[(j + t * B) * SE];
+      if constexpr (LP > 1) {
+        const int k = j % LP;
+        const float2* w = tw + k * (R - 1);
+#pragma unroll
+        for (int t = 1; t < R; ++t) x[t] = INV ? cmulc(x[t], w[t - 1]) : cmul(x[t], w[t - 1]);
+      }
+      Dft<R, INV>::run(x);
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[u][q] = x[q];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = p + u * TPT;
+    if (EXACT || j < B) {
+      const int k = j % LP;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) line[(base + q * LP) * SE] = v[u][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int TPT, int SE, bool INV, int S = 0>
+__device__ __forceinline__ void fft_lds(float2* line, int p, const float2* __restrict__ tw) {
+  constexpr Plan P = make_plan(N);
+  static_assert(P.ok, "FFT length must be 2,3,5,7-smooth");
+  if constexpr (S < P.ns) {
+    stockham_stage<N, P.r[S], P.lp[S], TPT, SE, INV>(line, p, tw + P.twoff[S]);
+    fft_lds<N, TPT, SE, INV, S + 1>(line, p, tw);
+  }
+}
+
+}  // namespace cdn

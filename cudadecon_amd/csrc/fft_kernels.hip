@@ -1,0 +1,616 @@
+// 3-D FFT pass kernels for the RL-Biggs-Andrews iteration on gfx950.
+//
+// One RL-BA iteration = 9 streaming passes over HBM:
+//   x-R2C (+prediction)  y-fwd  z-fwd.H.z-inv  y-inv  x-C2R.LRcore.x-R2C
+//   y-fwd  z-fwd.conjH.z-inv  y-inv  x-C2R.update.dots
+// replacing the reference's cufftExecR2C/C2R calls (src/RLgpuImpl.cu:183,218)
+// and the elementwise kernels at src/RLgpuImpl.cu:331-350,383-508,571-666.
+//
+// x passes: two real rows are packed as z = a + i b, transformed as one
+// complex line in LDS, and split into two half spectra (and the reverse for
+// C2R, where the imaginary parts of the kx = 0 and kx = nx/2 bins are
+// ignored, i.e. numpy irfft semantics).
+// y/z passes: a workgroup owns a tile of L adjacent kx columns of one slab,
+// held in LDS as [element][column] so global loads are L*8 bytes contiguous.
+#include "fft_core.hpp"
+#include "fft_kernels.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace cdn {
+
+// ---------------------------------------------------------------------------
+// Per-length launch configuration
+// ---------------------------------------------------------------------------
+// threads per line: largest divisor of N not above N/16 (about 16 elements
+// per thread)
+constexpr int tpt_for(int N) {
+  int t = N / 16;
+  if (t < 1) t = 1;
+  for (int d = t; d >= 1; --d)
+    if (N % d == 0) return d;
+  return 1;
+}
+
+// columns per tile for y/z passes: fill the LDS budget, at most 64 columns
+// and 1024 threads, power of two
+constexpr int lc_for(int N, int TPT) {
+  const int budget = (N >= 768) ? 131072 : 65536;
+  int L = budget / (N * 8);
+  if (L > 64) L = 64;
+  while (L > 1 && L * TPT > 1024) L /= 2;
+  int p = 1;
+  while (p * 2 <= L) p *= 2;
+  return p;
+}
+
+template <int N> struct XCfg {
+  static constexpr int TPT = tpt_for(N);
+  static constexpr int NL0 = (256 / TPT) < 1 ? 1 : (256 / TPT);
+  static constexpr int NL = (NL0 * N * 8 > 65536) ? ((65536 / (N * 8)) < 1 ? 1 : 65536 / (N * 8)) : NL0;
+  static constexpr int PAD = TPT < 64 ? TPT : 0;  // spreads lines over LDS banks
+  static constexpr int LS = N + PAD;
+  static constexpr int THREADS = TPT * NL;
+  static constexpr int E = (N + TPT - 1) / TPT;      // elements per thread
+  static constexpr int EH = (N / 2 + TPT) / TPT;     // half-spectrum bins per thread
+};
+
+template <int N> struct CCfg {
+  static constexpr int TPT = tpt_for(N);
+  static constexpr int L = lc_for(N, TPT);
+  static constexpr int THREADS = TPT * L;
+  static constexpr int E = (N + TPT - 1) / TPT;
+};
+
+// ---------------------------------------------------------------------------
+// x passes
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float load_y(int mode, const float* __restrict__ X, const float* __restrict__ Xp,
+                                        float* __restrict__ Y, float lam, long idx) {
+  if (mode == XIN_PRED) {
+    const float x = X[idx], xp = Xp[idx];
+    float y = x + lam * (x - xp);
+    y = y > 0.f ? y : 0.f;
+    Y[idx] = y;
+    return y;
+  }
+  if (mode == XIN_SEED) return Y[idx];
+  return X[idx];
+}
+
+// Half spectrum of two real rows from their packed transform in `line`.
+template <int N, int TPT, int EH>
+__device__ __forceinline__ void split_store(const float2* line, int p, float2* __restrict__ S, long r0, long pitch,
+                                            bool v0, bool v1) {
+#pragma unroll
+  for (int e = 0; e < EH; ++e) {
+    const int k = p + e * TPT;
+    if (k <= N / 2) {
+      const float2 zk = line[k];
+      const float2 zn = line[k == 0 ? 0 : N - k];
+      const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+      const float2 D = make_float2(0.5f * (zk.x - zn.x), 0.5f * (zk.y + zn.y));
+      if (v0) S[r0 * pitch + k] = A;
+      if (v1) S[(r0 + 1) * pitch + k] = make_float2(D.y, -D.x);
+    }
+  }
+}
+
+// Packed line z = a + i b from the half spectra of rows r0, r0+1.
+template <int N, int TPT, int EH>
+__device__ __forceinline__ void merge_load(float2* line, int p, const float2* __restrict__ S, long r0, long pitch,
+                                           bool v0, bool v1) {
+#pragma unroll
+  for (int e = 0; e < EH; ++e) {
+    const int k = p + e * TPT;
+    if (k <= N / 2) {
+      float2 A = v0 ? S[r0 * pitch + k] : make_float2(0.f, 0.f);
+      float2 B = v1 ? S[(r0 + 1) * pitch + k] : make_float2(0.f, 0.f);
+      const bool edge = (k == 0) || (2 * k == N);
+      if (edge) {
+        A.y = 0.f;
+        B.y = 0.f;
+      }
+      line[k] = make_float2(A.x - B.y, A.y + B.x);
+      if (!edge) line[N - k] = make_float2(A.x + B.y, B.x - A.y);
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(XCfg<N>::THREADS)
+    k_x_r2c(const float* __restrict__ X, const float* __restrict__ Xp, float* __restrict__ Y,
+            const double* __restrict__ lam, int mode, float2* __restrict__ S, long rows, long pitch,
+            const float2* __restrict__ tw) {
+  using C = XCfg<N>;
+  __shared__ float2 lds[C::NL * C::LS];
+  const int tid = threadIdx.x;
+  const int l = tid / C::TPT, p = tid % C::TPT;
+  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
+  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
+  float2* line = lds + l * C::LS;
+  const float lamf = (mode == XIN_PRED) ? (float)lam[0] : 0.f;
+#pragma unroll
+  for (int e = 0; e < C::E; ++e) {
+    const int i = p + e * C::TPT;
+    if (i < N) {
+      const float a = v0 ? load_y(mode, X, Xp, Y, lamf, r0 * N + i) : 0.f;
+      const float b = v1 ? load_y(mode, X, Xp, Y, lamf, (r0 + 1) * N + i) : 0.f;
+      line[i] = make_float2(a, b);
+    }
+  }
+  __syncthreads();
+  fft_lds<N, C::TPT, 1, false>(line, p, tw);
+  split_store<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
+}
+
+template <int N>
+__global__ void __launch_bounds__(XCfg<N>::THREADS)
+    k_x_c2r(const float2* __restrict__ S, float* __restrict__ out, long rows, long pitch,
+            const float2* __restrict__ tw) {
+  using C = XCfg<N>;
+  __shared__ float2 lds[C::NL * C::LS];
+  const int tid = threadIdx.x;
+  const int l = tid / C::TPT, p = tid % C::TPT;
+  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
+  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
+  float2* line = lds + l * C::LS;
+  merge_load<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
+  __syncthreads();
+  fft_lds<N, C::TPT, 1, true>(line, p, tw);
+#pragma unroll
+  for (int e = 0; e < C::E; ++e) {
+    const int i = p + e * C::TPT;
+    if (i < N) {
+      const float2 z = line[i];
+      if (v0) out[r0 * N + i] = z.x;
+      if (v1) out[(r0 + 1) * N + i] = z.y;
+    }
+  }
+}
+
+// LRcore_kernel (src/RLgpuImpl.cu:416-434)
+__device__ __forceinline__ float lrcore(float c, float raw, float eps) {
+  c = fabsf(c) > 0.f ? c : eps;
+  c = raw / c + eps;
+  c = c > 10.f ? 10.f : c;
+  c = c < -10.f ? -10.f : c;
+  return c;
+}
+
+template <int N>
+__global__ void __launch_bounds__(XCfg<N>::THREADS)
+    k_x_lrcore(float2* __restrict__ S, const float* __restrict__ raw, long rows, long pitch,
+               const float2* __restrict__ tw, float eps) {
+  using C = XCfg<N>;
+  __shared__ float2 lds[C::NL * C::LS];
+  const int tid = threadIdx.x;
+  const int l = tid / C::TPT, p = tid % C::TPT;
+  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
+  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
+  float2* line = lds + l * C::LS;
+  merge_load<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
+  __syncthreads();
+  fft_lds<N, C::TPT, 1, true>(line, p, tw);
+#pragma unroll
+  for (int e = 0; e < C::E; ++e) {
+    const int i = p + e * C::TPT;
+    if (i < N) {
+      const float2 z = line[i];
+      const float ra = v0 ? raw[r0 * N + i] : 0.f;
+      const float rb = v1 ? raw[(r0 + 1) * N + i] : 0.f;
+      line[i] = make_float2(lrcore(z.x, ra, eps), lrcore(z.y, rb, eps));
+    }
+  }
+  __syncthreads();
+  fft_lds<N, C::TPT, 1, false>(line, p, tw);
+  split_store<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
+}
+
+// Deterministic workgroup sum of two doubles; out[0], out[1] written by
+// thread 0.  Wavefront shuffles only when every wavefront is full.
+template <int THREADS>
+__device__ __forceinline__ void block_sum2(double a, double b, double (*red)[THREADS], double* out) {
+  const int tid = threadIdx.x;
+  if constexpr (THREADS % 64 == 0) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((tid & 63) == 0) {
+      red[0][tid >> 6] = a;
+      red[1][tid >> 6] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double sa = 0.0, sb = 0.0;
+      for (int w = 0; w < THREADS / 64; ++w) {
+        sa += red[0][w];
+        sb += red[1][w];
+      }
+      out[0] = sa;
+      out[1] = sb;
+    }
+  } else {
+    red[0][tid] = a;
+    red[1][tid] = b;
+    __syncthreads();
+    if (tid == 0) {
+      double sa = 0.0, sb = 0.0;
+      for (int w = 0; w < THREADS; ++w) {
+        sa += red[0][w];
+        sb += red[1][w];
+      }
+      out[0] = sa;
+      out[1] = sb;
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(XCfg<N>::THREADS)
+    k_x_update(const float2* __restrict__ S, const float* __restrict__ Y, float* __restrict__ Xout,
+               float* G, double* __restrict__ part, int dots, long rows, long pitch,
+               const float2* __restrict__ tw) {
+  using C = XCfg<N>;
+  __shared__ float2 lds[C::NL * C::LS];
+  __shared__ double red[2][C::THREADS];
+  const int tid = threadIdx.x;
+  const int l = tid / C::TPT, p = tid % C::TPT;
+  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
+  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
+  float2* line = lds + l * C::LS;
+  merge_load<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
+  __syncthreads();
+  fft_lds<N, C::TPT, 1, true>(line, p, tw);
+  double num = 0.0, den = 0.0;
+#pragma unroll
+  for (int e = 0; e < C::E; ++e) {
+    const int i = p + e * C::TPT;
+    if (i < N) {
+      const float2 z = line[i];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const bool v = h ? v1 : v0;
+        if (v) {
+          const long idx = (r0 + h) * N + i;
+          const float c = h ? z.y : z.x;
+          const float y = Y[idx];
+          float x = c * y;  // currEstimate_kernel (src/RLgpuImpl.cu:462-470)
+          x = x > 0.f ? x : 0.f;
+          const float g = x - y;  // currPrevDiff_kernel (src/RLgpuImpl.cu:501-508)
+          if (dots) {  // innerProduct_kernel: FP32 products, FP64 sums (:583-592)
+            const float gp = G[idx];  // G_{k-1}, overwritten below by G_k
+            num += (double)(g * gp);
+            den += (double)(gp * gp);
+          }
+          Xout[idx] = x;
+          G[idx] = g;
+        }
+      }
+    }
+  }
+  if (dots) block_sum2(num, den, red, part + 2 * (long)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// y / z column passes
+// ---------------------------------------------------------------------------
+template <int N, bool INV>
+__global__ void __launch_bounds__(CCfg<N>::THREADS)
+    k_col(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles,
+          const float2* __restrict__ tw) {
+  using C = CCfg<N>;
+  __shared__ float2 lds[N * C::L];
+  const int tid = threadIdx.x;
+  const int l = tid % C::L, p = tid / C::L;
+  const long outer = blockIdx.x / ntiles, tile = blockIdx.x - outer * ntiles;
+  const long c = tile * C::L + l;
+  const bool valid = c < ncols;
+  float2* base = S + outer * outer_stride + c;
+  float2* line = lds + l;
+#pragma unroll
+  for (int e = 0; e < C::E; ++e) {
+    const int i = p + e * C::TPT;
+    if (i < N) line[i * C::L] = valid ? base[i * line_stride] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<N, C::TPT, C::L, INV>(line, p, tw);
+  if (valid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+      const int i = p + e * C::TPT;
+      if (i < N) base[i * line_stride] = line[i * C::L];
+    }
+  }
+}
+
+// filter_kernel / filterConj_kernel (src/RLgpuImpl.cu:331-350) with the OTF
+// value computed on the fly as in dev_otfinterpolate (src/RLgpuImpl.cu:228-329)
+template <int N>
+__global__ void __launch_bounds__(CCfg<N>::THREADS)
+    k_z_filter(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles, long ny,
+               int conj, OtfParams otf, const float2* __restrict__ tw) {
+  using C = CCfg<N>;
+  __shared__ float2 lds[N * C::L];
+  const int tid = threadIdx.x;
+  const int l = tid % C::L, p = tid / C::L;
+  const long outer = blockIdx.x / ntiles, tile = blockIdx.x - outer * ntiles;
+  const long c = tile * C::L + l;
+  const bool valid = c < ncols;
+  float2* base = S + outer * outer_stride + c;
+  float2* line = lds + l;
+#pragma unroll
+  for (int e = 0; e < C::E; ++e) {
+    const int i = p + e * C::TPT;
+    if (i < N) line[i * C::L] = valid ? base[i * line_stride] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<N, C::TPT, C::L, false>(line, p, tw);
+
+  // signed ky (k > n>>1 -> k - n), scaled k coordinates
+  const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
+  const float kx = (float)c * otf.kxscale;
+  const float ky = (float)kyi * otf.kyscale;
+  const float sg = conj ? -1.f : 1.f;
+  if (otf.ra) {
+    const float kr = sqrtf(kx * kx + ky * ky);
+    const bool inside = kr < (float)(otf.ra_tab.nr - 1);
+    const int ir = inside ? (int)floorf(kr) : 0;
+    const float ar = kr - (float)ir;
+    const float2* t0 = otf.ra_tab.tz + (long)ir * N;
+    const float2* t1 = t0 + N;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+      const int i = p + e * C::TPT;
+      if (i < N) {
+        float2 v = make_float2(0.f, 0.f);
+        if (inside) {
+          const float2 a = t0[i], b = t1[i];
+          float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
+          h.y *= sg;
+          v = cscale(cmul(h, line[i * C::L]), otf.inv_n);
+        }
+        line[i * C::L] = v;
+      }
+    }
+  } else {
+    const Otf3d& T = otf.tab3;
+    const float kyindex = ky < 0.f ? ky + (float)T.nyotf : ky;
+    // the >= 0 test only excludes |ky| beyond the table, which the reference
+    // would read out of bounds
+    const bool inxy = (kx < (float)(T.nxotf - 1)) && (kyindex < (float)T.nyotf) && (kyindex >= 0.f);
+    const int ix = inxy ? (int)floorf(kx) : 0;
+    const int iy = inxy ? (int)floorf(kyindex) : 0;
+    const int iy1 = (iy + 1) % T.nyotf;
+    const float ax = kx - (float)ix, ay = kyindex - (float)iy;
+    const long nxy = (long)T.nxotf * T.nyotf;
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+      const int i = p + e * C::TPT;
+      if (i < N) {
+        const int kzi = (i > (N >> 1)) ? i - N : i;
+        const float kz = (float)kzi * otf.kzscale;
+        const float kzindex = kz < 0.f ? kz + (float)T.nzotf : kz;
+        float2 v = make_float2(0.f, 0.f);
+        if (inxy && kzindex < (float)T.nzotf && kzindex >= 0.f) {
+          const int iz = (int)floorf(kzindex);
+          const int iz1 = (iz + 1) % T.nzotf;
+          const float az = kzindex - (float)iz;
+          auto g = [&](int zz, int yy, int xx) { return T.raw[zz * nxy + (long)yy * T.nxotf + xx]; };
+          const float2 a000 = g(iz, iy, ix), a001 = g(iz, iy, ix + 1), a010 = g(iz, iy1, ix),
+                       a011 = g(iz, iy1, ix + 1), a100 = g(iz1, iy, ix), a101 = g(iz1, iy, ix + 1),
+                       a110 = g(iz1, iy1, ix), a111 = g(iz1, iy1, ix + 1);
+          float2 h;
+          h.x = (1 - az) * (a000.x * (1 - ay) * (1 - ax) + a001.x * (1 - ay) * ax + a010.x * ay * (1 - ax) +
+                            a011.x * ay * ax) +
+                az * (a100.x * (1 - ay) * (1 - ax) + a101.x * (1 - ay) * ax + a110.x * ay * (1 - ax) +
+                      a111.x * ay * ax);
+          h.y = (1 - az) * (a000.y * (1 - ay) * (1 - ax) + a001.y * (1 - ay) * ax + a010.y * ay * (1 - ax) +
+                            a011.y * ay * ax) +
+                az * (a100.y * (1 - ay) * (1 - ax) + a101.y * (1 - ay) * ax + a110.y * ay * (1 - ax) +
+                      a111.y * ay * ax);
+          h.y *= sg;
+          v = cscale(cmul(h, line[i * C::L]), otf.inv_n);
+        }
+        line[i * C::L] = v;
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<N, C::TPT, C::L, true>(line, p, tw);
+  if (valid) {
+#pragma unroll
+    for (int e = 0; e < C::E; ++e) {
+      const int i = p + e * C::TPT;
+      if (i < N) base[i * line_stride] = line[i * C::L];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side: length dispatch
+// ---------------------------------------------------------------------------
+#define CDN_FFT_LENGTHS(X) \
+  X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(810) X(1024) X(2048)
+
+bool fft_length_supported(int n) {
+  switch (n) {
+#define CDN_CASE(N) \
+  case N:           \
+    return true;
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      return false;
+  }
+}
+
+static Plan host_plan(int n) {
+  switch (n) {
+#define CDN_CASE(N) \
+  case N:           \
+    return make_plan(N);
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("FFT length " + std::to_string(n) + " is not compiled into this library");
+  }
+  return Plan{};
+}
+
+int fft_twiddle_count(int n) {
+  const Plan P = host_plan(n);
+  return P.twsize > 0 ? P.twsize : 1;
+}
+
+void fft_twiddles(int n, float2* out) {
+  const Plan P = host_plan(n);
+  out[0] = make_float2(1.f, 0.f);
+  for (int s = 0; s < P.ns; ++s) {
+    const int R = P.r[s], lp = P.lp[s];
+    if (lp == 1) continue;
+    for (int k = 0; k < lp; ++k)
+      for (int t = 1; t < R; ++t) {
+        // exp(-2 pi i t k / (lp R)) in long double, reduced by integer modulus
+        const long M = (long)lp * R;
+        const long m = ((long)t * k) % M;
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)M;
+        out[P.twoff[s] + k * (R - 1) + (t - 1)] = make_float2((float)cosl(a), (float)sinl(a));
+      }
+  }
+}
+
+static inline unsigned grid1(long n) {
+  if (n <= 0) return 0;
+  if (n > 0x7fffffffL) fail("grid too large");
+  return (unsigned)n;
+}
+
+#define CDN_LAUNCH(kernel, N, grid, block, ...) \
+  hipLaunchKernelGGL(kernel<N>, dim3(grid), dim3(block), 0, st, __VA_ARGS__)
+
+void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam, int mode,
+                  float2* S, long rows, long pitch, const float2* tw) {
+  switch (nx) {
+#define CDN_CASE(N)                                                                                   \
+  case N: {                                                                                           \
+    const long pairs = (rows + 1) / 2;                                                                \
+    CDN_LAUNCH(k_x_r2c, N, grid1((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL), XCfg<N>::THREADS, X, Xp, Y, \
+               lam, mode, S, rows, pitch, tw);                                                        \
+    break;                                                                                            \
+  }
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("unsupported x length");
+  }
+  CDN_HIP(hipGetLastError());
+}
+
+void launch_x_c2r(int nx, hipStream_t st, const float2* S, float* out, long rows, long pitch, const float2* tw) {
+  switch (nx) {
+#define CDN_CASE(N)                                                                                 \
+  case N: {                                                                                         \
+    const long pairs = (rows + 1) / 2;                                                              \
+    CDN_LAUNCH(k_x_c2r, N, grid1((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL), XCfg<N>::THREADS, S, out, \
+               rows, pitch, tw);                                                                    \
+    break;                                                                                          \
+  }
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("unsupported x length");
+  }
+  CDN_HIP(hipGetLastError());
+}
+
+void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long rows, long pitch, const float2* tw,
+                     float eps) {
+  switch (nx) {
+#define CDN_CASE(N)                                                                                    \
+  case N: {                                                                                            \
+    const long pairs = (rows + 1) / 2;                                                                 \
+    CDN_LAUNCH(k_x_lrcore, N, grid1((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL), XCfg<N>::THREADS, S, raw, \
+               rows, pitch, tw, eps);                                                                  \
+    break;                                                                                             \
+  }
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("unsupported x length");
+  }
+  CDN_HIP(hipGetLastError());
+}
+
+int x_update_blocks(int nx, long rows) {
+  const long pairs = (rows + 1) / 2;
+  switch (nx) {
+#define CDN_CASE(N) \
+  case N:           \
+    return (int)((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL);
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("unsupported x length");
+  }
+  return 0;
+}
+
+int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, float* Xout, float* G, double* part,
+                    int dots, long rows, long pitch, const float2* tw) {
+  const int nb = x_update_blocks(nx, rows);
+  switch (nx) {
+#define CDN_CASE(N)                                                                                   \
+  case N:                                                                                             \
+    CDN_LAUNCH(k_x_update, N, grid1(nb), XCfg<N>::THREADS, S, Y, Xout, G, part, dots, rows, pitch, tw); \
+    break;
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("unsupported x length");
+  }
+  CDN_HIP(hipGetLastError());
+  return nb;
+}
+
+void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols, long line_stride,
+                    long outer_stride, long n_outer, const float2* tw) {
+  switch (len) {
+#define CDN_CASE(N)                                                                                            \
+  case N: {                                                                                                    \
+    const long ntiles = (ncols + CCfg<N>::L - 1) / CCfg<N>::L;                                                 \
+    if (inverse)                                                                                               \
+      hipLaunchKernelGGL((k_col<N, true>), dim3(grid1(ntiles * n_outer)), dim3(CCfg<N>::THREADS), 0, st, S,    \
+                         ncols, line_stride, outer_stride, ntiles, tw);                                        \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_col<N, false>), dim3(grid1(ntiles * n_outer)), dim3(CCfg<N>::THREADS), 0, st, S,   \
+                         ncols, line_stride, outer_stride, ntiles, tw);                                        \
+    break;                                                                                                     \
+  }
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("unsupported column length");
+  }
+  CDN_HIP(hipGetLastError());
+}
+
+void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride, long ny,
+                     int conj, const OtfParams& otf, const float2* tw) {
+  switch (nz) {
+#define CDN_CASE(N)                                                                                       \
+  case N: {                                                                                               \
+    const long ntiles = (ncols + CCfg<N>::L - 1) / CCfg<N>::L;                                            \
+    CDN_LAUNCH(k_z_filter, N, grid1(ntiles * ny), CCfg<N>::THREADS, S, ncols, line_stride, outer_stride,  \
+               ntiles, ny, conj, otf, tw);                                                                \
+    break;                                                                                                \
+  }
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      fail("unsupported z length");
+  }
+  CDN_HIP(hipGetLastError());
+}
+
+}  // namespace cdn

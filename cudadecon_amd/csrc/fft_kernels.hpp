@@ -1,0 +1,69 @@
+// Host-side launch interface of the 3-D FFT pass kernels (fft_kernels.hip).
+//
+// Volume layout: real (nz, ny, nx), x fastest, no padding.  Half spectrum:
+// (nz, ny, nx/2+1) complex64 with row pitch `pitch` >= nx/2+1 (padded to a
+// multiple of 16 so column tiles start on 128-byte boundaries).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cdn {
+
+// How the x-forward pass obtains its real input Y (one RL-BA iteration's
+// first kernel, src/RL-Biggs-Andrews.cpp:296-321).
+enum XInMode : int {
+  XIN_PLAIN = 0,  // Y = X (read X)
+  XIN_PRED = 1,   // Y = max(X + lambda (X - Xp), 0); Y is also stored
+  XIN_SEED = 2,   // Y read from the Y buffer (warm-start seed)
+};
+
+struct RaOtf {
+  const float2* tz;  // z-interpolated radial table, (nr, nz) kz fastest
+  int nr;            // radial samples (nxotf); valid iff kr < nr - 1
+};
+
+struct Otf3d {
+  const float2* raw;  // (nzotf, nyotf, nxotf) complex, x fastest
+  int nxotf, nyotf, nzotf;
+};
+
+struct OtfParams {
+  int ra;  // 1: rotationally averaged (RaOtf), 0: 3-D table (Otf3d)
+  RaOtf ra_tab;
+  Otf3d tab3;
+  float kxscale, kyscale, kzscale;
+  float inv_n;  // 1 / (nx ny nz): the filter's normalisation
+};
+
+bool fft_length_supported(int n);
+// Twiddle table for a supported length (host, FP64-derived float2 values).
+int fft_twiddle_count(int n);
+void fft_twiddles(int n, float2* out);
+
+// R2C along x of row pairs with the prediction fused in; writes S.
+void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam,
+                  int mode, float2* S, long rows, long pitch, const float2* tw);
+// plain C2R along x (ignores imag of kx = 0 and kx = nx/2).
+void launch_x_c2r(int nx, hipStream_t st, const float2* S, float* out, long rows, long pitch,
+                  const float2* tw);
+// C2R along x -> LRcore with raw -> R2C along x, in place on S.
+void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long rows, long pitch,
+                     const float2* tw, float eps);
+// C2R along x -> X = max(C Y, 0), G_new = X - Y written over G; if dots:
+// per-block partial sums of G_new.G_old and G_old.G_old (FP32 products,
+// FP64 sums) into part[2*block..].  Returns the number of blocks launched.
+int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, float* Xout, float* G,
+                    double* part, int dots, long rows, long pitch, const float2* tw);
+int x_update_blocks(int nx, long rows);
+
+// Column FFT (forward or inverse) over `len` points at element stride
+// line_stride, for columns [0, ncols) of each of n_outer slabs.
+void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols, long line_stride,
+                    long outer_stride, long n_outer, const float2* tw);
+// z forward -> x H (or conj H) / N -> z inverse; slab index = ky.
+void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride,
+                     long ny, int conj, const OtfParams& otf, const float2* tw);
+
+}  // namespace cdn

@@ -1,0 +1,330 @@
+// Elementwise, reduction and geometry kernels around the RL-BA iteration.
+// 64-bit indexing throughout (the reference's 32-bit products overflow at
+// 2^31 voxels, src/RLgpuImpl.cu:167,383-388; src/geometryTransform.cu:15,23).
+#include "common.hpp"
+#include "rl_kernels.hpp"
+
+#include <cmath>
+
+namespace cdn {
+
+static constexpr int BS = 256;  // threads per block for streaming kernels
+
+static inline unsigned nblocks(long n, int per_thread = 1) {
+  long b = (n + (long)BS * per_thread - 1) / ((long)BS * per_thread);
+  if (b < 1) b = 1;
+  if (b > 65535L * 64) b = 65535L * 64;  // grid-stride loops cover the rest
+  return (unsigned)b;
+}
+
+// ---------------------------------------------------------------------------
+// uint16 -> float with corner crop (src/RL-Biggs-Andrews.cpp:674-677)
+// ---------------------------------------------------------------------------
+__global__ void k_u16_crop(const uint16_t* __restrict__ in, int nx, int ny, float* __restrict__ out, int onx, int ony,
+                           long n_out) {
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n_out; i += (long)gridDim.x * BS) {
+    const long x = i % onx;
+    const long r = i / onx;
+    const long y = r % ony, z = r / ony;
+    out[i] = (float)in[(z * ny + y) * nx + x];
+  }
+}
+void u16_to_f32_crop(hipStream_t st, const uint16_t* in, int nx, int ny, int nz, float* out, int onx, int ony,
+                     int onz) {
+  const long n = (long)onx * ony * onz;
+  hipLaunchKernelGGL(k_u16_crop, dim3(nblocks(n)), dim3(BS), 0, st, in, nx, ny, out, onx, ony, n);
+  CDN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// apodize_x_kernel / apodize_y_kernel (src/RLgpuImpl.cu:848-894)
+// ---------------------------------------------------------------------------
+__global__ void k_apodize_rows(int napodize, int nx, int ny, float* img) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nx) return;
+  float* s = img + (long)blockIdx.y * nx * ny;
+  const float diff = (float)((s[(long)(ny - 1) * nx + k] - s[k]) / 2.0);
+  for (int l = 0; l < napodize; ++l) {
+    const float fact = (float)(diff * (1.0 - sin((((float)l + 0.5) / (float)napodize) * M_PI * 0.5)));
+    s[(long)l * nx + k] += fact;
+    s[(long)(ny - 1 - l) * nx + k] -= fact;
+  }
+}
+__global__ void k_apodize_cols(int napodize, int nx, int ny, float* img) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= ny) return;
+  float* r = img + (long)blockIdx.y * nx * ny + (long)l * nx;
+  const float diff = (float)((r[nx - 1] - r[0]) / 2.0);
+  for (int k = 0; k < napodize; ++k) {
+    const float fact = (float)(diff * (1.0 - sin(((k + 0.5) / (float)napodize) * M_PI * 0.5)));
+    r[k] += fact;
+    r[nx - 1 - k] -= fact;
+  }
+}
+void apodize(hipStream_t st, float* img, int nx, int ny, int nz, int napodize) {
+  if (napodize <= 0) return;
+  hipLaunchKernelGGL(k_apodize_rows, dim3((nx + 63) / 64, nz), dim3(64), 0, st, napodize, nx, ny, img);
+  hipLaunchKernelGGL(k_apodize_cols, dim3((ny + 63) / 64, nz), dim3(64), 0, st, napodize, nx, ny, img);
+  CDN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// bgsubtr_kernel (src/RLgpuImpl.cu:124-132) and rescale via scale_kernel
+// (src/RLgpuImpl.cu:383-388, 829-846)
+// ---------------------------------------------------------------------------
+__global__ void k_bgsub(float* img, long n, float bg) {
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    const float v = img[i] - bg;
+    img[i] = v > 0.f ? v : 0.f;
+  }
+}
+void bgsub(hipStream_t st, float* img, long n, float bg) {
+  hipLaunchKernelGGL(k_bgsub, dim3(nblocks(n)), dim3(BS), 0, st, img, n, bg);
+  CDN_HIP(hipGetLastError());
+}
+__global__ void k_scale(float* img, long n, float s) {
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) img[i] *= s;
+}
+void rescale(hipStream_t st, float* img, long n, float s) {
+  hipLaunchKernelGGL(k_scale, dim3(nblocks(n)), dim3(BS), 0, st, img, n, s);
+  CDN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// meanAboveBackground_GPU (src/RLgpuImpl.cu:668-827): mean in FP64, rounded
+// to float; then FP64 sum and count of voxels strictly above it.
+// ---------------------------------------------------------------------------
+static constexpr int RED_BLOCKS = 1024;
+
+__device__ __forceinline__ void block_reduce2(double a, double b, double* out) {
+  __shared__ double ra[BS / 64], rb[BS / 64];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    ra[threadIdx.x >> 6] = a;
+    rb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0, sb = 0;
+    for (int w = 0; w < BS / 64; ++w) {
+      sa += ra[w];
+      sb += rb[w];
+    }
+    out[0] = sa;
+    out[1] = sb;
+  }
+}
+
+__global__ void __launch_bounds__(BS) k_sum_part(const float* __restrict__ img, long n, double* part) {
+  double s = 0;
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) s += (double)img[i];
+  block_reduce2(s, 0.0, part + 2 * blockIdx.x);
+}
+__global__ void __launch_bounds__(BS) k_above_part(const float* __restrict__ img, long n,
+                                                    const float* __restrict__ thresh, double* part) {
+  const float t = *thresh;
+  double s = 0, c = 0;
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    const float v = img[i];
+    if (v > t) {
+      s += (double)v;
+      c += 1.0;
+    }
+  }
+  block_reduce2(s, c, part + 2 * blockIdx.x);
+}
+// One block: out[0], out[1] = column sums of the nb partial pairs.
+__global__ void __launch_bounds__(BS) k_finish2(const double* __restrict__ part, int nb, double* out) {
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nb; i += BS) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  block_reduce2(a, b, out);
+}
+__global__ void k_mean_to_float(const double* sum, long n, float* mean) { *mean = (float)(sum[0] / (double)n); }
+
+double mean_above_background(hipStream_t st, const float* img, long n, double* d_part, double* d_acc, float* d_mean) {
+  const int nb = (int)std::min<long>(RED_BLOCKS, (n + BS - 1) / BS);
+  hipLaunchKernelGGL(k_sum_part, dim3(nb), dim3(BS), 0, st, img, n, d_part);
+  hipLaunchKernelGGL(k_finish2, dim3(1), dim3(BS), 0, st, d_part, nb, d_acc);
+  hipLaunchKernelGGL(k_mean_to_float, dim3(1), dim3(1), 0, st, d_acc, n, d_mean);
+  hipLaunchKernelGGL(k_above_part, dim3(nb), dim3(BS), 0, st, img, n, d_mean, d_part);
+  hipLaunchKernelGGL(k_finish2, dim3(1), dim3(BS), 0, st, d_part, nb, d_acc + 2);
+  CDN_HIP(hipGetLastError());
+  double h[4];
+  CDN_HIP(hipMemcpyAsync(h, d_acc, sizeof(h), hipMemcpyDeviceToHost, st));
+  CDN_HIP(hipStreamSynchronize(st));
+  return h[2] / h[3];
+}
+
+// ---------------------------------------------------------------------------
+// z-edge taper: the intended form of zBlend_kernel (src/RLgpuImpl.cu:908-925),
+// whose zidx = blockDim.z race is not reproduced.  The first/last slice
+// difference is taken once per (x, y), then slices zi and nz-1-zi receive
+// +/- diff * (1 - sin((zi + 0.5) / nZblend * pi / 2)).
+// ---------------------------------------------------------------------------
+__global__ void k_zblend(float* img, long nxy, int nz, int nZblend) {
+  for (long p = (long)blockIdx.x * BS + threadIdx.x; p < nxy; p += (long)gridDim.x * BS) {
+    const float diff = img[(long)(nz - 1) * nxy + p] - img[p];
+    for (int zi = 0; zi < nZblend; ++zi) {
+      const float fact = (float)(diff * (1.0 - sin(((zi + 0.5) / (float)nZblend) * M_PI * 0.5)));
+      img[(long)zi * nxy + p] += fact;
+      img[(long)(nz - zi - 1) * nxy + p] -= fact;
+    }
+  }
+}
+void zblend(hipStream_t st, float* img, int nx, int ny, int nz, int nZblend) {
+  if (nZblend <= 0) return;
+  const long nxy = (long)nx * ny;
+  hipLaunchKernelGGL(k_zblend, dim3(nblocks(nxy)), dim3(BS), 0, st, img, nxy, nz, nZblend);
+  CDN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Acceleration factor (calcAccelFactor, src/RLgpuImpl.cu:510-569, clamped to
+// [0, 1] at src/RL-Biggs-Andrews.cpp:297-298), kept on the device:
+// lam[0] = clamp(sum num / (sum den + eps)), lam[1..2] = the two sums.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BS) k_lambda(const double* __restrict__ part, int nb, double eps, double* lam) {
+  __shared__ double tot[2];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nb; i += BS) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  block_reduce2(a, b, tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double l = tot[0] / (tot[1] + eps);
+    l = l < 1.0 ? l : 1.0;
+    l = l > 0.0 ? l : 0.0;
+    lam[0] = l;
+    lam[1] = tot[0];
+    lam[2] = tot[1];
+  }
+}
+void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam) {
+  hipLaunchKernelGGL(k_lambda, dim3(1), dim3(BS), 0, st, part, nb, eps, lam);
+  CDN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Geometry
+// ---------------------------------------------------------------------------
+// deskew_kernel (src/geometryTransform.cu:4-31): xin evaluated in double,
+// rounded to float; linear interpolation along x; padVal outside [0, nx-1).
+__global__ void k_deskew(const float* __restrict__ in, int nx, int ny, int nz, float* __restrict__ out, int nxOut,
+                         int extraShift, double deskewFactor, float padVal) {
+  const int xo = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (xo >= nxOut) return;
+  const float xin = (float)(((double)xo - nxOut / 2. + extraShift) - deskewFactor * ((double)z - nz / 2.) + nx / 2.);
+  const long io = ((long)z * ny + y) * nxOut + xo;
+  if (xin >= 0.f && xin < (float)(nx - 1)) {
+    const float fl = floorf(xin);
+    const long ii = ((long)z * ny + y) * nx + (long)fl;
+    const float off = xin - fl;
+    out[io] = (1.f - off) * in[ii] + off * in[ii + 1];
+  } else {
+    out[io] = padVal;
+  }
+}
+void deskew(hipStream_t st, const float* in, int nx, int ny, int nz, double factor, float* out, int nxOut,
+            int extraShift, float padVal) {
+  hipLaunchKernelGGL(k_deskew, dim3((nxOut + 127) / 128, ny, nz), dim3(128), 0, st, in, nx, ny, nz, out, nxOut,
+                     extraShift, factor, padVal);
+  CDN_HIP(hipGetLastError());
+}
+
+// rotate_kernel (src/geometryTransform.cu:50-88): (z, x) rotation about the
+// centre with bilinear interpolation, z-flipped output, 0 outside.
+__global__ void k_rotate(const float* __restrict__ in, int nx_in, int ny, int nz_in, float* __restrict__ out,
+                         int nx_out, int nz_out, float r0, float r1, float r2, float r3) {
+  const int xo = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, zo = blockIdx.z;
+  if (xo >= nx_out) return;
+  const float xc = (float)(xo - nx_out / 2.);
+  const float zc = (float)(zo - nz_out / 2.);
+  const float zin = (float)((double)(r0 * zc + r1 * xc) + nz_in / 2.);
+  const float xin = (float)((double)(r2 * zc + r3 * xc) + nx_in / 2.);
+  const long io = ((long)(nz_out - 1 - zo) * ny + y) * nx_out + xo;
+  if (xin >= 0.f && xin < (float)(nx_in - 1) && zin >= 0.f && zin < (float)(nz_in - 1)) {
+    const float fx = floorf(xin), fz = floorf(zin);
+    const long nxy = (long)nx_in * ny;
+    const long i00 = (long)fz * nxy + (long)y * nx_in + (long)fx;
+    const float ax = xin - fx, az = zin - fz;
+    out[io] = (1.f - az) * ((1.f - ax) * in[i00] + ax * in[i00 + 1]) +
+              az * ((1.f - ax) * in[i00 + nxy] + ax * in[i00 + nxy + 1]);
+  } else {
+    out[io] = 0.f;
+  }
+}
+void rotate(hipStream_t st, const float* in, int nx, int ny, int nz, const float* rot, float* out, int nx_out,
+            int nz_out) {
+  hipLaunchKernelGGL(k_rotate, dim3((nx_out + 127) / 128, ny, nz_out), dim3(128), 0, st, in, nx, ny, nz, out, nx_out,
+                     nz_out, rot[0], rot[1], rot[2], rot[3]);
+  CDN_HIP(hipGetLastError());
+}
+
+// transformKernel / transformKernelRA (src/geometryTransform.cu:183-253):
+// pull affine, trilinear with the texture convention (texel centres at
+// +0.5) and border 0.  Weights are exact FP32 (the reference's texture unit
+// quantises them to 8 fractional bits).
+struct Affine {
+  float m[12];
+  float dx, dy, dz;
+  int ra;
+};
+__device__ __forceinline__ float tex_at(const float* __restrict__ in, int nx, int ny, int nz, int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) return 0.f;
+  return in[((long)k * ny + j) * nx + i];
+}
+__global__ void k_affine(const float* __restrict__ in, int nx, int ny, int nz, float* __restrict__ out, Affine A) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= nx) return;
+  float u = x, v = y, w = z;
+  float tu, tv, tw;
+  if (A.ra) {
+    u = (float)(0.5 + (u - 0.5) * A.dx);
+    v = (float)(0.5 + (v - 0.5) * A.dy);
+    w = (float)(0.5 + (w - 0.5) * A.dz);
+    tu = A.m[0] * u + A.m[1] * v + A.m[2] * w + A.m[3];
+    tv = A.m[4] * u + A.m[5] * v + A.m[6] * w + A.m[7];
+    tw = A.m[8] * u + A.m[9] * v + A.m[10] * w + A.m[11];
+    tu = (float)(0.5 + (tu - 0.5) / A.dx);
+    tv = (float)(0.5 + (tv - 0.5) / A.dy);
+    tw = (float)(0.5 + (tw - 0.5) / A.dz);
+  } else {
+    tu = A.m[0] * u + A.m[1] * v + A.m[2] * w + A.m[3] + 0.5f;
+    tv = A.m[4] * u + A.m[5] * v + A.m[6] * w + A.m[7] + 0.5f;
+    tw = A.m[8] * u + A.m[9] * v + A.m[10] * w + A.m[11] + 0.5f;
+  }
+  const float xb = tu - 0.5f, yb = tv - 0.5f, zb = tw - 0.5f;
+  const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+  const float ax = xb - fx, ay = yb - fy, az = zb - fz;
+  // clamp the base index into int range before the per-texel bounds test
+  const float lim = 1.0e9f;
+  const int i = (int)fmaxf(fminf(fx, lim), -lim), j = (int)fmaxf(fminf(fy, lim), -lim),
+            k = (int)fmaxf(fminf(fz, lim), -lim);
+  const float r = (1 - az) * ((1 - ay) * ((1 - ax) * tex_at(in, nx, ny, nz, i, j, k) + ax * tex_at(in, nx, ny, nz, i + 1, j, k)) +
+                              ay * ((1 - ax) * tex_at(in, nx, ny, nz, i, j + 1, k) + ax * tex_at(in, nx, ny, nz, i + 1, j + 1, k))) +
+                  az * ((1 - ay) * ((1 - ax) * tex_at(in, nx, ny, nz, i, j, k + 1) + ax * tex_at(in, nx, ny, nz, i + 1, j, k + 1)) +
+                        ay * ((1 - ax) * tex_at(in, nx, ny, nz, i, j + 1, k + 1) + ax * tex_at(in, nx, ny, nz, i + 1, j + 1, k + 1)));
+  out[((long)z * ny + y) * nx + x] = r;
+}
+void affine(hipStream_t st, const float* in, int nx, int ny, int nz, float* out, const float* mat, int ra, float dx,
+            float dy, float dz) {
+  Affine A;
+  for (int i = 0; i < 12; ++i) A.m[i] = mat[i];
+  A.dx = dx;
+  A.dy = dy;
+  A.dz = dz;
+  A.ra = ra;
+  hipLaunchKernelGGL(k_affine, dim3((nx + 127) / 128, ny, nz), dim3(128), 0, st, in, nx, ny, nz, out, A);
+  CDN_HIP(hipGetLastError());
+}
+
+}  // namespace cdn

@@ -1,0 +1,26 @@
+// Launch wrappers for the preprocessing, reduction and geometry kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cdn {
+
+void u16_to_f32_crop(hipStream_t st, const uint16_t* in, int nx, int ny, int nz, float* out, int onx, int ony,
+                     int onz);
+void apodize(hipStream_t st, float* img, int nx, int ny, int nz, int napodize);
+void bgsub(hipStream_t st, float* img, long n, float bg);
+void rescale(hipStream_t st, float* img, long n, float s);
+// d_part: >= 2048 doubles, d_acc: >= 4 doubles, d_mean: 1 float.  Synchronises.
+double mean_above_background(hipStream_t st, const float* img, long n, double* d_part, double* d_acc, float* d_mean);
+void zblend(hipStream_t st, float* img, int nx, int ny, int nz, int nZblend);
+void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam);
+void deskew(hipStream_t st, const float* in, int nx, int ny, int nz, double factor, float* out, int nxOut,
+            int extraShift, float padVal);
+void rotate(hipStream_t st, const float* in, int nx, int ny, int nz, const float* rot, float* out, int nx_out,
+            int nz_out);
+void affine(hipStream_t st, const float* in, int nx, int ny, int nz, float* out, const float* mat, int ra, float dx,
+            float dy, float dz);
+
+}  // namespace cdn

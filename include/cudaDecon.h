@@ -1,0 +1,136 @@
+/*
+ * libcudaDecon.so - C ABI of the MI355X (gfx950) build.
+ *
+ * The first block reproduces, signature for signature, the exported API of
+ * the reference library (src/linearDecon.h:243-317, implemented at
+ * src/RL-Biggs-Andrews.cpp:453-855), so ctypes/cgo/JNI callers bind the
+ * same symbols.  C++ default arguments of the reference header are kept for
+ * C++ callers.  Errors are reported as the reference's return values (1 ok,
+ * 0 failure); no function calls exit() or lets an exception cross the ABI.
+ * The message of the last failure is available from cudaDecon_last_error().
+ *
+ * Not thread-safe, not re-entrant: one process-global session, as in the
+ * reference (src/RL-Biggs-Andrews.cpp:459-466).  The current HIP device at
+ * RL_interface_init time is used (the reference hard-codes device 0).
+ *
+ * The second block adds symbols with no counterpart in the reference.
+ */
+#ifndef CUDADECON_AMD_H
+#define CUDADECON_AMD_H
+
+#ifdef __cplusplus
+#define CDN_DEFAULT(v) = v
+extern "C" {
+#else
+#include <stdbool.h>
+#define CDN_DEFAULT(v)
+#endif
+
+/* ---- reference API ---------------------------------------------------- */
+
+/* replaces RL_interface_init (src/linearDecon.h:260-264;
+ * src/RL-Biggs-Andrews.cpp:531-655).  Returns 1, or 0 if the OTF file cannot
+ * be read or an FFT length is not supported. */
+int RL_interface_init(int nx, int ny, int nz, float dr, float dz, float dr_psf, float dz_psf, float deskewAngle,
+                      float rotationAngle, int outputWidth, bool bSkewedDecon, char* OTF_file_name);
+
+/* replaces RL_interface (src/linearDecon.h:278-283;
+ * src/RL-Biggs-Andrews.cpp:657-705).  `result` holds
+ * get_output_nx()*get_output_ny()*get_output_nz() floats. */
+int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, float* result,
+                 float* raw_deskewed_result, float background, bool bDoRescale, bool bSaveDeskewedRaw, int nIters,
+                 int extraShift, int napodize CDN_DEFAULT(0), int nZblend CDN_DEFAULT(0),
+                 float padVal CDN_DEFAULT(0), bool bDupRevStack CDN_DEFAULT(false),
+                 bool bSkewedDecon CDN_DEFAULT(false));
+
+/* replaces Deskew_interface (src/linearDecon.h:285-288;
+ * src/RL-Biggs-Andrews.cpp:724-773). */
+int Deskew_interface(const float* const raw_data, int nx, int ny, int nz, float dz, float dr, float deskewAngle,
+                     float* const result, int outputWidth, int extraShift, float padVal CDN_DEFAULT(0));
+
+/* replaces Affine_interface (src/linearDecon.h:290-292;
+ * src/RL-Biggs-Andrews.cpp:776-814).  affMat: 16 floats, row-major 4x4. */
+int Affine_interface(const float* const raw_data, int nx, int ny, int nz, float* const result, const float* affMat);
+
+/* replaces Affine_interface_RA (src/linearDecon.h:294-296;
+ * src/RL-Biggs-Andrews.cpp:816-855). */
+int Affine_interface_RA(const float* const raw_data, int nx, int ny, int nz, float dx, float dy, float dz,
+                        float* const result, const float* affMat);
+
+/* declared by the reference (src/linearDecon.h:298-303) but never built
+ * into it; exported here as stubs that return 0. */
+int camcor_interface_init(int nx, int ny, int nz, const float* const camparam);
+int camcor_interface(const unsigned short* const raw_data, int nx, int ny, int nz, unsigned short* const result);
+
+/* replaces RL_cleanup (src/linearDecon.h:306; src/RL-Biggs-Andrews.cpp:707-715) */
+void RL_cleanup(void);
+/* replaces cuda_reset (src/linearDecon.h:307; src/RL-Biggs-Andrews.cpp:717-720) */
+void cuda_reset(void);
+
+/* replace get_output_nx/ny/nz (src/linearDecon.h:311-313;
+ * src/RL-Biggs-Andrews.cpp:474-503) */
+unsigned get_output_nx(void);
+unsigned get_output_ny(void);
+unsigned get_output_nz(void);
+
+/* replaces get_version (src/linearDecon.h:316) */
+const char* get_version(void);
+
+/* ---- extensions (no counterpart in the reference) --------------------- */
+
+/* Message of the last failed call ("" if none). */
+const char* cudaDecon_last_error(void);
+
+/* 1 if an FFT length is compiled into this library. */
+int cudaDecon_fft_supported(int n);
+
+/* findOptimalDimension (src/RL-Biggs-Andrews.cpp:33-43); n <= 1 returned as is. */
+unsigned cudaDecon_find_optimal_dimension(unsigned n, int step);
+
+/* Session sizes: which = 0 output nz (before rotation), 1 deskewed nx,
+ * 2 cropped nx, 3 FFT nx of the iteration.  0 without a session. */
+unsigned cudaDecon_get_session_dims(int which);
+
+/* Unnormalised 3-D R2C / C2R of host arrays: real (nz, ny, nx) <->
+ * complex64 interleaved (nz, ny, nx/2+1).  C2R ignores the imaginary parts
+ * of the kx = 0 and kx = nx/2 bins. */
+int cudaDecon_fft_r2c(const float* in, int nx, int ny, int nz, float* out);
+int cudaDecon_fft_c2r(const float* in, int nx, int ny, int nz, float* out);
+
+/* One filterGPU call (src/RLgpuImpl.cu:156-225) on a host volume, in place:
+ * v = C2R(H . R2C(v / N)), H (or conj H) interpolated from an OTF image laid
+ * out as its TIFF file (width, height, depth floats) with k-scales kx, ky, kz. */
+int cudaDecon_filter(float* vol, int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
+                     int otf_depth, float kxscale, float kyscale, float kzscale, int conj);
+
+/* RL-Biggs-Andrews engine on HBM-resident state (the iteration of
+ * RL_interface without pre/post-processing).  The OTF is given as its
+ * image (see cudaDecon_filter) and the k-scales of the grid. */
+typedef struct cudaDecon_rl cudaDecon_rl;
+cudaDecon_rl* cudaDecon_rl_create(int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
+                                  int otf_depth, float kxscale, float kyscale, float kzscale);
+/* preprocessed stack (also X_0), host memory */
+int cudaDecon_rl_set_raw(cudaDecon_rl* h, const float* raw);
+/* seed for Y_0 (the FlatStart hook, src/RL-Biggs-Andrews.cpp:305-316) */
+int cudaDecon_rl_set_seed(cudaDecon_rl* h, const float* seed);
+/* iterations k0 .. k0+n-1; lambdas (n, nullable) receives the factor used
+ * at each k; elapsed_ms (nullable) the device time of the n iterations. */
+int cudaDecon_rl_iterate(cudaDecon_rl* h, int n, int k0, int seeded, double* lambdas, double* elapsed_ms);
+int cudaDecon_rl_get(cudaDecon_rl* h, float* X);
+/* per-pass device time: enable != 0 turns event timing on (and clears the
+ * sums); pass_ms / pass_calls (7 entries: x-R2C, y-fwd, z-filter, y-inv,
+ * x-LRcore, x-update, lambda) receive the sums so far (nullable). */
+int cudaDecon_rl_timing(cudaDecon_rl* h, int enable, double* pass_ms, long* pass_calls);
+/* bytes of one half-spectrum row pitch (complex elements) and volume sizes */
+int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pitch);
+void cudaDecon_rl_destroy(cudaDecon_rl* h);
+
+/* Synchronise the current device. */
+int cudaDecon_device_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#undef CDN_DEFAULT
+
+#endif /* CUDADECON_AMD_H */

@@ -1,0 +1,105 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/cudaDecon.h declares, and its host-only entry points behave (no GPU
+call is made here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# exported symbols of the reference library (src/linearDecon.h:243-317)
+REFERENCE_SYMBOLS = [
+    "RL_interface_init", "RL_interface", "Deskew_interface", "Affine_interface", "Affine_interface_RA",
+    "camcor_interface_init", "camcor_interface", "RL_cleanup", "cuda_reset", "get_output_nx", "get_output_ny",
+    "get_output_nz", "get_version",
+]
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "cudaDecon.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"CDN_DEFAULT", "if", "defined"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def test_header_declares_reference_api():
+    decl = declared_symbols()
+    for s in REFERENCE_SYMBOLS:
+        assert s in decl, s
+
+
+def test_library_exports_every_declared_symbol(cdn):
+    L = ctypes.CDLL(cdn.lib_path())
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_python_bindings_cover_header(cdn):
+    assert set(declared_symbols()) == set(cdn.SIGNATURES)
+
+
+def test_version(cdn):
+    # src/CMakeLists.txt:17 (project version 0.7.0)
+    assert cdn.lib().get_version() == b"0.7.0"
+
+
+def test_no_session_outputs_zero(cdn):
+    L = cdn.lib()
+    L.RL_cleanup()
+    assert L.get_output_nx() == 0 and L.get_output_ny() == 0 and L.get_output_nz() == 0
+
+
+def test_rl_interface_before_init_is_an_error(cdn):
+    import numpy as np
+    L = cdn.lib()
+    L.RL_cleanup()
+    raw = np.zeros((4, 4, 4), np.uint16)
+    out = np.zeros((4, 4, 4), np.float32)
+    rc = L.RL_interface(raw, 4, 4, 4, out, None, 100.0, False, False, 1, 0, 0, 0, 0.0, False, False)
+    assert rc == 0
+    assert "before RL_interface_init" in cdn.last_error()
+
+
+def test_init_with_missing_otf_returns_zero(cdn, tmp_path):
+    L = cdn.lib()
+    rc = L.RL_interface_init(128, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False,
+                             str(tmp_path / "missing.tif").encode())
+    assert rc == 0
+    assert "cannot open" in cdn.last_error()
+
+
+def test_camcor_stubs_return_zero(cdn):
+    L = cdn.lib()
+    assert L.camcor_interface_init(1, 1, 1, None) == 0
+    assert L.camcor_interface(None, 1, 1, 1, None) == 0
+
+
+def test_find_optimal_dimension_matches_oracle(cdn, orc):
+    L = cdn.lib()
+    for n in list(range(1, 3000)) + [3727, 4096, 8191, 8192]:
+        assert L.cudaDecon_find_optimal_dimension(n, -1) == orc.find_optimal_dimension(n, -1), n
+        assert L.cudaDecon_find_optimal_dimension(n, 1) == orc.find_optimal_dimension(n, 1), n
+
+
+@pytest.mark.parametrize("n", [64, 96, 128, 192, 256, 384, 512, 768, 810, 1024, 2048])
+def test_supported_fft_lengths(cdn, n):
+    assert cdn.lib().cudaDecon_fft_supported(n) == 1
+
+
+def test_unsupported_fft_length_is_reported(cdn):
+    assert cdn.lib().cudaDecon_fft_supported(11) == 0
+
+
+def test_init_rejects_unsupported_length(cdn, tmp_path):
+    import numpy as np
+    from cudadecon_amd.otf import write_otf_tiff
+    p = tmp_path / "otf.tif"
+    write_otf_tiff(str(p), np.ones((17, 33), np.complex64))
+    L = cdn.lib()
+    # 1001 -> findOptimalDimension 1000 = 2^3 5^3, not compiled in
+    rc = L.RL_interface_init(1001, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, str(p).encode())
+    assert rc == 0
+    assert "1000" in cdn.last_error()

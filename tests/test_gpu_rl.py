@@ -1,0 +1,166 @@
+"""GPU parity of the RL-Biggs-Andrews path against the FP64 oracle.
+
+Tolerance (the stated per-config bar): max |X_gpu - X_oracle| <= 1e-4 *
+max |X_oracle| after the stated iteration count; acceleration factors agree
+to 1e-4 absolute.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _preprocessed(raw_u16, bg=100.0):
+    return np.maximum(raw_u16.astype(np.float64) - bg, 0.0)
+
+
+def _otf_and_geom(orc, shape, sxy=1.5, sz=3.0, dr=0.104, dz=0.2):
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import gaussian_psf
+    tab = make_otf(gaussian_psf(sxy, sz))
+    g = orc.init_geometry(shape[2], shape[1], shape[0], dr, dz, dr, dz, 0.0, 0.0, 0, False,
+                          (2 * tab.shape[1], tab.shape[0], 1))
+    return tab, g
+
+
+def _relmax(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("shape,iters", [((64, 128, 128), 10), ((64, 96, 810), 6), ((96, 64, 64), 12)],
+                         ids=["cfg0", "x810", "z96"])
+def test_engine_matches_oracle(cdn, orc, shape, iters):
+    print("engine", shape, iters, flush=True)
+    from cudadecon_amd.synth import make_stack
+    raw, _, _ = make_stack("cfg0", shape=shape)
+    R = _preprocessed(raw)
+    tab, g = _otf_and_geom(orc, shape)
+    H = orc.otf_interp(tab, g["otf_dims"], g["fft_dims"], g["kscale"])
+    ref, lam_ref = orc.rl_ba(R, H, iters)
+    eng = cdn.RLEngine(shape, tab, g["kscale"])
+    eng.set_raw(R.astype(np.float32))
+    lam, _ = eng.iterate(iters)
+    got = eng.get()
+    eng.close()
+    assert _relmax(got, ref) <= TOL
+    assert np.abs(lam - lam_ref).max() < 1e-4
+
+
+def test_engine_continuation_equals_single_run(cdn, orc):
+    shape = (64, 128, 128)
+    from cudadecon_amd.synth import make_stack
+    raw, _, _ = make_stack("cfg0", shape=shape)
+    R = _preprocessed(raw).astype(np.float32)
+    tab, g = _otf_and_geom(orc, shape)
+    eng = cdn.RLEngine(shape, tab, g["kscale"])
+    eng.set_raw(R)
+    eng.iterate(7)
+    a = eng.get()
+    eng.iterate(3, k0=0)
+    eng.iterate(4, k0=3)
+    b = eng.get()
+    eng.close()
+    assert np.array_equal(a, b)
+
+
+def test_seeded_start_matches_oracle(cdn, orc):
+    shape = (64, 128, 128)
+    from cudadecon_amd.synth import make_stack
+    raw, _, _ = make_stack("cfg0", shape=shape)
+    R = _preprocessed(raw)
+    seed = np.full(shape, R.mean())
+    tab, g = _otf_and_geom(orc, shape)
+    H = orc.otf_interp(tab, g["otf_dims"], g["fft_dims"], g["kscale"])
+    ref, _ = orc.rl_ba(R, H, 5, seed=seed)
+    eng = cdn.RLEngine(shape, tab, g["kscale"])
+    eng.set_raw(R.astype(np.float32))
+    eng.set_seed(seed.astype(np.float32))
+    eng.iterate(5, seeded=True)
+    got = eng.get()
+    eng.close()
+    assert _relmax(got, ref) <= TOL
+
+
+def _write_otf(tmp_path, tab):
+    from cudadecon_amd.otf import write_otf_tiff
+    p = str(tmp_path / "otf.tif")
+    write_otf_tiff(p, tab)
+    return p
+
+
+def _run_both(cdn, orc, tmp_path, raw, tab, dr, dz, angle, rot, iters, **kw):
+    nz, ny, nx = raw.shape
+    p = _write_otf(tmp_path, tab)
+    skewed = kw.get("skewed_decon", False)
+    cdn.rl_cleanup()
+    cdn.rl_init(raw.shape, p, dzdata=dz, dxdata=dr, dzpsf=dz, dxpsf=dr, deskew=angle, rotate=rot,
+                skewed_decon=skewed)
+    g = orc.init_geometry(nx, ny, nz, dr, dz, dr, dz, angle, rot, 0, skewed, (2 * tab.shape[1], tab.shape[0], 1))
+    assert cdn.output_shape() == (g["out_dims"][2], g["out_dims"][1], g["out_dims"][0])
+    sess = orc.Session(g, tab)
+    ref, _, _ = orc.rl_interface(sess, raw, 100.0, False, False, iters, 0, **kw)
+    got = cdn.rl_decon(raw, background=100.0, n_iters=iters, **kw)
+    cdn.rl_cleanup()
+    return got, ref
+
+
+def test_rl_interface_cfg0(cdn, orc, tmp_path):
+    from cudadecon_amd.synth import make_stack
+    raw, _, c = make_stack("cfg0")
+    tab, _ = _otf_and_geom(orc, raw.shape)
+    got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.2, 0.0, 0.0, c["iters"])
+    assert got.shape == ref.shape
+    assert _relmax(got, ref) <= TOL
+
+
+def test_rl_interface_deskew_rotate_small(cdn, orc, tmp_path):
+    # cfg1 geometry on a reduced stack: 64 x 128 x 100 raw, deskew 31.5 deg,
+    # dz 0.3 um -> deskewed width findOptimalDimension(100 + 157) = 256
+    from cudadecon_amd.synth import make_stack
+    raw, _, _ = make_stack("cfg1", shape=(64, 128, 100))
+    tab, _ = _otf_and_geom(orc, raw.shape, 1.3, 4.0, 0.104, 0.3)
+    nz, ny, nx = raw.shape
+    g = orc.init_geometry(nx, ny, nz, 0.104, 0.3, 0.104, 0.3, 31.5, 31.5, 0, False,
+                          (2 * tab.shape[1], tab.shape[0], 1))
+    if not cdn.lib().cudaDecon_fft_supported(g["deskewed_nx"]):
+        pytest.skip(f"deskewed width {g['deskewed_nx']} not compiled in")
+    got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.3, 31.5, 31.5, 4)
+    assert _relmax(got, ref) <= TOL
+
+
+def test_rl_interface_preprocessing_only_with_rotation(cdn, orc, tmp_path):
+    from cudadecon_amd.synth import make_stack
+    raw, _, _ = make_stack("cfg0", shape=(64, 64, 64))
+    tab, _ = _otf_and_geom(orc, raw.shape)
+    got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.2, 0.0, 20.0, 0, napodize=5)
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
+def test_deskew_interface(cdn, orc):
+    rng = np.random.default_rng(11)
+    im = rng.uniform(0, 1000, (32, 40, 100)).astype(np.float32)
+    for angle, width, shift, pad in [(31.5, 160, 0, 0.0), (-32.8, 180, 3, 7.0), (0.0, 120, -2, 1.0)]:
+        got = cdn.deskew(im, angle, 0.3, 0.104, width=width, shift=shift, pad_val=pad)
+        ref = orc.deskew_interface(im, 0.3, 0.104, angle, width, shift, pad)
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
+def test_affine_interfaces(cdn, orc):
+    rng = np.random.default_rng(12)
+    im = rng.uniform(0, 1000, (24, 48, 64)).astype(np.float32)
+    th = np.deg2rad(4.0)
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0], m[0, 1], m[1, 0], m[1, 1] = np.cos(th) * 1.03, -np.sin(th), np.sin(th), np.cos(th) * 0.97
+    m[0, 3], m[1, 3], m[2, 3] = 2.5, -1.25, 0.5
+    got = cdn.affine(im, m)
+    ref = orc.affine(im, m)
+    assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
+    got = cdn.affine_ra(im, m, 0.104, 0.104, 0.3)
+    ref = orc.affine_ra(im, 0.104, 0.104, 0.3, m)
+    assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
