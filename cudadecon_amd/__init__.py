@@ -59,6 +59,7 @@ SIGNATURES = {
     "cudaDecon_fft_supported": (C.c_int, [C.c_int]),
     "cudaDecon_find_optimal_dimension": (C.c_uint, [C.c_uint, C.c_int]),
     "cudaDecon_get_session_dims": (C.c_uint, [C.c_int]),
+    "cudaDecon_get_session_kscale": (C.c_int, [_f]),
     "cudaDecon_fft_r2c": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f]),
     "cudaDecon_fft_c2r": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f]),
     "cudaDecon_filter": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int, C.c_float,
@@ -145,6 +146,13 @@ def rl_decon(im, background=100.0, n_iters=10, shift=0, save_deskewed=False, res
     return (out, desk) if save_deskewed else out
 
 
+def session_kscale():
+    """OTF k-scales of the current session's iteration grid."""
+    ks = np.zeros(3, np.float32)
+    _check(lib().cudaDecon_get_session_kscale(ks), "cudaDecon_get_session_kscale")
+    return tuple(ks)
+
+
 def deskewed_shape():
     """(nz, ny, nx) of the deskewed raw stack of the current session."""
     L = lib()
@@ -157,10 +165,15 @@ def rl_cleanup():
 
 def deskew(im, angle, dz, dx, width=0, shift=0, pad_val=0.0, out_width=None):
     """Deskew_interface on a float (nz, ny, nx) stack.  out_width must be
-    given when width == 0 (the reference sizes it from its session globals)."""
+    given when width == 0 and angle != 0 (the library then sizes the output
+    from its session globals, src/RL-Biggs-Andrews.cpp:736-744); with a zero
+    angle the output keeps the input width."""
     im = _f32(im)
     nz, ny, nx = im.shape
-    w = int(width) if width else int(out_width)
+    if not abs(float(np.float32(angle))) > 0.0:
+        w = nx
+    else:
+        w = int(width) if width else int(out_width)
     out = np.empty((nz, ny, w), np.float32)
     _check(lib().Deskew_interface(im, nx, ny, nz, float(dz), float(dx), float(angle), out, int(width), int(shift),
                                   float(pad_val)), "Deskew_interface")

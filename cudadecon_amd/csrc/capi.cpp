@@ -193,6 +193,15 @@ unsigned get_output_nx(void) {
 }
 unsigned get_output_ny(void) { return g_s ? g_s->output_ny : 0; }
 
+int cudaDecon_get_session_kscale(float* ks3) {
+  return guard([&]() -> int {
+    if (!g_s) fail("cudaDecon_get_session_kscale: no session");
+    if (!ks3) fail("cudaDecon_get_session_kscale: null argument");
+    kscales(*g_s, g_s->fft_nx, ks3);
+    return 1;
+  });
+}
+
 unsigned cudaDecon_get_session_dims(int which) {
   if (!g_s) return 0;
   switch (which) {

@@ -12,9 +12,17 @@ be regenerated on any host.  Sizes and parameters of the named workloads:
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 PSF_SIZE = 33
+SLAB = 8  # z-slices per noise chunk (fixed, so results do not depend on threads)
+
+
+def _pool():
+    return ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1)))
 
 
 def gaussian_psf(sigma_xy: float, sigma_z: float, size: int = PSF_SIZE) -> np.ndarray:
@@ -41,9 +49,24 @@ def blur(vol: np.ndarray, sigma_xy: float, sigma_z: float) -> np.ndarray:
     (zero outside the volume)."""
     from scipy.ndimage import convolve1d
     kxy, kz = _kernels(sigma_xy, sigma_z)
-    out = convolve1d(vol.astype(np.float32), kxy, axis=2, mode="constant", cval=0.0)
-    out = convolve1d(out, kxy, axis=1, mode="constant", cval=0.0)
-    return convolve1d(out, kz, axis=0, mode="constant", cval=0.0)
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    nz, ny, _ = vol.shape
+    tmp = np.empty_like(vol)
+    out = np.empty_like(vol)
+
+    def xy(z0):
+        s = slice(z0, min(z0 + SLAB, nz))
+        tmp[s] = convolve1d(convolve1d(vol[s], kxy, axis=2, mode="constant", cval=0.0), kxy, axis=1,
+                            mode="constant", cval=0.0)
+
+    def zz(y0):
+        s = slice(y0, min(y0 + 32, ny))
+        out[:, s] = convolve1d(tmp[:, s], kz, axis=0, mode="constant", cval=0.0)
+
+    with _pool() as p:
+        list(p.map(xy, range(0, nz, SLAB)))
+        list(p.map(zz, range(0, ny, 32)))
+    return out
 
 
 def ground_truth(shape, n_beads, rng, n_gauss=3, gauss_sigma=(10.0, 30.0), gauss_amp=50.0,
@@ -70,11 +93,22 @@ def ground_truth(shape, n_beads, rng, n_gauss=3, gauss_sigma=(10.0, 30.0), gauss
     return vol
 
 
-def image(blurred: np.ndarray, rng, background=100.0, offset=100.0) -> np.ndarray:
-    """+background, Poisson, +camera offset, round, clamp to uint16."""
-    lam = np.maximum(blurred.astype(np.float64) + background, 0.0)
-    counts = rng.poisson(lam).astype(np.float64) + offset
-    return np.clip(np.rint(counts), 0, 65535).astype(np.uint16)
+def image(blurred: np.ndarray, seed_seq: np.random.SeedSequence, background=100.0, offset=100.0) -> np.ndarray:
+    """+background, Poisson, +camera offset, round, clamp to uint16.  Each
+    slab of SLAB z-slices draws from its own child generator of seed_seq."""
+    nz = blurred.shape[0]
+    out = np.empty(blurred.shape, np.uint16)
+    kids = seed_seq.spawn((nz + SLAB - 1) // SLAB)
+
+    def slab(i):
+        s = slice(i * SLAB, min((i + 1) * SLAB, nz))
+        lam = np.maximum(blurred[s].astype(np.float64) + background, 0.0)
+        counts = np.random.default_rng(kids[i]).poisson(lam).astype(np.float64) + offset
+        out[s] = np.clip(np.rint(counts), 0, 65535).astype(np.uint16)
+
+    with _pool() as p:
+        list(p.map(slab, range(len(kids))))
+    return out
 
 
 CONFIGS = {
@@ -95,8 +129,8 @@ def make_stack(name: str, shape=None, beads=None):
         vol0 = np.prod(c["shape"])
         c["beads"] = max(1, int(round(c["beads"] * np.prod(shape) / vol0))) if beads is None else beads
         c["shape"] = tuple(shape)
-    rng = np.random.default_rng(c["seed"])
-    gt = ground_truth(c["shape"], c["beads"], rng)
-    raw = image(blur(gt, c["sxy"], c["sz"]), rng)
+    ss_truth, ss_noise = np.random.SeedSequence(c["seed"]).spawn(2)
+    gt = ground_truth(c["shape"], c["beads"], np.random.default_rng(ss_truth))
+    raw = image(blur(gt, c["sxy"], c["sz"]), ss_noise)
     psf = gaussian_psf(c["sxy"], c["sz"])
     return raw, psf, c

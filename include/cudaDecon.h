@@ -91,6 +91,10 @@ unsigned cudaDecon_find_optimal_dimension(unsigned n, int step);
  * 2 cropped nx, 3 FFT nx of the iteration.  0 without a session. */
 unsigned cudaDecon_get_session_dims(int which);
 
+/* OTF k-scales (kx, ky, kz) of the current session's iteration grid, as
+ * RL_interface_init computes them (src/RL-Biggs-Andrews.cpp:634-642). */
+int cudaDecon_get_session_kscale(float* ks3);
+
 /* Unnormalised 3-D R2C / C2R of host arrays: real (nz, ny, nx) <->
  * complex64 interleaved (nz, ny, nx/2+1).  C2R ignores the imaginary parts
  * of the kx = 0 and kx = nx/2 bins. */
