@@ -176,11 +176,18 @@ constexpr Plan make_plan(int N) {
   Plan P{};
   int n = N;
   int ns = 0;
-  // radix 16 while it leaves at least a factor 8 (keeps radices large)
-  while (n % 16 == 0 && (n / 16) % 8 == 0) { P.r[ns++] = 16; n /= 16; }
-  while (n % 8 == 0) { P.r[ns++] = 8; n /= 8; }
-  if (n % 4 == 0) { P.r[ns++] = 4; n /= 4; }
-  if (n % 2 == 0) { P.r[ns++] = 2; n /= 2; }
+  int two = 0;
+  while (n % 2 == 0) {
+    n /= 2;
+    ++two;
+  }
+  // powers of two: radix 16 first (few stages, lcm 16); mixed lengths keep
+  // the power-of-two radices at most 8 so the radix lcm stays small
+  const int maxr = (n == 1) ? 16 : 8;
+  while (two >= 4 && maxr == 16) { P.r[ns++] = 16; two -= 4; }
+  while (two >= 3) { P.r[ns++] = 8; two -= 3; }
+  if (two == 2) P.r[ns++] = 4;
+  if (two == 1) P.r[ns++] = 2;
   while (n % 3 == 0) { P.r[ns++] = 3; n /= 3; }
   while (n % 5 == 0) { P.r[ns++] = 5; n /= 5; }
   while (n % 7 == 0) { P.r[ns++] = 7; n /= 7; }
@@ -197,6 +204,16 @@ constexpr Plan make_plan(int N) {
   P.twsize = off;
   P.lcm = l;
   return P;
+}
+
+// True when every stage's butterflies split evenly over TPT threads, so
+// thread p owns elements p + m TPT before the first and after the last stage.
+constexpr bool plan_exact(int N, int TPT) {
+  const Plan P = make_plan(N);
+  if (P.ns < 2) return false;
+  for (int s = 0; s < P.ns; ++s)
+    if ((N / P.r[s]) % TPT != 0) return false;
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -249,6 +266,72 @@ __device__ __forceinline__ void fft_lds(float2* line, int p, const float2* __res
   if constexpr (S < P.ns) {
     stockham_stage<N, P.r[S], P.lp[S], TPT, SE, INV>(line, p, tw + P.twoff[S]);
     fft_lds<N, TPT, SE, INV, S + 1>(line, p, tw);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident variant (exact plans): the first stage reads the
+// thread's registers, the last stage writes them; LDS only carries the
+// exchanges between stages.
+// ---------------------------------------------------------------------------
+template <int N, int TPT, int SE, bool INV, int S>
+__device__ __forceinline__ void reg_stage(float2 (&v)[N / TPT], float2* line, int p, const float2* __restrict__ tw) {
+  constexpr Plan P = make_plan(N);
+  constexpr int R = P.r[S], LP = P.lp[S], B = N / R, U = B / TPT;
+  constexpr bool FIRST = (S == 0), LAST = (S == P.ns - 1);
+  float2 w[U][R];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = p + u * TPT;
+    float2 x[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) x[t] = FIRST ? v[u + t * U] : line[(j + t * B) * SE];
+    if constexpr (LP > 1) {
+      const int k = j % LP;
+      const float2* wt = tw + P.twoff[S] + k * (R - 1);
+#pragma unroll
+      for (int t = 1; t < R; ++t) x[t] = INV ? cmulc(x[t], wt[t - 1]) : cmul(x[t], wt[t - 1]);
+    }
+    Dft<R, INV>::run(x);
+#pragma unroll
+    for (int q = 0; q < R; ++q) w[u][q] = x[q];
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[u + q * U] = w[u][q];
+  } else {
+    if constexpr (!FIRST) __syncthreads();  // this stage's reads precede its writes
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = p + u * TPT;
+      const int k = j % LP;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) line[(base + q * LP) * SE] = w[u][q];
+    }
+    __syncthreads();
+    reg_stage<N, TPT, SE, INV, S + 1>(v, line, p, tw);
+  }
+}
+
+// FFT of one line whose thread p holds elements p + m TPT in v[m] (m < N/TPT)
+// on entry and on exit.  Contract: the caller has made the LDS line free
+// (a barrier after its last use) and must barrier again before reusing it.
+template <int N, int TPT, int SE, bool INV>
+__device__ __forceinline__ void fft_line(float2 (&v)[N / TPT], float2* line, int p, const float2* __restrict__ tw) {
+  static_assert(N % TPT == 0, "threads per line must divide the length");
+  constexpr int E = N / TPT;
+  if constexpr (plan_exact(N, TPT)) {
+    reg_stage<N, TPT, SE, INV, 0>(v, line, p, tw);
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) line[(p + m * TPT) * SE] = v[m];
+    __syncthreads();
+    fft_lds<N, TPT, SE, INV>(line, p, tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = line[(p + m * TPT) * SE];
   }
 }
 

@@ -23,14 +23,34 @@ namespace cdn {
 // ---------------------------------------------------------------------------
 // Per-length launch configuration
 // ---------------------------------------------------------------------------
-// threads per line: largest divisor of N not above N/16 (about 16 elements
-// per thread)
+// threads per line for the all-LDS fallback: largest divisor of N not above
+// N/16 (about 16 elements per thread)
 constexpr int tpt_for(int N) {
   int t = N / 16;
   if (t < 1) t = 1;
   for (int d = t; d >= 1; --d)
     if (N % d == 0) return d;
   return 1;
+}
+
+// threads per line: elements per thread E = radix lcm (doubled up to 8 when
+// it is small) makes the plan exact, so lines stay in registers between the
+// first and last stages; lengths with a radix lcm above 32 use the fallback
+constexpr int tpt_col(int N) {
+  const Plan P = make_plan(N);
+  if (P.lcm <= 32) {
+    int E = P.lcm;
+    while (E < 8 && N % (E * 2) == 0) E *= 2;
+    return N / E;
+  }
+  return tpt_for(N);
+}
+
+// x passes want lines that fit in one wavefront (split/merge by shuffles)
+constexpr int tpt_x(int N) {
+  const int t = tpt_col(N);
+  if (t > 64 && t % 64 == 0 && plan_exact(N, 64)) return 64;
+  return t;
 }
 
 // columns per tile for y/z passes: fill the LDS budget, at most 64 columns
@@ -46,25 +66,27 @@ constexpr int lc_for(int N, int TPT) {
 }
 
 template <int N> struct XCfg {
-  static constexpr int TPT = tpt_for(N);
+  static constexpr int TPT = tpt_x(N);
+  static constexpr int E = N / TPT;  // elements per thread
+  static constexpr bool SHUFFLE = plan_exact(N, TPT) && TPT <= 64 && (64 % TPT) == 0 && (E % 2) == 0;
   static constexpr int NL0 = (256 / TPT) < 1 ? 1 : (256 / TPT);
   static constexpr int NL = (NL0 * N * 8 > 65536) ? ((65536 / (N * 8)) < 1 ? 1 : 65536 / (N * 8)) : NL0;
   static constexpr int PAD = TPT < 64 ? TPT : 0;  // spreads lines over LDS banks
   static constexpr int LS = N + PAD;
   static constexpr int THREADS = TPT * NL;
-  static constexpr int E = (N + TPT - 1) / TPT;      // elements per thread
-  static constexpr int EH = (N / 2 + TPT) / TPT;     // half-spectrum bins per thread
+  static constexpr int EH = (N / 2 + TPT) / TPT;  // half-spectrum bins per thread (LDS path)
 };
 
 template <int N> struct CCfg {
-  static constexpr int TPT = tpt_for(N);
+  static constexpr int TPT = tpt_col(N);
   static constexpr int L = lc_for(N, TPT);
   static constexpr int THREADS = TPT * L;
-  static constexpr int E = (N + TPT - 1) / TPT;
+  static constexpr int E = N / TPT;
 };
 
 // ---------------------------------------------------------------------------
-// x passes
+// x passes.  Thread p of a line holds v[m] = z[p + m TPT] with z = a + i b
+// for the two real rows r0, r0+1 packed into one complex line.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float load_y(int mode, const float* __restrict__ X, const float* __restrict__ Xp,
                                         float* __restrict__ Y, float lam, long idx) {
@@ -79,42 +101,100 @@ __device__ __forceinline__ float load_y(int mode, const float* __restrict__ X, c
   return X[idx];
 }
 
-// Half spectrum of two real rows from their packed transform in `line`.
-template <int N, int TPT, int EH>
-__device__ __forceinline__ void split_store(const float2* line, int p, float2* __restrict__ S, long r0, long pitch,
-                                            bool v0, bool v1) {
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// Half spectra A (row r0) and B (row r0+1) from the packed transform Z:
+// A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
+template <int N>
+__device__ __forceinline__ void split_store(float2 (&v)[XCfg<N>::E], float2* line, int p, float2* __restrict__ S,
+                                            long r0, long pitch, bool v0, bool v1) {
+  using C = XCfg<N>;
+  constexpr int E = C::E, TPT = C::TPT;
+  auto put = [&](int k, float2 zk, float2 zn) {
+    const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+    const float2 D = make_float2(0.5f * (zk.x - zn.x), 0.5f * (zk.y + zn.y));
+    if (v0) S[r0 * pitch + k] = A;
+    if (v1) S[(r0 + 1) * pitch + k] = make_float2(D.y, -D.x);
+  };
+  if constexpr (C::SHUFFLE) {
+    // element N-k of k = p + m TPT sits in lane (TPT-p) % TPT of this line,
+    // slot E-1-m (p > 0) or in this lane, slot (E-m) % E (p = 0)
+    const int lane = threadIdx.x & 63;
+    const int src = lane - p + ((TPT - p) % TPT);
 #pragma unroll
-  for (int e = 0; e < EH; ++e) {
-    const int k = p + e * TPT;
-    if (k <= N / 2) {
-      const float2 zk = line[k];
-      const float2 zn = line[k == 0 ? 0 : N - k];
-      const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-      const float2 D = make_float2(0.5f * (zk.x - zn.x), 0.5f * (zk.y + zn.y));
-      if (v0) S[r0 * pitch + k] = A;
-      if (v1) S[(r0 + 1) * pitch + k] = make_float2(D.y, -D.x);
+    for (int m = 0; m <= E / 2; ++m) {
+      const float2 t = shfl2(v[E - 1 - m], src);
+      const float2 zn = (p == 0) ? v[(E - m) % E] : t;
+      if (m < E / 2 || p == 0) put(p + m * TPT, v[m], zn);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) line[p + m * TPT] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < C::EH; ++e) {
+      const int k = p + e * TPT;
+      if (k <= N / 2) put(k, line[k], line[k == 0 ? 0 : N - k]);
     }
   }
 }
 
-// Packed line z = a + i b from the half spectra of rows r0, r0+1.
-template <int N, int TPT, int EH>
-__device__ __forceinline__ void merge_load(float2* line, int p, const float2* __restrict__ S, long r0, long pitch,
-                                           bool v0, bool v1) {
-#pragma unroll
-  for (int e = 0; e < EH; ++e) {
-    const int k = p + e * TPT;
-    if (k <= N / 2) {
-      float2 A = v0 ? S[r0 * pitch + k] : make_float2(0.f, 0.f);
-      float2 B = v1 ? S[(r0 + 1) * pitch + k] : make_float2(0.f, 0.f);
-      const bool edge = (k == 0) || (2 * k == N);
-      if (edge) {
-        A.y = 0.f;
-        B.y = 0.f;
-      }
-      line[k] = make_float2(A.x - B.y, A.y + B.x);
-      if (!edge) line[N - k] = make_float2(A.x + B.y, B.x - A.y);
+// Packed line Z = A + i B (Hermitian extension of both rows; imaginary parts
+// of the kx = 0 and kx = N/2 bins ignored) into v.
+template <int N>
+__device__ __forceinline__ void merge_load(float2 (&v)[XCfg<N>::E], float2* line, int p, const float2* __restrict__ S,
+                                           long r0, long pitch, bool v0, bool v1) {
+  using C = XCfg<N>;
+  constexpr int E = C::E, TPT = C::TPT;
+  auto ld = [&](int k, float2& A, float2& B) {
+    A = v0 ? S[r0 * pitch + k] : make_float2(0.f, 0.f);
+    B = v1 ? S[(r0 + 1) * pitch + k] : make_float2(0.f, 0.f);
+    if (k == 0 || 2 * k == N) {
+      A.y = 0.f;
+      B.y = 0.f;
     }
+  };
+  if constexpr (C::SHUFFLE) {
+    float2 A[E / 2 + 1], B[E / 2 + 1];
+#pragma unroll
+    for (int m = 0; m < E / 2; ++m) ld(p + m * TPT, A[m], B[m]);
+    A[E / 2] = make_float2(0.f, 0.f);
+    B[E / 2] = make_float2(0.f, 0.f);
+    if (p == 0) ld(N / 2, A[E / 2], B[E / 2]);
+#pragma unroll
+    for (int m = 0; m < E / 2; ++m) v[m] = make_float2(A[m].x - B[m].y, A[m].y + B[m].x);
+    const int lane = threadIdx.x & 63;
+    const int src = lane - p + ((TPT - p) % TPT);
+#pragma unroll
+    for (int m = E / 2; m < E; ++m) {
+      // Z[k] for k > N/2 is conj(A[N-k]) + i conj(B[N-k])
+      float2 a = shfl2(A[E - 1 - m], src), b = shfl2(B[E - 1 - m], src);
+      if (p == 0) {
+        a = A[E - m];
+        b = B[E - m];
+      }
+      if (p == 0 && m == E / 2)
+        v[m] = make_float2(a.x - b.y, a.y + b.x);  // the kx = N/2 bin itself
+      else
+        v[m] = make_float2(a.x + b.y, b.x - a.y);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < C::EH; ++e) {
+      const int k = p + e * TPT;
+      if (k <= N / 2) {
+        float2 A, B;
+        ld(k, A, B);
+        line[k] = make_float2(A.x - B.y, A.y + B.x);
+        if (k != 0 && 2 * k != N) line[N - k] = make_float2(A.x + B.y, B.x - A.y);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = line[p + m * TPT];
+    __syncthreads();
   }
 }
 
@@ -131,18 +211,41 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
   const float lamf = (mode == XIN_PRED) ? (float)lam[0] : 0.f;
+  float2 v[C::E];
+  // all loads are issued before any store
+  const float* src = (mode == XIN_SEED) ? Y : X;
+  float a[C::E], b[C::E];
 #pragma unroll
-  for (int e = 0; e < C::E; ++e) {
-    const int i = p + e * C::TPT;
-    if (i < N) {
-      const float a = v0 ? load_y(mode, X, Xp, Y, lamf, r0 * N + i) : 0.f;
-      const float b = v1 ? load_y(mode, X, Xp, Y, lamf, (r0 + 1) * N + i) : 0.f;
-      line[i] = make_float2(a, b);
+  for (int m = 0; m < C::E; ++m) {
+    const int i = p + m * C::TPT;
+    a[m] = v0 ? src[r0 * N + i] : 0.f;
+    b[m] = v1 ? src[(r0 + 1) * N + i] : 0.f;
+  }
+  if (mode == XIN_PRED) {
+    // updatePrediction_kernel (src/RLgpuImpl.cu:658-666)
+    float ap[C::E], bp[C::E];
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      const int i = p + m * C::TPT;
+      ap[m] = v0 ? Xp[r0 * N + i] : 0.f;
+      bp[m] = v1 ? Xp[(r0 + 1) * N + i] : 0.f;
+    }
+#pragma unroll
+    for (int m = 0; m < C::E; ++m) {
+      const int i = p + m * C::TPT;
+      float ya = a[m] + lamf * (a[m] - ap[m]);
+      float yb = b[m] + lamf * (b[m] - bp[m]);
+      a[m] = ya > 0.f ? ya : 0.f;
+      b[m] = yb > 0.f ? yb : 0.f;
+      if (v0) Y[r0 * N + i] = a[m];
+      if (v1) Y[(r0 + 1) * N + i] = b[m];
     }
   }
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) v[m] = make_float2(a[m], b[m]);
+  fft_line<N, C::TPT, 1, false>(v, line, p, tw);
   __syncthreads();
-  fft_lds<N, C::TPT, 1, false>(line, p, tw);
-  split_store<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
+  split_store<N>(v, line, p, S, r0, pitch, v0, v1);
 }
 
 template <int N>
@@ -156,17 +259,14 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
-  merge_load<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
-  __syncthreads();
-  fft_lds<N, C::TPT, 1, true>(line, p, tw);
+  float2 v[C::E];
+  merge_load<N>(v, line, p, S, r0, pitch, v0, v1);
+  fft_line<N, C::TPT, 1, true>(v, line, p, tw);
 #pragma unroll
-  for (int e = 0; e < C::E; ++e) {
-    const int i = p + e * C::TPT;
-    if (i < N) {
-      const float2 z = line[i];
-      if (v0) out[r0 * N + i] = z.x;
-      if (v1) out[(r0 + 1) * N + i] = z.y;
-    }
+  for (int m = 0; m < C::E; ++m) {
+    const int i = p + m * C::TPT;
+    if (v0) out[r0 * N + i] = v[m].x;
+    if (v1) out[(r0 + 1) * N + i] = v[m].y;
   }
 }
 
@@ -190,22 +290,20 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
-  merge_load<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
-  __syncthreads();
-  fft_lds<N, C::TPT, 1, true>(line, p, tw);
+  float2 v[C::E];
+  merge_load<N>(v, line, p, S, r0, pitch, v0, v1);
+  fft_line<N, C::TPT, 1, true>(v, line, p, tw);
 #pragma unroll
-  for (int e = 0; e < C::E; ++e) {
-    const int i = p + e * C::TPT;
-    if (i < N) {
-      const float2 z = line[i];
-      const float ra = v0 ? raw[r0 * N + i] : 0.f;
-      const float rb = v1 ? raw[(r0 + 1) * N + i] : 0.f;
-      line[i] = make_float2(lrcore(z.x, ra, eps), lrcore(z.y, rb, eps));
-    }
+  for (int m = 0; m < C::E; ++m) {
+    const int i = p + m * C::TPT;
+    const float ra = v0 ? raw[r0 * N + i] : 0.f;
+    const float rb = v1 ? raw[(r0 + 1) * N + i] : 0.f;
+    v[m] = make_float2(lrcore(v[m].x, ra, eps), lrcore(v[m].y, rb, eps));
   }
   __syncthreads();
-  fft_lds<N, C::TPT, 1, false>(line, p, tw);
-  split_store<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
+  fft_line<N, C::TPT, 1, false>(v, line, p, tw);
+  __syncthreads();
+  split_store<N>(v, line, p, S, r0, pitch, v0, v1);
 }
 
 // Deterministic workgroup sum of two doubles; out[0], out[1] written by
@@ -259,41 +357,53 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
-  merge_load<N, C::TPT, C::EH>(line, p, S, r0, pitch, v0, v1);
-  __syncthreads();
-  fft_lds<N, C::TPT, 1, true>(line, p, tw);
+  float2 v[C::E];
+  merge_load<N>(v, line, p, S, r0, pitch, v0, v1);
+  fft_line<N, C::TPT, 1, true>(v, line, p, tw);
+  // all loads are issued before any store (G is read, then overwritten)
+  float ya[C::E], yb[C::E], ga[C::E], gb[C::E];
+#pragma unroll
+  for (int m = 0; m < C::E; ++m) {
+    const int i = p + m * C::TPT;
+    ya[m] = v0 ? Y[r0 * N + i] : 0.f;
+    yb[m] = v1 ? Y[(r0 + 1) * N + i] : 0.f;
+    ga[m] = (dots && v0) ? G[r0 * N + i] : 0.f;
+    gb[m] = (dots && v1) ? G[(r0 + 1) * N + i] : 0.f;
+  }
   double num = 0.0, den = 0.0;
 #pragma unroll
-  for (int e = 0; e < C::E; ++e) {
-    const int i = p + e * C::TPT;
-    if (i < N) {
-      const float2 z = line[i];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const bool v = h ? v1 : v0;
-        if (v) {
-          const long idx = (r0 + h) * N + i;
-          const float c = h ? z.y : z.x;
-          const float y = Y[idx];
-          float x = c * y;  // currEstimate_kernel (src/RLgpuImpl.cu:462-470)
-          x = x > 0.f ? x : 0.f;
-          const float g = x - y;  // currPrevDiff_kernel (src/RLgpuImpl.cu:501-508)
-          if (dots) {  // innerProduct_kernel: FP32 products, FP64 sums (:583-592)
-            const float gp = G[idx];  // G_{k-1}, overwritten below by G_k
-            num += (double)(g * gp);
-            den += (double)(gp * gp);
-          }
-          Xout[idx] = x;
-          G[idx] = g;
-        }
+  for (int m = 0; m < C::E; ++m) {
+    const int i = p + m * C::TPT;
+    // currEstimate_kernel and currPrevDiff_kernel (src/RLgpuImpl.cu:462-508)
+    float xa = v[m].x * ya[m], xb = v[m].y * yb[m];
+    xa = xa > 0.f ? xa : 0.f;
+    xb = xb > 0.f ? xb : 0.f;
+    const float gna = xa - ya[m], gnb = xb - yb[m];
+    if (dots) {
+      // innerProduct_kernel: FP32 products, FP64 sums (src/RLgpuImpl.cu:583-592)
+      if (v0) {
+        num += (double)(gna * ga[m]);
+        den += (double)(ga[m] * ga[m]);
       }
+      if (v1) {
+        num += (double)(gnb * gb[m]);
+        den += (double)(gb[m] * gb[m]);
+      }
+    }
+    if (v0) {
+      Xout[r0 * N + i] = xa;
+      G[r0 * N + i] = gna;
+    }
+    if (v1) {
+      Xout[(r0 + 1) * N + i] = xb;
+      G[(r0 + 1) * N + i] = gnb;
     }
   }
   if (dots) block_sum2(num, den, red, part + 2 * (long)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
-// y / z column passes
+// y / z column passes: thread (l, p) holds column l's elements p + m TPT
 // ---------------------------------------------------------------------------
 template <int N, bool INV>
 __global__ void __launch_bounds__(CCfg<N>::THREADS)
@@ -307,20 +417,14 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
   float2* base = S + outer * outer_stride + c;
-  float2* line = lds + l;
+  float2 v[C::E];
 #pragma unroll
-  for (int e = 0; e < C::E; ++e) {
-    const int i = p + e * C::TPT;
-    if (i < N) line[i * C::L] = valid ? base[i * line_stride] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  fft_lds<N, C::TPT, C::L, INV>(line, p, tw);
+  for (int m = 0; m < C::E; ++m)
+    v[m] = valid ? base[(long)(p + m * C::TPT) * line_stride] : make_float2(0.f, 0.f);
+  fft_line<N, C::TPT, C::L, INV>(v, lds + l, p, tw);
   if (valid) {
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) {
-      const int i = p + e * C::TPT;
-      if (i < N) base[i * line_stride] = line[i * C::L];
-    }
+    for (int m = 0; m < C::E; ++m) base[(long)(p + m * C::TPT) * line_stride] = v[m];
   }
 }
 
@@ -338,14 +442,11 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
   float2* base = S + outer * outer_stride + c;
-  float2* line = lds + l;
+  float2 v[C::E];
 #pragma unroll
-  for (int e = 0; e < C::E; ++e) {
-    const int i = p + e * C::TPT;
-    if (i < N) line[i * C::L] = valid ? base[i * line_stride] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  fft_lds<N, C::TPT, C::L, false>(line, p, tw);
+  for (int m = 0; m < C::E; ++m)
+    v[m] = valid ? base[(long)(p + m * C::TPT) * line_stride] : make_float2(0.f, 0.f);
+  fft_line<N, C::TPT, C::L, false>(v, lds + l, p, tw);
 
   // signed ky (k > n>>1 -> k - n), scaled k coordinates
   const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
@@ -360,18 +461,16 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
     const float2* t0 = otf.ra_tab.tz + (long)ir * N;
     const float2* t1 = t0 + N;
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) {
-      const int i = p + e * C::TPT;
-      if (i < N) {
-        float2 v = make_float2(0.f, 0.f);
-        if (inside) {
-          const float2 a = t0[i], b = t1[i];
-          float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
-          h.y *= sg;
-          v = cscale(cmul(h, line[i * C::L]), otf.inv_n);
-        }
-        line[i * C::L] = v;
+    for (int m = 0; m < C::E; ++m) {
+      const int i = p + m * C::TPT;
+      float2 r = make_float2(0.f, 0.f);
+      if (inside) {
+        const float2 a = t0[i], b = t1[i];
+        float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
+        h.y *= sg;
+        r = cscale(cmul(h, v[m]), otf.inv_n);
       }
+      v[m] = r;
     }
   } else {
     const Otf3d& T = otf.tab3;
@@ -385,45 +484,40 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
     const float ax = kx - (float)ix, ay = kyindex - (float)iy;
     const long nxy = (long)T.nxotf * T.nyotf;
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) {
-      const int i = p + e * C::TPT;
-      if (i < N) {
-        const int kzi = (i > (N >> 1)) ? i - N : i;
-        const float kz = (float)kzi * otf.kzscale;
-        const float kzindex = kz < 0.f ? kz + (float)T.nzotf : kz;
-        float2 v = make_float2(0.f, 0.f);
-        if (inxy && kzindex < (float)T.nzotf && kzindex >= 0.f) {
-          const int iz = (int)floorf(kzindex);
-          const int iz1 = (iz + 1) % T.nzotf;
-          const float az = kzindex - (float)iz;
-          auto g = [&](int zz, int yy, int xx) { return T.raw[zz * nxy + (long)yy * T.nxotf + xx]; };
-          const float2 a000 = g(iz, iy, ix), a001 = g(iz, iy, ix + 1), a010 = g(iz, iy1, ix),
-                       a011 = g(iz, iy1, ix + 1), a100 = g(iz1, iy, ix), a101 = g(iz1, iy, ix + 1),
-                       a110 = g(iz1, iy1, ix), a111 = g(iz1, iy1, ix + 1);
-          float2 h;
-          h.x = (1 - az) * (a000.x * (1 - ay) * (1 - ax) + a001.x * (1 - ay) * ax + a010.x * ay * (1 - ax) +
-                            a011.x * ay * ax) +
-                az * (a100.x * (1 - ay) * (1 - ax) + a101.x * (1 - ay) * ax + a110.x * ay * (1 - ax) +
-                      a111.x * ay * ax);
-          h.y = (1 - az) * (a000.y * (1 - ay) * (1 - ax) + a001.y * (1 - ay) * ax + a010.y * ay * (1 - ax) +
-                            a011.y * ay * ax) +
-                az * (a100.y * (1 - ay) * (1 - ax) + a101.y * (1 - ay) * ax + a110.y * ay * (1 - ax) +
-                      a111.y * ay * ax);
-          h.y *= sg;
-          v = cscale(cmul(h, line[i * C::L]), otf.inv_n);
-        }
-        line[i * C::L] = v;
+    for (int m = 0; m < C::E; ++m) {
+      const int i = p + m * C::TPT;
+      const int kzi = (i > (N >> 1)) ? i - N : i;
+      const float kz = (float)kzi * otf.kzscale;
+      const float kzindex = kz < 0.f ? kz + (float)T.nzotf : kz;
+      float2 r = make_float2(0.f, 0.f);
+      if (inxy && kzindex < (float)T.nzotf && kzindex >= 0.f) {
+        const int iz = (int)floorf(kzindex);
+        const int iz1 = (iz + 1) % T.nzotf;
+        const float az = kzindex - (float)iz;
+        auto g = [&](int zz, int yy, int xx) { return T.raw[zz * nxy + (long)yy * T.nxotf + xx]; };
+        const float2 a000 = g(iz, iy, ix), a001 = g(iz, iy, ix + 1), a010 = g(iz, iy1, ix),
+                     a011 = g(iz, iy1, ix + 1), a100 = g(iz1, iy, ix), a101 = g(iz1, iy, ix + 1),
+                     a110 = g(iz1, iy1, ix), a111 = g(iz1, iy1, ix + 1);
+        float2 h;
+        h.x = (1 - az) * (a000.x * (1 - ay) * (1 - ax) + a001.x * (1 - ay) * ax + a010.x * ay * (1 - ax) +
+                          a011.x * ay * ax) +
+              az * (a100.x * (1 - ay) * (1 - ax) + a101.x * (1 - ay) * ax + a110.x * ay * (1 - ax) +
+                    a111.x * ay * ax);
+        h.y = (1 - az) * (a000.y * (1 - ay) * (1 - ax) + a001.y * (1 - ay) * ax + a010.y * ay * (1 - ax) +
+                          a011.y * ay * ax) +
+              az * (a100.y * (1 - ay) * (1 - ax) + a101.y * (1 - ay) * ax + a110.y * ay * (1 - ax) +
+                    a111.y * ay * ax);
+        h.y *= sg;
+        r = cscale(cmul(h, v[m]), otf.inv_n);
       }
+      v[m] = r;
     }
   }
   __syncthreads();
-  fft_lds<N, C::TPT, C::L, true>(line, p, tw);
+  fft_line<N, C::TPT, C::L, true>(v, lds + l, p, tw);
   if (valid) {
 #pragma unroll
-    for (int e = 0; e < C::E; ++e) {
-      const int i = p + e * C::TPT;
-      if (i < N) base[i * line_stride] = line[i * C::L];
-    }
+    for (int m = 0; m < C::E; ++m) base[(long)(p + m * C::TPT) * line_stride] = v[m];
   }
 }
 

@@ -187,26 +187,48 @@ void zblend(hipStream_t st, float* img, int nx, int ny, int nz, int nZblend) {
 // [0, 1] at src/RL-Biggs-Andrews.cpp:297-298), kept on the device:
 // lam[0] = clamp(sum num / (sum den + eps)), lam[1..2] = the two sums.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BS) k_lambda(const double* __restrict__ part, int nb, double eps, double* lam) {
-  __shared__ double tot[2];
-  double a = 0, b = 0;
-  for (int i = threadIdx.x; i < nb; i += BS) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
+static constexpr int LBS = 1024;  // threads of the acceleration-factor kernel
+__global__ void __launch_bounds__(LBS) k_lambda(const double2* __restrict__ part, int nb, double eps, double* lam) {
+  __shared__ double ra[LBS / 64], rb[LBS / 64];
+  // 4 independent accumulators per thread keep several loads in flight
+  double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+  int i = threadIdx.x;
+  for (; i + 3 * LBS < nb; i += 4 * LBS) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 v = part[i + u * LBS];
+      a[u] += v.x;
+      b[u] += v.y;
+    }
   }
-  block_reduce2(a, b, tot);
+  for (; i < nb; i += LBS) {
+    const double2 v = part[i];
+    a[0] += v.x;
+    b[0] += v.y;
+  }
+  double sa = wave_sum((a[0] + a[1]) + (a[2] + a[3]));
+  double sb = wave_sum((b[0] + b[1]) + (b[2] + b[3]));
+  if ((threadIdx.x & 63) == 0) {
+    ra[threadIdx.x >> 6] = sa;
+    rb[threadIdx.x >> 6] = sb;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double l = tot[0] / (tot[1] + eps);
+    double ta = 0, tb = 0;
+    for (int w = 0; w < LBS / 64; ++w) {
+      ta += ra[w];
+      tb += rb[w];
+    }
+    double l = ta / (tb + eps);
     l = l < 1.0 ? l : 1.0;
     l = l > 0.0 ? l : 0.0;
     lam[0] = l;
-    lam[1] = tot[0];
-    lam[2] = tot[1];
+    lam[1] = ta;
+    lam[2] = tb;
   }
 }
 void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam) {
-  hipLaunchKernelGGL(k_lambda, dim3(1), dim3(BS), 0, st, part, nb, eps, lam);
+  hipLaunchKernelGGL(k_lambda, dim3(1), dim3(LBS), 0, st, reinterpret_cast<const double2*>(part), nb, eps, lam);
   CDN_HIP(hipGetLastError());
 }
 
