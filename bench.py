@@ -42,6 +42,18 @@ def pass_bytes(R: float, P: float) -> dict:
     }
 
 
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 PMC runs and
+    committed in profiles/rl_iteration_t256_pmc.md (JSON block), else None."""
+    path = os.path.join(ROOT, "profiles", "rl_iteration_t256_pmc.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)["traffic_bytes_per_launch"].get(kernel)
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int):
     """FP32 oracle RL-BA (OpenMP) on the first `slab` z-planes of the stack."""
     import oracle as orc
@@ -119,6 +131,7 @@ def main():
     eng.timing(1)
     eng.iterate(args.pass_iters, k0=args.warmup + args.steps)
     pms, calls = eng.timing(0)
+    copy_ms = eng.copy_probe(5)
     eng.close()
 
     Rb = 4.0 * nvox
@@ -166,6 +179,8 @@ def main():
             "GBps": round(iter_gbps, 1),
             "frac_peak": round(iter_gbps / HBM_PEAK_GBPS, 4),
             "device_ms_per_step": round(dev_ms / args.steps, 4),
+            "stream_copy_GBps": round(2 * 4.0 * nvox / (copy_ms * 1e-3) / 1e9, 1),
+            "frac_of_stream_copy": round(iter_gbps / (2 * 4.0 * nvox / (copy_ms * 1e-3) / 1e9), 4),
         },
         "roofline": {
             "kernel": dom,
@@ -174,7 +189,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": passes[dom]["frac_peak"],
-            "traffic": None,
+            "traffic": pmc_traffic(dom),
         },
         "passes": passes,
         "setup_s": round(gen_s, 1),

@@ -72,6 +72,7 @@ SIGNATURES = {
     "cudaDecon_rl_get": (C.c_int, [C.c_void_p, _f]),
     "cudaDecon_rl_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "cudaDecon_rl_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cudaDecon_rl_copy_probe": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cudaDecon_rl_destroy": (None, [C.c_void_p]),
     "cudaDecon_device_sync": (C.c_int, []),
 }
@@ -284,6 +285,12 @@ class RLEngine:
         _check(lib().cudaDecon_rl_geometry(self.h, C.byref(v, 0), C.byref(v, 8), C.byref(v, 16)),
                "cudaDecon_rl_geometry")
         return int(v[0]), int(v[1]), int(v[2])
+
+    def copy_probe(self, n=10):
+        """Average ms of one plain streaming copy of the stack (read + write)."""
+        ms = np.zeros(1, np.float64)
+        _check(lib().cudaDecon_rl_copy_probe(self.h, int(n), ms.ctypes.data), "cudaDecon_rl_copy_probe")
+        return float(ms[0])
 
     def close(self):
         if self.h:

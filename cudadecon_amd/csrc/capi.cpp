@@ -599,6 +599,26 @@ int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pi
   });
 }
 
+int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms) {
+  return guard([&]() -> int {
+    if (!h || n <= 0) fail("cudaDecon_rl_copy_probe: bad argument");
+    const size_t bytes = sizeof(float) * (size_t)h->eng.fft.nvox / 16 * 16;
+    hipEvent_t e0, e1;
+    CDN_HIP(hipEventCreate(&e0));
+    CDN_HIP(hipEventCreate(&e1));
+    CDN_HIP(hipEventRecord(e0, h->st));
+    for (int i = 0; i < n; ++i) stream_copy(h->st, h->eng.raw.p, h->eng.y.p, bytes);
+    CDN_HIP(hipEventRecord(e1, h->st));
+    CDN_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    CDN_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t / n;
+    return 1;
+  });
+}
+
 void cudaDecon_rl_destroy(cudaDecon_rl* h) { delete h; }
 
 }  // extern "C"

@@ -38,6 +38,37 @@ template <int S> __device__ __forceinline__ float2 cmuli(float2 a) {
   return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// Streaming global accesses: NT selects non-temporal instructions for data
+// that a pass reads or writes exactly once (measured per kernel family).
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <bool NT> __device__ __forceinline__ float2 ld_stream(const float2* p) {
+  if constexpr (NT) {
+    const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+    return make_float2(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT> __device__ __forceinline__ float ld_stream(const float* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT> __device__ __forceinline__ void st_stream(float2* p, float2 v) {
+  if constexpr (NT) {
+    f2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+  } else {
+    *p = v;
+  }
+}
+template <bool NT> __device__ __forceinline__ void st_stream(float* p, float v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Sum over a wavefront of 64 lanes (doubles), result valid in every lane.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -20,6 +20,24 @@
 
 namespace cdn {
 
+// Non-temporal policy per kernel family (measured at 256 x 1024 x 1024):
+// x passes and the z filter stream both ways; plain y/z column transforms
+// keep cached loads and stream their stores.
+#ifndef CDN_NT_X
+#define CDN_NT_X 1
+#endif
+#ifndef CDN_NT_Z
+#define CDN_NT_Z 1
+#endif
+#ifndef CDN_NT_COL_LOAD
+#define CDN_NT_COL_LOAD 0
+#endif
+#ifndef CDN_NT_COL_STORE
+#define CDN_NT_COL_STORE 1
+#endif
+constexpr bool NTX = CDN_NT_X != 0, NTZ = CDN_NT_Z != 0;
+constexpr bool NTCL = CDN_NT_COL_LOAD != 0, NTCS = CDN_NT_COL_STORE != 0;
+
 // ---------------------------------------------------------------------------
 // Per-length launch configuration
 // ---------------------------------------------------------------------------
@@ -53,9 +71,27 @@ constexpr int tpt_x(int N) {
   return t;
 }
 
+// Build-time tile overrides (pairs "length, value"): columns per y/z tile and
+// threads per x-pass workgroup.  Used to sweep launch shapes.
+#ifndef CDN_COLS_OVERRIDE
+#define CDN_COLS_OVERRIDE 0, 0
+#endif
+#ifndef CDN_XTHREADS_OVERRIDE
+#define CDN_XTHREADS_OVERRIDE 0, 0
+#endif
+constexpr int kColsOverride[] = {CDN_COLS_OVERRIDE};
+constexpr int kXThreadsOverride[] = {CDN_XTHREADS_OVERRIDE};
+template <int K>
+constexpr int lookup_override(const int (&tab)[K], int N) {
+  for (int i = 0; i + 1 < K; i += 2)
+    if (tab[i] == N) return tab[i + 1];
+  return 0;
+}
+
 // columns per tile for y/z passes: fill the LDS budget, at most 64 columns
 // and 1024 threads, power of two
 constexpr int lc_for(int N, int TPT) {
+  if (lookup_override(kColsOverride, N) > 0) return lookup_override(kColsOverride, N);
   const int budget = (N >= 768) ? 131072 : 65536;
   int L = budget / (N * 8);
   if (L > 64) L = 64;
@@ -69,7 +105,8 @@ template <int N> struct XCfg {
   static constexpr int TPT = tpt_x(N);
   static constexpr int E = N / TPT;  // elements per thread
   static constexpr bool SHUFFLE = plan_exact(N, TPT) && TPT <= 64 && (64 % TPT) == 0 && (E % 2) == 0;
-  static constexpr int NL0 = (256 / TPT) < 1 ? 1 : (256 / TPT);
+  static constexpr int XT = lookup_override(kXThreadsOverride, N) > 0 ? lookup_override(kXThreadsOverride, N) : 128;
+  static constexpr int NL0 = (XT / TPT) < 1 ? 1 : (XT / TPT);
   static constexpr int NL = (NL0 * N * 8 > 65536) ? ((65536 / (N * 8)) < 1 ? 1 : 65536 / (N * 8)) : NL0;
   static constexpr int PAD = TPT < 64 ? TPT : 0;  // spreads lines over LDS banks
   static constexpr int LS = N + PAD;
@@ -115,8 +152,8 @@ __device__ __forceinline__ void split_store(float2 (&v)[XCfg<N>::E], float2* lin
   auto put = [&](int k, float2 zk, float2 zn) {
     const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
     const float2 D = make_float2(0.5f * (zk.x - zn.x), 0.5f * (zk.y + zn.y));
-    if (v0) S[r0 * pitch + k] = A;
-    if (v1) S[(r0 + 1) * pitch + k] = make_float2(D.y, -D.x);
+    if (v0) st_stream<NTX>(S + r0 * pitch + k, A);
+    if (v1) st_stream<NTX>(S + (r0 + 1) * pitch + k, make_float2(D.y, -D.x));
   };
   if constexpr (C::SHUFFLE) {
     // element N-k of k = p + m TPT sits in lane (TPT-p) % TPT of this line,
@@ -149,8 +186,8 @@ __device__ __forceinline__ void merge_load(float2 (&v)[XCfg<N>::E], float2* line
   using C = XCfg<N>;
   constexpr int E = C::E, TPT = C::TPT;
   auto ld = [&](int k, float2& A, float2& B) {
-    A = v0 ? S[r0 * pitch + k] : make_float2(0.f, 0.f);
-    B = v1 ? S[(r0 + 1) * pitch + k] : make_float2(0.f, 0.f);
+    A = v0 ? ld_stream<NTX>(S + r0 * pitch + k) : make_float2(0.f, 0.f);
+    B = v1 ? ld_stream<NTX>(S + (r0 + 1) * pitch + k) : make_float2(0.f, 0.f);
     if (k == 0 || 2 * k == N) {
       A.y = 0.f;
       B.y = 0.f;
@@ -218,8 +255,8 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
     const int i = p + m * C::TPT;
-    a[m] = v0 ? src[r0 * N + i] : 0.f;
-    b[m] = v1 ? src[(r0 + 1) * N + i] : 0.f;
+    a[m] = v0 ? ld_stream<NTX>(src + r0 * N + i) : 0.f;
+    b[m] = v1 ? ld_stream<NTX>(src + (r0 + 1) * N + i) : 0.f;
   }
   if (mode == XIN_PRED) {
     // updatePrediction_kernel (src/RLgpuImpl.cu:658-666)
@@ -227,8 +264,8 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
 #pragma unroll
     for (int m = 0; m < C::E; ++m) {
       const int i = p + m * C::TPT;
-      ap[m] = v0 ? Xp[r0 * N + i] : 0.f;
-      bp[m] = v1 ? Xp[(r0 + 1) * N + i] : 0.f;
+      ap[m] = v0 ? ld_stream<NTX>(Xp + r0 * N + i) : 0.f;
+      bp[m] = v1 ? ld_stream<NTX>(Xp + (r0 + 1) * N + i) : 0.f;
     }
 #pragma unroll
     for (int m = 0; m < C::E; ++m) {
@@ -237,8 +274,8 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
       float yb = b[m] + lamf * (b[m] - bp[m]);
       a[m] = ya > 0.f ? ya : 0.f;
       b[m] = yb > 0.f ? yb : 0.f;
-      if (v0) Y[r0 * N + i] = a[m];
-      if (v1) Y[(r0 + 1) * N + i] = b[m];
+      if (v0) st_stream<NTX>(Y + r0 * N + i, a[m]);
+      if (v1) st_stream<NTX>(Y + (r0 + 1) * N + i, b[m]);
     }
   }
 #pragma unroll
@@ -296,8 +333,8 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
     const int i = p + m * C::TPT;
-    const float ra = v0 ? raw[r0 * N + i] : 0.f;
-    const float rb = v1 ? raw[(r0 + 1) * N + i] : 0.f;
+    const float ra = v0 ? ld_stream<NTX>(raw + r0 * N + i) : 0.f;
+    const float rb = v1 ? ld_stream<NTX>(raw + (r0 + 1) * N + i) : 0.f;
     v[m] = make_float2(lrcore(v[m].x, ra, eps), lrcore(v[m].y, rb, eps));
   }
   __syncthreads();
@@ -365,10 +402,10 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
     const int i = p + m * C::TPT;
-    ya[m] = v0 ? Y[r0 * N + i] : 0.f;
-    yb[m] = v1 ? Y[(r0 + 1) * N + i] : 0.f;
-    ga[m] = (dots && v0) ? G[r0 * N + i] : 0.f;
-    gb[m] = (dots && v1) ? G[(r0 + 1) * N + i] : 0.f;
+    ya[m] = v0 ? ld_stream<NTX>(Y + r0 * N + i) : 0.f;
+    yb[m] = v1 ? ld_stream<NTX>(Y + (r0 + 1) * N + i) : 0.f;
+    ga[m] = (dots && v0) ? ld_stream<NTX>(G + r0 * N + i) : 0.f;
+    gb[m] = (dots && v1) ? ld_stream<NTX>(G + (r0 + 1) * N + i) : 0.f;
   }
   double num = 0.0, den = 0.0;
 #pragma unroll
@@ -391,12 +428,12 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
       }
     }
     if (v0) {
-      Xout[r0 * N + i] = xa;
-      G[r0 * N + i] = gna;
+      st_stream<NTX>(Xout + r0 * N + i, xa);
+      st_stream<NTX>(G + r0 * N + i, gna);
     }
     if (v1) {
-      Xout[(r0 + 1) * N + i] = xb;
-      G[(r0 + 1) * N + i] = gnb;
+      st_stream<NTX>(Xout + (r0 + 1) * N + i, xb);
+      st_stream<NTX>(G + (r0 + 1) * N + i, gnb);
     }
   }
   if (dots) block_sum2(num, den, red, part + 2 * (long)blockIdx.x);
@@ -420,11 +457,11 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   float2 v[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m)
-    v[m] = valid ? base[(long)(p + m * C::TPT) * line_stride] : make_float2(0.f, 0.f);
+    v[m] = valid ? ld_stream<NTCL>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
   fft_line<N, C::TPT, C::L, INV>(v, lds + l, p, tw);
   if (valid) {
 #pragma unroll
-    for (int m = 0; m < C::E; ++m) base[(long)(p + m * C::TPT) * line_stride] = v[m];
+    for (int m = 0; m < C::E; ++m) st_stream<NTCS>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
   }
 }
 
@@ -445,7 +482,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   float2 v[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m)
-    v[m] = valid ? base[(long)(p + m * C::TPT) * line_stride] : make_float2(0.f, 0.f);
+    v[m] = valid ? ld_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
   fft_line<N, C::TPT, C::L, false>(v, lds + l, p, tw);
 
   // signed ky (k > n>>1 -> k - n), scaled k coordinates
@@ -517,7 +554,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   fft_line<N, C::TPT, C::L, true>(v, lds + l, p, tw);
   if (valid) {
 #pragma unroll
-    for (int m = 0; m < C::E; ++m) base[(long)(p + m * C::TPT) * line_stride] = v[m];
+    for (int m = 0; m < C::E; ++m) st_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
   }
 }
 

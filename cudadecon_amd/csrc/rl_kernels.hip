@@ -233,6 +233,35 @@ void accel_factor(hipStream_t st, const double* part, int nb, double eps, double
 }
 
 // ---------------------------------------------------------------------------
+// Plain streaming copy (16-byte non-temporal vectors, grid-stride): the bandwidth
+// reference the iteration's passes are compared with.
+// ---------------------------------------------------------------------------
+#ifndef CDN_COPY_UNROLL
+#define CDN_COPY_UNROLL 1
+#endif
+__global__ void __launch_bounds__(BS) k_copy16(const float4* __restrict__ in, float4* __restrict__ out, long n4) {
+  constexpr int U = CDN_COPY_UNROLL;
+  const long stride = (long)gridDim.x * BS;
+  long i = (long)blockIdx.x * BS + threadIdx.x;
+  const f4v* src = reinterpret_cast<const f4v*>(in);
+  f4v* dst = reinterpret_cast<f4v*>(out);
+  for (; i + (U - 1) * stride < n4; i += U * stride) {
+    f4v v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < U; ++u) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+  }
+  for (; i < n4; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes) {
+  const long n4 = (long)(bytes / 16);
+  hipLaunchKernelGGL(k_copy16, dim3(nblocks(n4, 4 * CDN_COPY_UNROLL)), dim3(BS), 0, st, (const float4*)in,
+                     (float4*)out, n4);
+  CDN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Geometry
 // ---------------------------------------------------------------------------
 // deskew_kernel (src/geometryTransform.cu:4-31): xin evaluated in double,

@@ -127,6 +127,9 @@ int cudaDecon_rl_get(cudaDecon_rl* h, float* X);
 int cudaDecon_rl_timing(cudaDecon_rl* h, int enable, double* pass_ms, long* pass_calls);
 /* bytes of one half-spectrum row pitch (complex elements) and volume sizes */
 int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pitch);
+/* Plain 16-byte streaming copy of the stack into a scratch volume, n times;
+ * *ms receives the average device time of one copy (bandwidth reference). */
+int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms);
 void cudaDecon_rl_destroy(cudaDecon_rl* h);
 
 /* Synchronise the current device. */
