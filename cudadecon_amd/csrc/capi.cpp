@@ -175,8 +175,7 @@ int RL_interface_init(int nx, int ny, int nz, float dr, float dz, float dr_psf, 
       if (!fft_length_supported((int)n))
         fail("RL_interface_init: FFT length " + std::to_string(n) + " is not compiled into libcudaDecon");
     S->fft_nx = fft_nx;
-    S->st = make_stream();
-    g_output_nz = S->output_nz;
+    g_output_nz = S->output_nz;  // the stream is created on first use
     g_s = std::move(S);
     return 1;
   });
@@ -233,6 +232,7 @@ int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, f
     if (!g_s) fail("RL_interface called before RL_interface_init");
     Session& S = *g_s;
     if (!raw_data || !result) fail("RL_interface: null buffer");
+    if (!S.st) S.st = make_stream();
     if ((unsigned)nx < S.output_nx || (unsigned)ny < S.output_ny || (unsigned)nz < S.output_nz)
       fail("RL_interface: stack smaller than the initialised size");
     // the reference indexes outside the stack for these (apodize and z-blend

@@ -103,3 +103,54 @@ def test_init_rejects_unsupported_length(cdn, tmp_path):
     rc = L.RL_interface_init(1001, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, str(p).encode())
     assert rc == 0
     assert "1000" in cdn.last_error()
+
+
+# RL_interface_init scalars: library against the oracle's restatement of
+# src/RL-Biggs-Andrews.cpp:531-655 (no GPU work happens before RL_interface)
+INIT_CASES = [
+    # nx, ny, nz, dr, dz, dr_psf, dz_psf, angle, rot, width, skewed
+    (128, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False),
+    (512, 512, 128, 0.104, 0.3, 0.104, 0.15675, 31.5, 31.5, 0, False),
+    (100, 128, 64, 0.104, 0.3, 0.104, 0.3, -32.8, -32.8, 256, False),
+    (100, 128, 64, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, 256, False),
+    (130, 130, 66, 0.1, 0.25, 0.1, 0.25, 0.0, 15.0, 0, False),  # crops to 128 x 128 x 64
+    (1026, 520, 260, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False),  # crops to 1024 x 512 x 256
+    (512, 512, 128, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, 0, True),
+]
+
+
+@pytest.mark.parametrize("case", INIT_CASES, ids=[str(i) for i in range(len(INIT_CASES))])
+def test_init_geometry_matches_oracle(cdn, orc, tmp_path, case):
+    import numpy as np
+    from cudadecon_amd.otf import write_otf_tiff
+    nx, ny, nz, dr, dz, drp, dzp, ang, rot, width, skewed = case
+    tab = np.ones((17, 33), np.complex64)
+    p = tmp_path / "otf.tif"
+    write_otf_tiff(str(p), tab)
+    L = cdn.lib()
+    L.RL_cleanup()
+    rc = L.RL_interface_init(nx, ny, nz, dr, dz, drp, dzp, ang, rot, width, skewed, str(p).encode())
+    assert rc == 1, cdn.last_error()
+    g = orc.init_geometry(nx, ny, nz, dr, dz, drp, dzp, ang, rot, width, skewed, (66, 17, 1))
+    assert (L.get_output_nx(), L.get_output_ny(), L.get_output_nz()) == g["out_dims"]
+    assert L.cudaDecon_get_session_dims(1) == g["deskewed_nx"]
+    assert L.cudaDecon_get_session_dims(2) == g["output_nx"]
+    assert L.cudaDecon_get_session_dims(3) == g["fft_dims"][0]
+    ks = cdn.session_kscale()
+    assert all(np.float32(a) == np.float32(b) for a, b in zip(ks, g["kscale"])), (ks, g["kscale"])
+    L.RL_cleanup()
+
+
+def test_negative_rotation_sizes_use_abs(cdn, orc, tmp_path):
+    """get_output_nx of the reference omits fabs(p[1]) (src/RL-Biggs-Andrews.cpp:479)
+    while the result is sized with it (:411); this library uses fabs in both."""
+    import numpy as np
+    from cudadecon_amd.otf import write_otf_tiff
+    p = tmp_path / "otf.tif"
+    write_otf_tiff(str(p), np.ones((17, 33), np.complex64))
+    L = cdn.lib()
+    assert L.RL_interface_init(100, 128, 64, 0.104, 0.3, 0.104, 0.3, -31.5, -31.5, 0, False, str(p).encode()) == 1
+    g = orc.init_geometry(100, 128, 64, 0.104, 0.3, 0.104, 0.3, -31.5, -31.5, 0, False, (66, 17, 1))
+    assert g["rot"][1] < 0
+    assert L.get_output_nx() == g["out_dims"][0] > 0
+    L.RL_cleanup()
