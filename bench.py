@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--cpu-slab", type=int, default=64)
+    ap.add_argument("--no-e2e", action="store_true", help="skip the RL_interface end-to-end timing")
+    ap.add_argument("--e2e-iters", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1 "
@@ -133,6 +135,21 @@ def main():
     pms, calls = eng.timing(0)
     copy_ms = eng.copy_probe(5)
     eng.close()
+
+    # the drop-in entry point on host buffers (PCIe-inclusive, not the value)
+    e2e = None
+    if not args.no_e2e:
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "otf.tif")
+            write_otf_tiff(p, tab)
+            cdn.rl_init(raw.shape, p, dzdata=c["dz"], dxdata=c["dr"], dzpsf=c["dz"], dxpsf=c["dr"])
+            cdn.rl_decon(raw, background=100.0, n_iters=1)  # allocates the session's buffers
+            t0 = time.perf_counter()
+            cdn.rl_decon(raw, background=100.0, n_iters=args.e2e_iters)
+            e2e_s = time.perf_counter() - t0
+            cdn.rl_cleanup()
+        e2e = {"entry": "RL_interface", "iterations": args.e2e_iters, "seconds": round(e2e_s, 4),
+               "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
     Rb = 4.0 * nvox
     Pb = 8.0 * nspec
@@ -192,6 +209,7 @@ def main():
             "traffic": pmc_traffic(dom),
         },
         "passes": passes,
+        "rl_interface_e2e": e2e,
         "setup_s": round(gen_s, 1),
     }
     if rank == 0 and not args.no_cpu_baseline:
