@@ -171,9 +171,13 @@ int RL_interface_init(int nx, int ny, int nz, float dr, float dz, float dr_psf, 
     S->dr = dr;
     S->dz = dz;
     const unsigned fft_nx = bSkewedDecon ? S->output_nx : S->deskewedXdim;
-    for (unsigned n : {fft_nx, S->output_ny, S->output_nz})
+    // precompiled lengths need nothing; others are compiled now (like a
+    // cuFFT plan, src/RL-Biggs-Andrews.cpp:622-631), without touching the GPU
+    for (unsigned n : {fft_nx, S->output_ny, S->output_nz}) {
       if (!fft_length_supported((int)n))
-        fail("RL_interface_init: FFT length " + std::to_string(n) + " is not compiled into libcudaDecon");
+        fail("RL_interface_init: FFT length " + std::to_string(n) + " is not supported (2,3,5,7-smooth, at most 16384)");
+      fft_prepare_length((int)n);
+    }
     S->fft_nx = fft_nx;
     g_output_nz = S->output_nz;  // the stream is created on first use
     g_s = std::move(S);

@@ -39,9 +39,11 @@ static void upload_twiddles(DevBuf& b, int n, hipStream_t st) {
 }
 
 void Fft3::setup(int x, int y, int z, hipStream_t st) {
-  for (int n : {x, y, z})
+  for (int n : {x, y, z}) {
     if (!fft_length_supported(n))
-      fail("FFT length " + std::to_string(n) + " is not among the lengths compiled into libcudaDecon");
+      fail("FFT length " + std::to_string(n) + " is not supported (2,3,5,7-smooth, at most 16384)");
+    fft_prepare_length(n);
+  }
   nx = x;
   ny = y;
   nz = z;

@@ -15,7 +15,7 @@
 // registers across the barrier.
 #pragma once
 
-#include "common.hpp"
+#include "common_dev.hpp"
 
 namespace cdn {
 

@@ -1,562 +1,16 @@
-// 3-D FFT pass kernels for the RL-Biggs-Andrews iteration on gfx950.
-//
-// One RL-BA iteration = 9 streaming passes over HBM:
-//   x-R2C (+prediction)  y-fwd  z-fwd.H.z-inv  y-inv  x-C2R.LRcore.x-R2C
-//   y-fwd  z-fwd.conjH.z-inv  y-inv  x-C2R.update.dots
-// replacing the reference's cufftExecR2C/C2R calls (src/RLgpuImpl.cu:183,218)
-// and the elementwise kernels at src/RLgpuImpl.cu:331-350,383-508,571-666.
-//
-// x passes: two real rows are packed as z = a + i b, transformed as one
-// complex line in LDS, and split into two half spectra (and the reverse for
-// C2R, where the imaginary parts of the kx = 0 and kx = nx/2 bins are
-// ignored, i.e. numpy irfft semantics).
-// y/z passes: a workgroup owns a tile of L adjacent kx columns of one slab,
-// held in LDS as [element][column] so global loads are L*8 bytes contiguous.
-#include "fft_core.hpp"
+// Host launchers of the FFT pass kernels: precompiled lengths dispatch to
+// template instantiations; any other 2,3,5,7-smooth length runs the same
+// kernels compiled at run time (jit.cpp).
+#include "common.hpp"
+#include "fft_kernels_dev.hpp"
 #include "fft_kernels.hpp"
+#include "jit.hpp"
 
 #include <cmath>
+#include <string>
 #include <vector>
 
 namespace cdn {
-
-// Non-temporal policy per kernel family (measured at 256 x 1024 x 1024):
-// x passes and the z filter stream both ways; plain y/z column transforms
-// keep cached loads and stream their stores.
-#ifndef CDN_NT_X
-#define CDN_NT_X 1
-#endif
-#ifndef CDN_NT_Z
-#define CDN_NT_Z 1
-#endif
-#ifndef CDN_NT_COL_LOAD
-#define CDN_NT_COL_LOAD 0
-#endif
-#ifndef CDN_NT_COL_STORE
-#define CDN_NT_COL_STORE 1
-#endif
-constexpr bool NTX = CDN_NT_X != 0, NTZ = CDN_NT_Z != 0;
-constexpr bool NTCL = CDN_NT_COL_LOAD != 0, NTCS = CDN_NT_COL_STORE != 0;
-
-// ---------------------------------------------------------------------------
-// Per-length launch configuration
-// ---------------------------------------------------------------------------
-// threads per line for the all-LDS fallback: largest divisor of N not above
-// N/16 (about 16 elements per thread)
-constexpr int tpt_for(int N) {
-  int t = N / 16;
-  if (t < 1) t = 1;
-  for (int d = t; d >= 1; --d)
-    if (N % d == 0) return d;
-  return 1;
-}
-
-// threads per line: elements per thread E = radix lcm (doubled up to 8 when
-// it is small) makes the plan exact, so lines stay in registers between the
-// first and last stages; lengths with a radix lcm above 32 use the fallback
-constexpr int tpt_col(int N) {
-  const Plan P = make_plan(N);
-  if (P.lcm <= 32) {
-    int E = P.lcm;
-    while (E < 8 && N % (E * 2) == 0) E *= 2;
-    return N / E;
-  }
-  return tpt_for(N);
-}
-
-// x passes want lines that fit in one wavefront (split/merge by shuffles)
-constexpr int tpt_x(int N) {
-  const int t = tpt_col(N);
-  if (t > 64 && t % 64 == 0 && plan_exact(N, 64)) return 64;
-  return t;
-}
-
-// Build-time tile overrides (pairs "length, value"): columns per y/z tile and
-// threads per x-pass workgroup.  Used to sweep launch shapes.
-#ifndef CDN_COLS_OVERRIDE
-#define CDN_COLS_OVERRIDE 0, 0
-#endif
-#ifndef CDN_XTHREADS_OVERRIDE
-#define CDN_XTHREADS_OVERRIDE 0, 0
-#endif
-constexpr int kColsOverride[] = {CDN_COLS_OVERRIDE};
-constexpr int kXThreadsOverride[] = {CDN_XTHREADS_OVERRIDE};
-template <int K>
-constexpr int lookup_override(const int (&tab)[K], int N) {
-  for (int i = 0; i + 1 < K; i += 2)
-    if (tab[i] == N) return tab[i + 1];
-  return 0;
-}
-
-// columns per tile for y/z passes: fill the LDS budget, at most 64 columns
-// and 1024 threads, power of two
-constexpr int lc_for(int N, int TPT) {
-  if (lookup_override(kColsOverride, N) > 0) return lookup_override(kColsOverride, N);
-  const int budget = (N >= 768) ? 131072 : 65536;
-  int L = budget / (N * 8);
-  if (L > 64) L = 64;
-  while (L > 1 && L * TPT > 1024) L /= 2;
-  int p = 1;
-  while (p * 2 <= L) p *= 2;
-  return p;
-}
-
-template <int N> struct XCfg {
-  static constexpr int TPT = tpt_x(N);
-  static constexpr int E = N / TPT;  // elements per thread
-  static constexpr bool SHUFFLE = plan_exact(N, TPT) && TPT <= 64 && (64 % TPT) == 0 && (E % 2) == 0;
-  static constexpr int XT = lookup_override(kXThreadsOverride, N) > 0 ? lookup_override(kXThreadsOverride, N) : 128;
-  static constexpr int NL0 = (XT / TPT) < 1 ? 1 : (XT / TPT);
-  static constexpr int NL = (NL0 * N * 8 > 65536) ? ((65536 / (N * 8)) < 1 ? 1 : 65536 / (N * 8)) : NL0;
-  static constexpr int PAD = TPT < 64 ? TPT : 0;  // spreads lines over LDS banks
-  static constexpr int LS = N + PAD;
-  static constexpr int THREADS = TPT * NL;
-  static constexpr int EH = (N / 2 + TPT) / TPT;  // half-spectrum bins per thread (LDS path)
-};
-
-template <int N> struct CCfg {
-  static constexpr int TPT = tpt_col(N);
-  static constexpr int L = lc_for(N, TPT);
-  static constexpr int THREADS = TPT * L;
-  static constexpr int E = N / TPT;
-};
-
-// ---------------------------------------------------------------------------
-// x passes.  Thread p of a line holds v[m] = z[p + m TPT] with z = a + i b
-// for the two real rows r0, r0+1 packed into one complex line.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float load_y(int mode, const float* __restrict__ X, const float* __restrict__ Xp,
-                                        float* __restrict__ Y, float lam, long idx) {
-  if (mode == XIN_PRED) {
-    const float x = X[idx], xp = Xp[idx];
-    float y = x + lam * (x - xp);
-    y = y > 0.f ? y : 0.f;
-    Y[idx] = y;
-    return y;
-  }
-  if (mode == XIN_SEED) return Y[idx];
-  return X[idx];
-}
-
-__device__ __forceinline__ float2 shfl2(float2 v, int src) {
-  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
-}
-
-// Half spectra A (row r0) and B (row r0+1) from the packed transform Z:
-// A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
-template <int N>
-__device__ __forceinline__ void split_store(float2 (&v)[XCfg<N>::E], float2* line, int p, float2* __restrict__ S,
-                                            long r0, long pitch, bool v0, bool v1) {
-  using C = XCfg<N>;
-  constexpr int E = C::E, TPT = C::TPT;
-  auto put = [&](int k, float2 zk, float2 zn) {
-    const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-    const float2 D = make_float2(0.5f * (zk.x - zn.x), 0.5f * (zk.y + zn.y));
-    if (v0) st_stream<NTX>(S + r0 * pitch + k, A);
-    if (v1) st_stream<NTX>(S + (r0 + 1) * pitch + k, make_float2(D.y, -D.x));
-  };
-  if constexpr (C::SHUFFLE) {
-    // element N-k of k = p + m TPT sits in lane (TPT-p) % TPT of this line,
-    // slot E-1-m (p > 0) or in this lane, slot (E-m) % E (p = 0)
-    const int lane = threadIdx.x & 63;
-    const int src = lane - p + ((TPT - p) % TPT);
-#pragma unroll
-    for (int m = 0; m <= E / 2; ++m) {
-      const float2 t = shfl2(v[E - 1 - m], src);
-      const float2 zn = (p == 0) ? v[(E - m) % E] : t;
-      if (m < E / 2 || p == 0) put(p + m * TPT, v[m], zn);
-    }
-  } else {
-#pragma unroll
-    for (int m = 0; m < E; ++m) line[p + m * TPT] = v[m];
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < C::EH; ++e) {
-      const int k = p + e * TPT;
-      if (k <= N / 2) put(k, line[k], line[k == 0 ? 0 : N - k]);
-    }
-  }
-}
-
-// Packed line Z = A + i B (Hermitian extension of both rows; imaginary parts
-// of the kx = 0 and kx = N/2 bins ignored) into v.
-template <int N>
-__device__ __forceinline__ void merge_load(float2 (&v)[XCfg<N>::E], float2* line, int p, const float2* __restrict__ S,
-                                           long r0, long pitch, bool v0, bool v1) {
-  using C = XCfg<N>;
-  constexpr int E = C::E, TPT = C::TPT;
-  auto ld = [&](int k, float2& A, float2& B) {
-    A = v0 ? ld_stream<NTX>(S + r0 * pitch + k) : make_float2(0.f, 0.f);
-    B = v1 ? ld_stream<NTX>(S + (r0 + 1) * pitch + k) : make_float2(0.f, 0.f);
-    if (k == 0 || 2 * k == N) {
-      A.y = 0.f;
-      B.y = 0.f;
-    }
-  };
-  if constexpr (C::SHUFFLE) {
-    float2 A[E / 2 + 1], B[E / 2 + 1];
-#pragma unroll
-    for (int m = 0; m < E / 2; ++m) ld(p + m * TPT, A[m], B[m]);
-    A[E / 2] = make_float2(0.f, 0.f);
-    B[E / 2] = make_float2(0.f, 0.f);
-    if (p == 0) ld(N / 2, A[E / 2], B[E / 2]);
-#pragma unroll
-    for (int m = 0; m < E / 2; ++m) v[m] = make_float2(A[m].x - B[m].y, A[m].y + B[m].x);
-    const int lane = threadIdx.x & 63;
-    const int src = lane - p + ((TPT - p) % TPT);
-#pragma unroll
-    for (int m = E / 2; m < E; ++m) {
-      // Z[k] for k > N/2 is conj(A[N-k]) + i conj(B[N-k])
-      float2 a = shfl2(A[E - 1 - m], src), b = shfl2(B[E - 1 - m], src);
-      if (p == 0) {
-        a = A[E - m];
-        b = B[E - m];
-      }
-      if (p == 0 && m == E / 2)
-        v[m] = make_float2(a.x - b.y, a.y + b.x);  // the kx = N/2 bin itself
-      else
-        v[m] = make_float2(a.x + b.y, b.x - a.y);
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < C::EH; ++e) {
-      const int k = p + e * TPT;
-      if (k <= N / 2) {
-        float2 A, B;
-        ld(k, A, B);
-        line[k] = make_float2(A.x - B.y, A.y + B.x);
-        if (k != 0 && 2 * k != N) line[N - k] = make_float2(A.x + B.y, B.x - A.y);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = line[p + m * TPT];
-    __syncthreads();
-  }
-}
-
-template <int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
-    k_x_r2c(const float* __restrict__ X, const float* __restrict__ Xp, float* __restrict__ Y,
-            const double* __restrict__ lam, int mode, float2* __restrict__ S, long rows, long pitch,
-            const float2* __restrict__ tw) {
-  using C = XCfg<N>;
-  __shared__ float2 lds[C::NL * C::LS];
-  const int tid = threadIdx.x;
-  const int l = tid / C::TPT, p = tid % C::TPT;
-  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
-  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
-  float2* line = lds + l * C::LS;
-  const float lamf = (mode == XIN_PRED) ? (float)lam[0] : 0.f;
-  float2 v[C::E];
-  // all loads are issued before any store
-  const float* src = (mode == XIN_SEED) ? Y : X;
-  float a[C::E], b[C::E];
-#pragma unroll
-  for (int m = 0; m < C::E; ++m) {
-    const int i = p + m * C::TPT;
-    a[m] = v0 ? ld_stream<NTX>(src + r0 * N + i) : 0.f;
-    b[m] = v1 ? ld_stream<NTX>(src + (r0 + 1) * N + i) : 0.f;
-  }
-  if (mode == XIN_PRED) {
-    // updatePrediction_kernel (src/RLgpuImpl.cu:658-666)
-    float ap[C::E], bp[C::E];
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) {
-      const int i = p + m * C::TPT;
-      ap[m] = v0 ? ld_stream<NTX>(Xp + r0 * N + i) : 0.f;
-      bp[m] = v1 ? ld_stream<NTX>(Xp + (r0 + 1) * N + i) : 0.f;
-    }
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) {
-      const int i = p + m * C::TPT;
-      float ya = a[m] + lamf * (a[m] - ap[m]);
-      float yb = b[m] + lamf * (b[m] - bp[m]);
-      a[m] = ya > 0.f ? ya : 0.f;
-      b[m] = yb > 0.f ? yb : 0.f;
-      if (v0) st_stream<NTX>(Y + r0 * N + i, a[m]);
-      if (v1) st_stream<NTX>(Y + (r0 + 1) * N + i, b[m]);
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < C::E; ++m) v[m] = make_float2(a[m], b[m]);
-  fft_line<N, C::TPT, 1, false>(v, line, p, tw);
-  __syncthreads();
-  split_store<N>(v, line, p, S, r0, pitch, v0, v1);
-}
-
-template <int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
-    k_x_c2r(const float2* __restrict__ S, float* __restrict__ out, long rows, long pitch,
-            const float2* __restrict__ tw) {
-  using C = XCfg<N>;
-  __shared__ float2 lds[C::NL * C::LS];
-  const int tid = threadIdx.x;
-  const int l = tid / C::TPT, p = tid % C::TPT;
-  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
-  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
-  float2* line = lds + l * C::LS;
-  float2 v[C::E];
-  merge_load<N>(v, line, p, S, r0, pitch, v0, v1);
-  fft_line<N, C::TPT, 1, true>(v, line, p, tw);
-#pragma unroll
-  for (int m = 0; m < C::E; ++m) {
-    const int i = p + m * C::TPT;
-    if (v0) out[r0 * N + i] = v[m].x;
-    if (v1) out[(r0 + 1) * N + i] = v[m].y;
-  }
-}
-
-// LRcore_kernel (src/RLgpuImpl.cu:416-434)
-__device__ __forceinline__ float lrcore(float c, float raw, float eps) {
-  c = fabsf(c) > 0.f ? c : eps;
-  c = raw / c + eps;
-  c = c > 10.f ? 10.f : c;
-  c = c < -10.f ? -10.f : c;
-  return c;
-}
-
-template <int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
-    k_x_lrcore(float2* __restrict__ S, const float* __restrict__ raw, long rows, long pitch,
-               const float2* __restrict__ tw, float eps) {
-  using C = XCfg<N>;
-  __shared__ float2 lds[C::NL * C::LS];
-  const int tid = threadIdx.x;
-  const int l = tid / C::TPT, p = tid % C::TPT;
-  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
-  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
-  float2* line = lds + l * C::LS;
-  float2 v[C::E];
-  merge_load<N>(v, line, p, S, r0, pitch, v0, v1);
-  fft_line<N, C::TPT, 1, true>(v, line, p, tw);
-#pragma unroll
-  for (int m = 0; m < C::E; ++m) {
-    const int i = p + m * C::TPT;
-    const float ra = v0 ? ld_stream<NTX>(raw + r0 * N + i) : 0.f;
-    const float rb = v1 ? ld_stream<NTX>(raw + (r0 + 1) * N + i) : 0.f;
-    v[m] = make_float2(lrcore(v[m].x, ra, eps), lrcore(v[m].y, rb, eps));
-  }
-  __syncthreads();
-  fft_line<N, C::TPT, 1, false>(v, line, p, tw);
-  __syncthreads();
-  split_store<N>(v, line, p, S, r0, pitch, v0, v1);
-}
-
-// Deterministic workgroup sum of two doubles; out[0], out[1] written by
-// thread 0.  Wavefront shuffles only when every wavefront is full.
-template <int THREADS>
-__device__ __forceinline__ void block_sum2(double a, double b, double (*red)[THREADS], double* out) {
-  const int tid = threadIdx.x;
-  if constexpr (THREADS % 64 == 0) {
-    a = wave_sum(a);
-    b = wave_sum(b);
-    if ((tid & 63) == 0) {
-      red[0][tid >> 6] = a;
-      red[1][tid >> 6] = b;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      double sa = 0.0, sb = 0.0;
-      for (int w = 0; w < THREADS / 64; ++w) {
-        sa += red[0][w];
-        sb += red[1][w];
-      }
-      out[0] = sa;
-      out[1] = sb;
-    }
-  } else {
-    red[0][tid] = a;
-    red[1][tid] = b;
-    __syncthreads();
-    if (tid == 0) {
-      double sa = 0.0, sb = 0.0;
-      for (int w = 0; w < THREADS; ++w) {
-        sa += red[0][w];
-        sb += red[1][w];
-      }
-      out[0] = sa;
-      out[1] = sb;
-    }
-  }
-}
-
-template <int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
-    k_x_update(const float2* __restrict__ S, const float* __restrict__ Y, float* __restrict__ Xout,
-               float* G, double* __restrict__ part, int dots, long rows, long pitch,
-               const float2* __restrict__ tw) {
-  using C = XCfg<N>;
-  __shared__ float2 lds[C::NL * C::LS];
-  __shared__ double red[2][C::THREADS];
-  const int tid = threadIdx.x;
-  const int l = tid / C::TPT, p = tid % C::TPT;
-  const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
-  const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
-  float2* line = lds + l * C::LS;
-  float2 v[C::E];
-  merge_load<N>(v, line, p, S, r0, pitch, v0, v1);
-  fft_line<N, C::TPT, 1, true>(v, line, p, tw);
-  // all loads are issued before any store (G is read, then overwritten)
-  float ya[C::E], yb[C::E], ga[C::E], gb[C::E];
-#pragma unroll
-  for (int m = 0; m < C::E; ++m) {
-    const int i = p + m * C::TPT;
-    ya[m] = v0 ? ld_stream<NTX>(Y + r0 * N + i) : 0.f;
-    yb[m] = v1 ? ld_stream<NTX>(Y + (r0 + 1) * N + i) : 0.f;
-    ga[m] = (dots && v0) ? ld_stream<NTX>(G + r0 * N + i) : 0.f;
-    gb[m] = (dots && v1) ? ld_stream<NTX>(G + (r0 + 1) * N + i) : 0.f;
-  }
-  double num = 0.0, den = 0.0;
-#pragma unroll
-  for (int m = 0; m < C::E; ++m) {
-    const int i = p + m * C::TPT;
-    // currEstimate_kernel and currPrevDiff_kernel (src/RLgpuImpl.cu:462-508)
-    float xa = v[m].x * ya[m], xb = v[m].y * yb[m];
-    xa = xa > 0.f ? xa : 0.f;
-    xb = xb > 0.f ? xb : 0.f;
-    const float gna = xa - ya[m], gnb = xb - yb[m];
-    if (dots) {
-      // innerProduct_kernel: FP32 products, FP64 sums (src/RLgpuImpl.cu:583-592)
-      if (v0) {
-        num += (double)(gna * ga[m]);
-        den += (double)(ga[m] * ga[m]);
-      }
-      if (v1) {
-        num += (double)(gnb * gb[m]);
-        den += (double)(gb[m] * gb[m]);
-      }
-    }
-    if (v0) {
-      st_stream<NTX>(Xout + r0 * N + i, xa);
-      st_stream<NTX>(G + r0 * N + i, gna);
-    }
-    if (v1) {
-      st_stream<NTX>(Xout + (r0 + 1) * N + i, xb);
-      st_stream<NTX>(G + (r0 + 1) * N + i, gnb);
-    }
-  }
-  if (dots) block_sum2(num, den, red, part + 2 * (long)blockIdx.x);
-}
-
-// ---------------------------------------------------------------------------
-// y / z column passes: thread (l, p) holds column l's elements p + m TPT
-// ---------------------------------------------------------------------------
-template <int N, bool INV>
-__global__ void __launch_bounds__(CCfg<N>::THREADS)
-    k_col(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles,
-          const float2* __restrict__ tw) {
-  using C = CCfg<N>;
-  __shared__ float2 lds[N * C::L];
-  const int tid = threadIdx.x;
-  const int l = tid % C::L, p = tid / C::L;
-  const long outer = blockIdx.x / ntiles, tile = blockIdx.x - outer * ntiles;
-  const long c = tile * C::L + l;
-  const bool valid = c < ncols;
-  float2* base = S + outer * outer_stride + c;
-  float2 v[C::E];
-#pragma unroll
-  for (int m = 0; m < C::E; ++m)
-    v[m] = valid ? ld_stream<NTCL>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
-  fft_line<N, C::TPT, C::L, INV>(v, lds + l, p, tw);
-  if (valid) {
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) st_stream<NTCS>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
-  }
-}
-
-// filter_kernel / filterConj_kernel (src/RLgpuImpl.cu:331-350) with the OTF
-// value computed on the fly as in dev_otfinterpolate (src/RLgpuImpl.cu:228-329)
-template <int N>
-__global__ void __launch_bounds__(CCfg<N>::THREADS)
-    k_z_filter(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles, long ny,
-               int conj, OtfParams otf, const float2* __restrict__ tw) {
-  using C = CCfg<N>;
-  __shared__ float2 lds[N * C::L];
-  const int tid = threadIdx.x;
-  const int l = tid % C::L, p = tid / C::L;
-  const long outer = blockIdx.x / ntiles, tile = blockIdx.x - outer * ntiles;
-  const long c = tile * C::L + l;
-  const bool valid = c < ncols;
-  float2* base = S + outer * outer_stride + c;
-  float2 v[C::E];
-#pragma unroll
-  for (int m = 0; m < C::E; ++m)
-    v[m] = valid ? ld_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
-  fft_line<N, C::TPT, C::L, false>(v, lds + l, p, tw);
-
-  // signed ky (k > n>>1 -> k - n), scaled k coordinates
-  const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
-  const float kx = (float)c * otf.kxscale;
-  const float ky = (float)kyi * otf.kyscale;
-  const float sg = conj ? -1.f : 1.f;
-  if (otf.ra) {
-    const float kr = sqrtf(kx * kx + ky * ky);
-    const bool inside = kr < (float)(otf.ra_tab.nr - 1);
-    const int ir = inside ? (int)floorf(kr) : 0;
-    const float ar = kr - (float)ir;
-    const float2* t0 = otf.ra_tab.tz + (long)ir * N;
-    const float2* t1 = t0 + N;
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) {
-      const int i = p + m * C::TPT;
-      float2 r = make_float2(0.f, 0.f);
-      if (inside) {
-        const float2 a = t0[i], b = t1[i];
-        float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
-        h.y *= sg;
-        r = cscale(cmul(h, v[m]), otf.inv_n);
-      }
-      v[m] = r;
-    }
-  } else {
-    const Otf3d& T = otf.tab3;
-    const float kyindex = ky < 0.f ? ky + (float)T.nyotf : ky;
-    // the >= 0 test only excludes |ky| beyond the table, which the reference
-    // would read out of bounds
-    const bool inxy = (kx < (float)(T.nxotf - 1)) && (kyindex < (float)T.nyotf) && (kyindex >= 0.f);
-    const int ix = inxy ? (int)floorf(kx) : 0;
-    const int iy = inxy ? (int)floorf(kyindex) : 0;
-    const int iy1 = (iy + 1) % T.nyotf;
-    const float ax = kx - (float)ix, ay = kyindex - (float)iy;
-    const long nxy = (long)T.nxotf * T.nyotf;
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) {
-      const int i = p + m * C::TPT;
-      const int kzi = (i > (N >> 1)) ? i - N : i;
-      const float kz = (float)kzi * otf.kzscale;
-      const float kzindex = kz < 0.f ? kz + (float)T.nzotf : kz;
-      float2 r = make_float2(0.f, 0.f);
-      if (inxy && kzindex < (float)T.nzotf && kzindex >= 0.f) {
-        const int iz = (int)floorf(kzindex);
-        const int iz1 = (iz + 1) % T.nzotf;
-        const float az = kzindex - (float)iz;
-        auto g = [&](int zz, int yy, int xx) { return T.raw[zz * nxy + (long)yy * T.nxotf + xx]; };
-        const float2 a000 = g(iz, iy, ix), a001 = g(iz, iy, ix + 1), a010 = g(iz, iy1, ix),
-                     a011 = g(iz, iy1, ix + 1), a100 = g(iz1, iy, ix), a101 = g(iz1, iy, ix + 1),
-                     a110 = g(iz1, iy1, ix), a111 = g(iz1, iy1, ix + 1);
-        float2 h;
-        h.x = (1 - az) * (a000.x * (1 - ay) * (1 - ax) + a001.x * (1 - ay) * ax + a010.x * ay * (1 - ax) +
-                          a011.x * ay * ax) +
-              az * (a100.x * (1 - ay) * (1 - ax) + a101.x * (1 - ay) * ax + a110.x * ay * (1 - ax) +
-                    a111.x * ay * ax);
-        h.y = (1 - az) * (a000.y * (1 - ay) * (1 - ax) + a001.y * (1 - ay) * ax + a010.y * ay * (1 - ax) +
-                          a011.y * ay * ax) +
-              az * (a100.y * (1 - ay) * (1 - ax) + a101.y * (1 - ay) * ax + a110.y * ay * (1 - ax) +
-                    a111.y * ay * ax);
-        h.y *= sg;
-        r = cscale(cmul(h, v[m]), otf.inv_n);
-      }
-      v[m] = r;
-    }
-  }
-  __syncthreads();
-  fft_line<N, C::TPT, C::L, true>(v, lds + l, p, tw);
-  if (valid) {
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) st_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // host side: length dispatch
@@ -564,7 +18,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
 #define CDN_FFT_LENGTHS(X) \
   X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(810) X(1024) X(2048)
 
-bool fft_length_supported(int n) {
+bool fft_length_precompiled(int n) {
   switch (n) {
 #define CDN_CASE(N) \
   case N:           \
@@ -576,17 +30,16 @@ bool fft_length_supported(int n) {
   }
 }
 
+bool fft_length_supported(int n) { return fft_length_precompiled(n) || jit_length_ok(n); }
+
+void fft_prepare_length(int n) {
+  if (!fft_length_precompiled(n)) jit_prepare(n);
+}
+
 static Plan host_plan(int n) {
-  switch (n) {
-#define CDN_CASE(N) \
-  case N:           \
-    return make_plan(N);
-    CDN_FFT_LENGTHS(CDN_CASE)
-#undef CDN_CASE
-    default:
-      fail("FFT length " + std::to_string(n) + " is not compiled into this library");
-  }
-  return Plan{};
+  const Plan P = make_plan(n);
+  if (!P.ok || n < 1) fail("FFT length " + std::to_string(n) + " is not 2,3,5,7-smooth");
+  return P;
 }
 
 int fft_twiddle_count(int n) {
@@ -622,83 +75,80 @@ static inline unsigned grid1(long n) {
 
 void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam, int mode,
                   float2* S, long rows, long pitch, const float2* tw) {
+  const XShape xs = x_shape(nx);
+  const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
   switch (nx) {
-#define CDN_CASE(N)                                                                                   \
-  case N: {                                                                                           \
-    const long pairs = (rows + 1) / 2;                                                                \
-    CDN_LAUNCH(k_x_r2c, N, grid1((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL), XCfg<N>::THREADS, X, Xp, Y, \
-               lam, mode, S, rows, pitch, tw);                                                        \
-    break;                                                                                            \
-  }
+#define CDN_CASE(N)                                                                     \
+  case N:                                                                               \
+    CDN_LAUNCH(k_x_r2c, N, grid, xs.THREADS, X, Xp, Y, lam, mode, S, rows, pitch, tw); \
+    break;
     CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default:
-      fail("unsupported x length");
+    default: {
+      void* args[] = {&X, &Xp, &Y, &lam, &mode, &S, &rows, &pitch, &tw};
+      jit_launch(nx, JK_X_R2C, grid, xs.THREADS, st, args);
+    }
   }
   CDN_HIP(hipGetLastError());
 }
 
 void launch_x_c2r(int nx, hipStream_t st, const float2* S, float* out, long rows, long pitch, const float2* tw) {
+  const XShape xs = x_shape(nx);
+  const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
   switch (nx) {
-#define CDN_CASE(N)                                                                                 \
-  case N: {                                                                                         \
-    const long pairs = (rows + 1) / 2;                                                              \
-    CDN_LAUNCH(k_x_c2r, N, grid1((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL), XCfg<N>::THREADS, S, out, \
-               rows, pitch, tw);                                                                    \
-    break;                                                                                          \
-  }
+#define CDN_CASE(N)                                                      \
+  case N:                                                                \
+    CDN_LAUNCH(k_x_c2r, N, grid, xs.THREADS, S, out, rows, pitch, tw); \
+    break;
     CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default:
-      fail("unsupported x length");
+    default: {
+      void* args[] = {&S, &out, &rows, &pitch, &tw};
+      jit_launch(nx, JK_X_C2R, grid, xs.THREADS, st, args);
+    }
   }
   CDN_HIP(hipGetLastError());
 }
 
 void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long rows, long pitch, const float2* tw,
                      float eps) {
+  const XShape xs = x_shape(nx);
+  const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
   switch (nx) {
-#define CDN_CASE(N)                                                                                    \
-  case N: {                                                                                            \
-    const long pairs = (rows + 1) / 2;                                                                 \
-    CDN_LAUNCH(k_x_lrcore, N, grid1((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL), XCfg<N>::THREADS, S, raw, \
-               rows, pitch, tw, eps);                                                                  \
-    break;                                                                                             \
-  }
+#define CDN_CASE(N)                                                              \
+  case N:                                                                        \
+    CDN_LAUNCH(k_x_lrcore, N, grid, xs.THREADS, S, raw, rows, pitch, tw, eps); \
+    break;
     CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default:
-      fail("unsupported x length");
+    default: {
+      void* args[] = {&S, &raw, &rows, &pitch, &tw, &eps};
+      jit_launch(nx, JK_X_LRCORE, grid, xs.THREADS, st, args);
+    }
   }
   CDN_HIP(hipGetLastError());
 }
 
 int x_update_blocks(int nx, long rows) {
-  const long pairs = (rows + 1) / 2;
-  switch (nx) {
-#define CDN_CASE(N) \
-  case N:           \
-    return (int)((pairs + XCfg<N>::NL - 1) / XCfg<N>::NL);
-    CDN_FFT_LENGTHS(CDN_CASE)
-#undef CDN_CASE
-    default:
-      fail("unsupported x length");
-  }
-  return 0;
+  const XShape xs = x_shape(nx);
+  return (int)(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
 }
 
 int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, float* Xout, float* G, double* part,
                     int dots, long rows, long pitch, const float2* tw) {
+  const XShape xs = x_shape(nx);
   const int nb = x_update_blocks(nx, rows);
   switch (nx) {
-#define CDN_CASE(N)                                                                                   \
-  case N:                                                                                             \
-    CDN_LAUNCH(k_x_update, N, grid1(nb), XCfg<N>::THREADS, S, Y, Xout, G, part, dots, rows, pitch, tw); \
+#define CDN_CASE(N)                                                                                        \
+  case N:                                                                                                  \
+    CDN_LAUNCH(k_x_update, N, grid1(nb), xs.THREADS, S, Y, Xout, G, part, dots, rows, pitch, tw);        \
     break;
     CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default:
-      fail("unsupported x length");
+    default: {
+      void* args[] = {&S, &Y, &Xout, &G, &part, &dots, &rows, &pitch, &tw};
+      jit_launch(nx, JK_X_UPDATE, grid1(nb), xs.THREADS, st, args);
+    }
   }
   CDN_HIP(hipGetLastError());
   return nb;
@@ -706,40 +156,47 @@ int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, flo
 
 void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols, long line_stride,
                     long outer_stride, long n_outer, const float2* tw) {
+  const CShape cs = c_shape(len);
+  long ntiles = (ncols + cs.L - 1) / cs.L;
+  const unsigned grid = grid1(ntiles * n_outer);
   switch (len) {
 #define CDN_CASE(N)                                                                                            \
-  case N: {                                                                                                    \
-    const long ntiles = (ncols + CCfg<N>::L - 1) / CCfg<N>::L;                                                 \
+  case N:                                                                                                      \
     if (inverse)                                                                                               \
-      hipLaunchKernelGGL((k_col<N, true>), dim3(grid1(ntiles * n_outer)), dim3(CCfg<N>::THREADS), 0, st, S,    \
-                         ncols, line_stride, outer_stride, ntiles, tw);                                        \
+      hipLaunchKernelGGL((k_col<N, true>), dim3(grid), dim3(cs.THREADS), 0, st, S, ncols, line_stride,         \
+                         outer_stride, ntiles, tw);                                                            \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_col<N, false>), dim3(grid1(ntiles * n_outer)), dim3(CCfg<N>::THREADS), 0, st, S,   \
-                         ncols, line_stride, outer_stride, ntiles, tw);                                        \
-    break;                                                                                                     \
-  }
+      hipLaunchKernelGGL((k_col<N, false>), dim3(grid), dim3(cs.THREADS), 0, st, S, ncols, line_stride,        \
+                         outer_stride, ntiles, tw);                                                            \
+    break;
     CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default:
-      fail("unsupported column length");
+    default: {
+      void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &tw};
+      jit_launch(len, inverse ? JK_COL_I : JK_COL_F, grid, cs.THREADS, st, args);
+    }
   }
   CDN_HIP(hipGetLastError());
 }
 
 void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride, long ny,
                      int conj, const OtfParams& otf, const float2* tw) {
+  const CShape cs = c_shape(nz);
+  long ntiles = (ncols + cs.L - 1) / cs.L;
+  const unsigned grid = grid1(ntiles * ny);
+  OtfParams op = otf;
   switch (nz) {
-#define CDN_CASE(N)                                                                                       \
-  case N: {                                                                                               \
-    const long ntiles = (ncols + CCfg<N>::L - 1) / CCfg<N>::L;                                            \
-    CDN_LAUNCH(k_z_filter, N, grid1(ntiles * ny), CCfg<N>::THREADS, S, ncols, line_stride, outer_stride,  \
-               ntiles, ny, conj, otf, tw);                                                                \
-    break;                                                                                                \
-  }
+#define CDN_CASE(N)                                                                                      \
+  case N:                                                                                                \
+    CDN_LAUNCH(k_z_filter, N, grid, cs.THREADS, S, ncols, line_stride, outer_stride, ntiles, ny, conj,  \
+               op, tw);                                                                                  \
+    break;
     CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default:
-      fail("unsupported z length");
+    default: {
+      void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &ny, &conj, &op, &tw};
+      jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args);
+    }
   }
   CDN_HIP(hipGetLastError());
 }

@@ -9,35 +9,17 @@
 
 #include <cstdint>
 
+#include "fft_types.hpp"
+
 namespace cdn {
 
-// How the x-forward pass obtains its real input Y (one RL-BA iteration's
-// first kernel, src/RL-Biggs-Andrews.cpp:296-321).
-enum XInMode : int {
-  XIN_PLAIN = 0,  // Y = X (read X)
-  XIN_PRED = 1,   // Y = max(X + lambda (X - Xp), 0); Y is also stored
-  XIN_SEED = 2,   // Y read from the Y buffer (warm-start seed)
-};
-
-struct RaOtf {
-  const float2* tz;  // z-interpolated radial table, (nr, nz) kz fastest
-  int nr;            // radial samples (nxotf); valid iff kr < nr - 1
-};
-
-struct Otf3d {
-  const float2* raw;  // (nzotf, nyotf, nxotf) complex, x fastest
-  int nxotf, nyotf, nzotf;
-};
-
-struct OtfParams {
-  int ra;  // 1: rotationally averaged (RaOtf), 0: 3-D table (Otf3d)
-  RaOtf ra_tab;
-  Otf3d tab3;
-  float kxscale, kyscale, kzscale;
-  float inv_n;  // 1 / (nx ny nz): the filter's normalisation
-};
-
+// Length is compiled in, or can be compiled at run time (2,3,5,7-smooth).
 bool fft_length_supported(int n);
+// Length is among the lengths built into the library.
+bool fft_length_precompiled(int n);
+// Compile the kernels of a length that is not precompiled (no-op otherwise);
+// throws with the compiler log on failure.  Needs no GPU.
+void fft_prepare_length(int n);
 // Twiddle table for a supported length (host, FP64-derived float2 values).
 int fft_twiddle_count(int n);
 void fft_twiddles(int n, float2* out);

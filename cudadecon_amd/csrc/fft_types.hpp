@@ -1,0 +1,36 @@
+// Types shared by the FFT pass kernels and their host launchers.
+#pragma once
+
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+
+namespace cdn {
+
+// How the x-forward pass obtains its real input Y (one RL-BA iteration's
+// first kernel, src/RL-Biggs-Andrews.cpp:296-321).
+enum XInMode : int {
+  XIN_PLAIN = 0,  // Y = X (read X)
+  XIN_PRED = 1,   // Y = max(X + lambda (X - Xp), 0); Y is also stored
+  XIN_SEED = 2,   // Y read from the Y buffer (warm-start seed)
+};
+
+struct RaOtf {
+  const float2* tz;  // z-interpolated radial table, (nr, nz) kz fastest
+  int nr;            // radial samples (nxotf); valid iff kr < nr - 1
+};
+
+struct Otf3d {
+  const float2* raw;  // (nzotf, nyotf, nxotf) complex, x fastest
+  int nxotf, nyotf, nzotf;
+};
+
+struct OtfParams {
+  int ra;  // 1: rotationally averaged (RaOtf), 0: 3-D table (Otf3d)
+  RaOtf ra_tab;
+  Otf3d tab3;
+  float kxscale, kyscale, kzscale;
+  float inv_n;  // 1 / (nx ny nz): the filter's normalisation
+};
+
+}  // namespace cdn

@@ -90,19 +90,37 @@ def test_supported_fft_lengths(cdn, n):
 
 
 def test_unsupported_fft_length_is_reported(cdn):
-    assert cdn.lib().cudaDecon_fft_supported(11) == 0
+    L = cdn.lib()
+    assert L.cudaDecon_fft_supported(11) == 0
+    assert L.cudaDecon_fft_supported(20000) == 0
+    # any other 2,3,5,7-smooth length is compiled at run time
+    assert L.cudaDecon_fft_supported(1000) == 1 and L.cudaDecon_fft_supported(150) == 1
 
 
-def test_init_rejects_unsupported_length(cdn, tmp_path):
+def test_init_rejects_too_long_fft(cdn, tmp_path):
     import numpy as np
     from cudadecon_amd.otf import write_otf_tiff
     p = tmp_path / "otf.tif"
     write_otf_tiff(str(p), np.ones((17, 33), np.complex64))
     L = cdn.lib()
-    # 1001 -> findOptimalDimension 1000 = 2^3 5^3, not compiled in
-    rc = L.RL_interface_init(1001, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, str(p).encode())
+    # 20000 = 2^5 5^4 is smooth but beyond the 16384 kernel limit
+    rc = L.RL_interface_init(20000, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, str(p).encode())
     assert rc == 0
-    assert "1000" in cdn.last_error()
+    assert "20000" in cdn.last_error()
+
+
+def test_init_compiles_other_smooth_lengths_at_run_time(cdn, tmp_path):
+    """150 x 72 x 70 (2*3*5^2, 2^3*3^2, 2*5*7) is not precompiled: init
+    compiles its kernels with hipRTC (no GPU needed) and succeeds."""
+    import numpy as np
+    from cudadecon_amd.otf import write_otf_tiff
+    p = tmp_path / "otf.tif"
+    write_otf_tiff(str(p), np.ones((17, 33), np.complex64))
+    L = cdn.lib()
+    rc = L.RL_interface_init(150, 72, 70, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, str(p).encode())
+    assert rc == 1, cdn.last_error()
+    assert (L.get_output_nx(), L.get_output_ny(), L.get_output_nz()) == (150, 72, 70)
+    L.RL_cleanup()
 
 
 # RL_interface_init scalars: library against the oracle's restatement of
@@ -111,10 +129,10 @@ INIT_CASES = [
     # nx, ny, nz, dr, dz, dr_psf, dz_psf, angle, rot, width, skewed
     (128, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False),
     (512, 512, 128, 0.104, 0.3, 0.104, 0.15675, 31.5, 31.5, 0, False),
-    (100, 128, 64, 0.104, 0.3, 0.104, 0.3, -32.8, -32.8, 256, False),
+    (100, 128, 64, 0.104, 0.3, 0.104, 0.3, -32.8, -32.8, 0, False),  # deskewed 255 -> 252 (run-time kernels)
     (100, 128, 64, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, 256, False),
     (130, 130, 66, 0.1, 0.25, 0.1, 0.25, 0.0, 15.0, 0, False),  # crops to 128 x 128 x 64
-    (1026, 520, 260, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False),  # crops to 1024 x 512 x 256
+    (1030, 520, 260, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False),  # crops to 1029 x 512 x 256
     (512, 512, 128, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, 0, True),
 ]
 

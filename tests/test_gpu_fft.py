@@ -81,3 +81,19 @@ def test_filter_3d_otf_matches_oracle(cdn, orc):
     ref = orc.filt(v.astype(np.float64), H)
     got = cdn.filter_volume(v, tab, ks)
     assert _relerr(got, ref) < 1e-5
+
+
+# lengths outside the precompiled set: kernels compiled at run time (hipRTC)
+JIT_SHAPES = [(70, 72, 150), (16, 20, 3675), (45, 250, 64), (105, 64, 96), (8, 12, 1000)]
+
+
+@pytest.mark.parametrize("shape", JIT_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_runtime_compiled_lengths_match_oracle(cdn, orc, shape):
+    print("jit", shape, flush=True)
+    nz, ny, nx = shape
+    rng = np.random.default_rng(sum(shape) + 17)
+    v = rng.standard_normal(shape).astype(np.float32)
+    assert _relerr(cdn.fft_r2c(v), orc.rfftn(v.astype(np.float64))) < 2e-6
+    s = (rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1))).astype(
+        np.complex64)
+    assert _relerr(cdn.fft_c2r(s, nx), orc.irfftn(s.astype(np.complex128), nx)) < 2e-6

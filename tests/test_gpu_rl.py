@@ -31,8 +31,9 @@ def _relmax(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-@pytest.mark.parametrize("shape,iters", [((64, 128, 128), 10), ((64, 96, 810), 6), ((96, 64, 64), 12)],
-                         ids=["cfg0", "x810", "z96"])
+@pytest.mark.parametrize("shape,iters", [((64, 128, 128), 10), ((64, 96, 810), 6), ((96, 64, 64), 12),
+                                         ((70, 72, 150), 8)],
+                         ids=["cfg0", "x810", "z96", "runtime-compiled"])
 def test_engine_matches_oracle(cdn, orc, shape, iters):
     print("engine", shape, iters, flush=True)
     from cudadecon_amd.synth import make_stack
@@ -129,6 +130,16 @@ def test_rl_interface_deskew_rotate_small(cdn, orc, tmp_path):
     if not cdn.lib().cudaDecon_fft_supported(g["deskewed_nx"]):
         pytest.skip(f"deskewed width {g['deskewed_nx']} not compiled in")
     got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.3, 31.5, 31.5, 4)
+    assert _relmax(got, ref) <= TOL
+
+
+def test_rl_interface_deskew_to_runtime_compiled_width(cdn, orc, tmp_path):
+    # 64 x 128 x 100 raw at -32.8 deg: deskewed width findOptimalDimension(255) = 252
+    from cudadecon_amd.synth import make_stack
+    raw, _, _ = make_stack("cfg1", shape=(64, 128, 100))
+    tab, _ = _otf_and_geom(orc, raw.shape, 1.3, 4.0, 0.104, 0.3)
+    got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.3, -32.8, 0.0, 5)
+    assert got.shape == ref.shape == (64, 128, 252)
     assert _relmax(got, ref) <= TOL
 
 
