@@ -73,6 +73,10 @@ SIGNATURES = {
     "cudaDecon_rl_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "cudaDecon_rl_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cudaDecon_rl_copy_probe": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cudaDecon_rl_set_raw_block": (C.c_int, [C.c_void_p, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_float]),
+    "cudaDecon_rl_get_block": (C.c_int, [C.c_void_p, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cudaDecon_rl_stats": (C.c_int, [C.c_void_p, _d]),
     "cudaDecon_rl_destroy": (None, [C.c_void_p]),
     "cudaDecon_device_sync": (C.c_int, []),
 }
@@ -285,6 +289,28 @@ class RLEngine:
         _check(lib().cudaDecon_rl_geometry(self.h, C.byref(v, 0), C.byref(v, 8), C.byref(v, 16)),
                "cudaDecon_rl_geometry")
         return int(v[0]), int(v[1]), int(v[2])
+
+    def set_raw_block(self, block, offset_zyx, background=0.0):
+        """Stack = background everywhere, with `block` (bz, by, bx) at offset (oz, oy, ox)."""
+        b = _f32(block)
+        bz, by, bx = b.shape
+        oz, oy, ox = offset_zyx
+        _check(lib().cudaDecon_rl_set_raw_block(self.h, b, bx, by, bz, int(ox), int(oy), int(oz),
+                                                float(background)), "cudaDecon_rl_set_raw_block")
+
+    def get_block(self, shape_zyx, offset_zyx):
+        bz, by, bx = shape_zyx
+        oz, oy, ox = offset_zyx
+        out = np.empty((bz, by, bx), np.float32)
+        _check(lib().cudaDecon_rl_get_block(self.h, out, bx, by, bz, int(ox), int(oy), int(oz)),
+               "cudaDecon_rl_get_block")
+        return out
+
+    def stats(self):
+        """{'raw': (sum, min, max), 'x': (sum, min, max)} computed on the device."""
+        o = np.zeros(6, np.float64)
+        _check(lib().cudaDecon_rl_stats(self.h, o), "cudaDecon_rl_stats")
+        return {"raw": tuple(o[:3]), "x": tuple(o[3:])}
 
     def copy_probe(self, n=10):
         """Average ms of one plain streaming copy of the stack (read + write)."""

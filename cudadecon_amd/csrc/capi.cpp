@@ -603,6 +603,55 @@ int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pi
   });
 }
 
+static void block_copy(cudaDecon_rl* h, float* dev, const float* host_in, float* host_out, int bx, int by, int bz,
+                       int ox, int oy, int oz) {
+  const Fft3& F = h->eng.fft;
+  if (bx <= 0 || by <= 0 || bz <= 0 || ox < 0 || oy < 0 || oz < 0 || ox + bx > F.nx || oy + by > F.ny ||
+      oz + bz > F.nz)
+    fail("block outside the stack");
+  for (int z = 0; z < bz; ++z) {
+    float* d = dev + ((size_t)(oz + z) * F.ny + oy) * F.nx + ox;
+    if (host_in)
+      CDN_HIP(hipMemcpy2DAsync(d, sizeof(float) * F.nx, host_in + (size_t)z * bx * by, sizeof(float) * bx,
+                               sizeof(float) * bx, by, hipMemcpyHostToDevice, h->st));
+    else
+      CDN_HIP(hipMemcpy2DAsync(host_out + (size_t)z * bx * by, sizeof(float) * bx, d, sizeof(float) * F.nx,
+                               sizeof(float) * bx, by, hipMemcpyDeviceToHost, h->st));
+  }
+  CDN_HIP(hipStreamSynchronize(h->st));
+}
+
+int cudaDecon_rl_set_raw_block(cudaDecon_rl* h, const float* block, int bx, int by, int bz, int ox, int oy, int oz,
+                               float background) {
+  return guard([&]() -> int {
+    if (!h || !block) fail("cudaDecon_rl_set_raw_block: null argument");
+    fill(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, background);
+    block_copy(h, h->eng.raw.as<float>(), block, nullptr, bx, by, bz, ox, oy, oz);
+    h->eng.reset(h->st);
+    CDN_HIP(hipStreamSynchronize(h->st));
+    h->k_next = 0;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_get_block(cudaDecon_rl* h, float* out, int bx, int by, int bz, int ox, int oy, int oz) {
+  return guard([&]() -> int {
+    if (!h || !out) fail("cudaDecon_rl_get_block: null argument");
+    block_copy(h, h->eng.result(), nullptr, out, bx, by, bz, ox, oy, oz);
+    return 1;
+  });
+}
+
+int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6) {
+  return guard([&]() -> int {
+    if (!h || !out6) fail("cudaDecon_rl_stats: null argument");
+    h->eng.part.ensure(std::max(h->eng.part.bytes, sizeof(double) * 3 * 1024));
+    stats(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, h->eng.part.as<double>(), out6);
+    stats(h->st, h->eng.result(), h->eng.fft.nvox, h->eng.part.as<double>(), out6 + 3);
+    return 1;
+  });
+}
+
 int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms) {
   return guard([&]() -> int {
     if (!h || n <= 0) fail("cudaDecon_rl_copy_probe: bad argument");

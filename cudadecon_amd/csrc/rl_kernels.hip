@@ -4,7 +4,9 @@
 #include "common.hpp"
 #include "rl_kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace cdn {
 
@@ -259,6 +261,68 @@ void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes) {
   hipLaunchKernelGGL(k_copy16, dim3(nblocks(n4, 4 * CDN_COPY_UNROLL)), dim3(BS), 0, st, (const float4*)in,
                      (float4*)out, n4);
   CDN_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Fill, and FP64 sum / min / max of a volume (test and diagnostics support)
+// ---------------------------------------------------------------------------
+__global__ void k_fill(float* p, long n, float v) {
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) p[i] = v;
+}
+void fill(hipStream_t st, float* p, long n, float v) {
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(BS), 0, st, p, n, v);
+  CDN_HIP(hipGetLastError());
+}
+__global__ void __launch_bounds__(BS) k_stats_part(const float* __restrict__ img, long n, double* part) {
+  __shared__ double r[3][BS / 64];
+  double s = 0;
+  float lo = 3.4e38f, hi = -3.4e38f;
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    const float v = img[i];
+    s += (double)v;
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+  }
+  double dl = lo, dh = hi;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o, 64);
+    dl = fmin(dl, __shfl_xor(dl, o, 64));
+    dh = fmax(dh, __shfl_xor(dh, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    r[0][threadIdx.x >> 6] = s;
+    r[1][threadIdx.x >> 6] = dl;
+    r[2][threadIdx.x >> 6] = dh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 3.4e38, c = -3.4e38;
+    for (int w = 0; w < BS / 64; ++w) {
+      a += r[0][w];
+      b = fmin(b, r[1][w]);
+      c = fmax(c, r[2][w]);
+    }
+    part[3 * blockIdx.x] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = c;
+  }
+}
+void stats(hipStream_t st, const float* img, long n, double* d_part, double out[3]) {
+  const int nb = (int)std::min<long>(RED_BLOCKS, (n + BS - 1) / BS);
+  hipLaunchKernelGGL(k_stats_part, dim3(nb), dim3(BS), 0, st, img, n, d_part);
+  CDN_HIP(hipGetLastError());
+  std::vector<double> h(3 * (size_t)nb);
+  CDN_HIP(hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  CDN_HIP(hipStreamSynchronize(st));
+  out[0] = 0;
+  out[1] = 3.4e38;
+  out[2] = -3.4e38;
+  for (int b = 0; b < nb; ++b) {
+    out[0] += h[3 * b];
+    out[1] = std::min(out[1], h[3 * b + 1]);
+    out[2] = std::max(out[2], h[3 * b + 2]);
+  }
 }
 
 // ---------------------------------------------------------------------------

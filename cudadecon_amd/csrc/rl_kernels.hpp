@@ -15,6 +15,9 @@ void rescale(hipStream_t st, float* img, long n, float s);
 // d_part: >= 2048 doubles, d_acc: >= 4 doubles, d_mean: 1 float.  Synchronises.
 double mean_above_background(hipStream_t st, const float* img, long n, double* d_part, double* d_acc, float* d_mean);
 void zblend(hipStream_t st, float* img, int nx, int ny, int nz, int nZblend);
+void fill(hipStream_t st, float* p, long n, float v);
+// sum, min, max of n floats (FP64 sum); d_part holds >= 3 * 1024 doubles.  Synchronises.
+void stats(hipStream_t st, const float* img, long n, double* d_part, double out[3]);
 // bytes must be a multiple of 16
 void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes);
 void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam);

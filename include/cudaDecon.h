@@ -127,6 +127,14 @@ int cudaDecon_rl_get(cudaDecon_rl* h, float* X);
 int cudaDecon_rl_timing(cudaDecon_rl* h, int enable, double* pass_ms, long* pass_calls);
 /* bytes of one half-spectrum row pitch (complex elements) and volume sizes */
 int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pitch);
+/* Stage a sparse stack without a full-volume transfer: every voxel is set
+ * to `background`, then the host block (bz, by, bx) is placed at offset
+ * (oz, oy, ox); X_0 := the stack.  get_block reads a block of X back. */
+int cudaDecon_rl_set_raw_block(cudaDecon_rl* h, const float* block, int bx, int by, int bz, int ox, int oy, int oz,
+                               float background);
+int cudaDecon_rl_get_block(cudaDecon_rl* h, float* out, int bx, int by, int bz, int ox, int oy, int oz);
+/* FP64 sum, min, max of the staged stack (out6[0..2]) and of X (out6[3..5]). */
+int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6);
 /* Plain 16-byte streaming copy of the stack into a scratch volume, n times;
  * *ms receives the average device time of one copy (bandwidth reference). */
 int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms);
