@@ -175,3 +175,24 @@ def test_affine_interfaces(cdn, orc):
     got = cdn.affine_ra(im, m, 0.104, 0.104, 0.3)
     ref = orc.affine_ra(im, 0.104, 0.104, 0.3, m)
     assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
+def test_rl_interface_cfg1_full_size_against_fp64_digest(cdn, tmp_path):
+    """cfg1 at full size: 128 x 512 x 512 raw, deskew 31.5 deg -> 128 x 512 x 810,
+    15 RL-BA iterations, rotation -> 192 x 512 x 791, against the FP64 oracle's
+    result sampled at 40,000 seeded voxels (tests/golden/make_cfg1_digest.py)."""
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import make_stack
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1_digest.npz"))
+    raw, psf, c = make_stack("cfg1")
+    tab = make_otf(psf)
+    p = _write_otf(tmp_path, tab)
+    cdn.rl_cleanup()
+    cdn.rl_init(raw.shape, p, dzdata=c["dz"], dxdata=c["dr"], dzpsf=c["dz"], dxpsf=c["dr"], deskew=c["angle"],
+                rotate=c["rot"])
+    got = cdn.rl_decon(raw, background=100.0, n_iters=c["iters"])
+    cdn.rl_cleanup()
+    assert got.shape == tuple(d["shape"]) == (192, 512, 791)
+    err = float(np.abs(got.reshape(-1)[d["idx"]] - d["val"]).max() / d["xmax"])
+    print(f"cfg1 max |gpu - fp64| / max = {err:.2e}", flush=True)
+    assert err <= TOL

@@ -30,8 +30,8 @@ def test_cfg4_size_shift_equivariance_and_flux(cdn, orc):
     raw, _, _ = make_stack("cfg0")  # 64 x 128 x 128 bead block
     block = np.maximum(raw.astype(np.float32) - 100.0, 0.0)
     bg = 1.0
-    offA = (8, 24, 40)
-    offB = (440, 1900, 1880)  # last voxel index near 2^31, byte offsets > 2^32
+    offA = (24, 24, 40)
+    offB = (424, 1896, 1896)  # window ends at the last voxel (index 2^31 - 1); byte offsets > 2^32
     margin = 24
     win = (block.shape[0] + 2 * margin, block.shape[1] + 2 * margin, block.shape[2] + 2 * margin)
     eng = cdn.RLEngine(SHAPE, tab, g["kscale"])
@@ -39,7 +39,10 @@ def test_cfg4_size_shift_equivariance_and_flux(cdn, orc):
     for name, off in (("A", offA), ("B", offB)):
         print("scale run", name, off, flush=True)
         eng.set_raw_block(block, off, background=bg)
+        eng.timing(1)
         lam, ms = eng.iterate(10)
+        pms, calls = eng.timing(0)
+        print("  per-pass ms:", {k: round(pms[k] / max(calls[k], 1), 3) for k in pms}, flush=True)
         print(f"  10 iterations at {SHAPE}: {ms:.1f} ms ({ms / 10:.2f} ms/iter, "
               f"{np.prod(SHAPE) / (ms / 10 * 1e-3) / 1e9:.1f} Gvoxel/s)", flush=True)
         st = eng.stats()
