@@ -43,6 +43,23 @@ constexpr bool NTCL = CDN_NT_COL_LOAD != 0, NTCS = CDN_NT_COL_STORE != 0;
 // ---------------------------------------------------------------------------
 // Per-length launch configuration
 // ---------------------------------------------------------------------------
+// Build-time tile overrides (pairs "length, value"): columns per y/z tile and
+// threads per x-pass workgroup.  Used to sweep launch shapes.
+#ifndef CDN_COLS_OVERRIDE
+#define CDN_COLS_OVERRIDE 0, 0
+#endif
+#ifndef CDN_XTHREADS_OVERRIDE
+#define CDN_XTHREADS_OVERRIDE 0, 0
+#endif
+constexpr int kColsOverride[] = {CDN_COLS_OVERRIDE};
+constexpr int kXThreadsOverride[] = {CDN_XTHREADS_OVERRIDE};
+template <int K>
+constexpr int lookup_override(const int (&tab)[K], int N) {
+  for (int i = 0; i + 1 < K; i += 2)
+    if (tab[i] == N) return tab[i + 1];
+  return 0;
+}
+
 // threads per line for the all-LDS fallback: largest divisor of N not above
 // N/16 (about 16 elements per thread)
 constexpr int tpt_for(int N) {
@@ -66,28 +83,17 @@ constexpr int tpt_col(int N) {
   return tpt_for(N);
 }
 
-// x passes want lines that fit in one wavefront (split/merge by shuffles)
-constexpr int tpt_x(int N) {
-  const int t = tpt_col(N);
-  if (t > 64 && t % 64 == 0 && plan_exact(N, 64)) return 64;
-  return t;
-}
+#ifndef CDN_XTPT_OVERRIDE
+#define CDN_XTPT_OVERRIDE 0, 0
+#endif
+constexpr int kXTptOverride[] = {CDN_XTPT_OVERRIDE};
 
-// Build-time tile overrides (pairs "length, value"): columns per y/z tile and
-// threads per x-pass workgroup.  Used to sweep launch shapes.
-#ifndef CDN_COLS_OVERRIDE
-#define CDN_COLS_OVERRIDE 0, 0
-#endif
-#ifndef CDN_XTHREADS_OVERRIDE
-#define CDN_XTHREADS_OVERRIDE 0, 0
-#endif
-constexpr int kColsOverride[] = {CDN_COLS_OVERRIDE};
-constexpr int kXThreadsOverride[] = {CDN_XTHREADS_OVERRIDE};
-template <int K>
-constexpr int lookup_override(const int (&tab)[K], int N) {
-  for (int i = 0; i + 1 < K; i += 2)
-    if (tab[i] == N) return tab[i + 1];
-  return 0;
+// x passes: lines of up to 64 threads split/merge the packed row pair with
+// wavefront shuffles; longer lines (N >= 2048, measured faster with 16
+// elements per thread) use an LDS exchange
+constexpr int tpt_x(int N) {
+  if (lookup_override(kXTptOverride, N) > 0) return lookup_override(kXTptOverride, N);
+  return tpt_col(N);
 }
 
 // columns per tile for y/z passes: fill the LDS budget, at most 64 columns
@@ -465,6 +471,20 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   if (dots) block_sum2(num, den, red, part + 2 * (long)blockIdx.x);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (separate L2s).
+// Renumber them so that consecutive logical tiles, which may share 128-byte
+// lines when a tile row is narrower than a line, run on the same XCD.
+#ifndef CDN_XCD_REMAP
+#define CDN_XCD_REMAP 1
+#endif
+template <int L>
+__device__ __forceinline__ long xcd_logical_block(long b, long G) {
+  // only tiles narrower than a 128-byte line share lines with a neighbour
+  if constexpr (CDN_XCD_REMAP == 0 || L * 8 >= 128) return b;
+  const long q = G / 8, r = G % 8, x = b % 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
+}
+
 // ---------------------------------------------------------------------------
 // y / z column passes: thread (l, p) holds column l's elements p + m TPT
 // ---------------------------------------------------------------------------
@@ -476,7 +496,8 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   __shared__ float2 lds[N * C::L];
   const int tid = threadIdx.x;
   const int l = tid % C::L, p = tid / C::L;
-  const long outer = blockIdx.x / ntiles, tile = blockIdx.x - outer * ntiles;
+  const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
+  const long outer = lb / ntiles, tile = lb - outer * ntiles;
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
   float2* base = S + outer * outer_stride + c;
@@ -501,7 +522,8 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   __shared__ float2 lds[N * C::L];
   const int tid = threadIdx.x;
   const int l = tid % C::L, p = tid / C::L;
-  const long outer = blockIdx.x / ntiles, tile = blockIdx.x - outer * ntiles;
+  const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
+  const long outer = lb / ntiles, tile = lb - outer * ntiles;
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
   float2* base = S + outer * outer_stride + c;
