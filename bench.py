@@ -54,6 +54,31 @@ def pmc_traffic(kernel: str):
         return None
 
 
+def batch_cfg2(cdn, tab, iters=10, nstreams=8):
+    """48 stacks of uneven shape (z in {64..256}, y, x in {256..1024}, drawn
+    with seed 2): iterations of all stacks one at a time vs one captured
+    hipGraph over nstreams streams.  Stacks are staged on the device (a bead
+    block on a constant background), so no full-volume transfers."""
+    rng = np.random.default_rng(2)
+    zs, xys = [64, 96, 128, 192, 256], [256, 384, 512, 768, 1024]
+    shapes = [(int(rng.choice(zs)), int(rng.choice(xys)), int(rng.choice(xys))) for _ in range(48)]
+    ks = [(np.float32(0.25 * 128 / s[2]), np.float32(0.25 * 128 / s[1]), np.float32(0.515625 * 64 / s[0]))
+          for s in shapes]
+    b = cdn.Batch(shapes, tab, ks, nstreams=nstreams)
+    blk = np.random.default_rng(0).uniform(0, 100, (32, 64, 64)).astype(np.float32)
+    for i in range(len(shapes)):
+        b.set_raw_block(i, blk, (8, 8, 8), 20.0)
+    vox = int(sum(np.prod(s) for s in shapes))
+    out = {"stacks": len(shapes), "voxels": vox, "iterations": iters, "streams": nstreams}
+    for mode in ("sequential", "graph"):
+        b.run(iters, mode)
+        ms = min(b.run(iters, mode) for _ in range(2))
+        out[mode] = {"ms": round(ms, 2), "Gvoxel_iter_per_s": round(vox * iters / (ms * 1e-3) / 1e9, 2)}
+    out["speedup"] = round(out["sequential"]["ms"] / out["graph"]["ms"], 3)
+    b.close()
+    return out
+
+
 def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int):
     """FP32 oracle RL-BA (OpenMP) on the first `slab` z-planes of the stack."""
     import oracle as orc
@@ -90,6 +115,7 @@ def main():
     ap.add_argument("--cpu-slab", type=int, default=64)
     ap.add_argument("--no-e2e", action="store_true", help="skip the RL_interface end-to-end timing")
     ap.add_argument("--e2e-iters", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-stack batch measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1 "
@@ -151,6 +177,8 @@ def main():
         e2e = {"entry": "RL_interface", "iterations": args.e2e_iters, "seconds": round(e2e_s, 4),
                "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
+    batch = None if args.no_batch else batch_cfg2(cdn, tab)
+
     Rb = 4.0 * nvox
     Pb = 8.0 * nspec
     pb = pass_bytes(Rb, Pb)
@@ -210,6 +238,7 @@ def main():
         },
         "passes": passes,
         "rl_interface_e2e": e2e,
+        "batch_cfg2": batch,
         "setup_s": round(gen_s, 1),
     }
     if rank == 0 and not args.no_cpu_baseline:

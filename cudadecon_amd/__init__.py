@@ -78,6 +78,13 @@ SIGNATURES = {
     "cudaDecon_rl_get_block": (C.c_int, [C.c_void_p, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cudaDecon_rl_stats": (C.c_int, [C.c_void_p, _d]),
     "cudaDecon_rl_destroy": (None, [C.c_void_p]),
+    "cudaDecon_batch_create": (C.c_void_p, [C.c_int, C.c_void_p, _f, C.c_int, C.c_int, C.c_int, _f, C.c_int]),
+    "cudaDecon_batch_set_raw": (C.c_int, [C.c_void_p, C.c_int, _f]),
+    "cudaDecon_batch_set_raw_block": (C.c_int, [C.c_void_p, C.c_int, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.c_int, C.c_float]),
+    "cudaDecon_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "cudaDecon_batch_get": (C.c_int, [C.c_void_p, C.c_int, _f]),
+    "cudaDecon_batch_destroy": (None, [C.c_void_p]),
     "cudaDecon_device_sync": (C.c_int, []),
 }
 
@@ -321,6 +328,59 @@ class RLEngine:
     def close(self):
         if self.h:
             lib().cudaDecon_rl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent stacks of uneven shape (cudaDecon_batch_* handle).
+
+    shapes: list of (nz, ny, nx); kscales: one (kx, ky, kz) per stack."""
+
+    MODES = {"sequential": 0, "streams": 1, "graph": 2}
+
+    def __init__(self, shapes, otf_table, kscales, nstreams=8):
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        dims = np.ascontiguousarray(np.array(self.shapes, np.int32).reshape(-1))
+        ks = _f32(np.array(kscales, np.float32).reshape(-1))
+        img = otf_image(otf_table)
+        d, h, w = img.shape
+        self._img = np.ascontiguousarray(img).reshape(-1)
+        self.h = lib().cudaDecon_batch_create(len(self.shapes), dims.ctypes.data, self._img, w, h, d, ks,
+                                              int(nstreams))
+        if not self.h:
+            raise RuntimeError(f"cudaDecon_batch_create failed: {last_error()}")
+
+    def set_raw(self, i, raw):
+        _check(lib().cudaDecon_batch_set_raw(self.h, int(i), _f32(raw)), "cudaDecon_batch_set_raw")
+
+    def set_raw_block(self, i, block, offset_zyx, background=0.0):
+        b = _f32(block)
+        bz, by, bx = b.shape
+        oz, oy, ox = offset_zyx
+        _check(lib().cudaDecon_batch_set_raw_block(self.h, int(i), b, bx, by, bz, int(ox), int(oy), int(oz),
+                                                   float(background)), "cudaDecon_batch_set_raw_block")
+
+    def run(self, iters, mode="graph"):
+        """Run `iters` iterations of every stack; returns device ms."""
+        ms = np.zeros(1, np.float64)
+        _check(lib().cudaDecon_batch_run(self.h, int(iters), self.MODES[mode], ms.ctypes.data),
+               "cudaDecon_batch_run")
+        return float(ms[0])
+
+    def get(self, i):
+        out = np.empty(self.shapes[i], np.float32)
+        _check(lib().cudaDecon_batch_get(self.h, int(i), out), "cudaDecon_batch_get")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().cudaDecon_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -6,6 +6,7 @@
 // cudaDecon_last_error().
 #include "../../include/cudaDecon.h"
 
+#include "batch.hpp"
 #include "decon.hpp"
 #include "rl_kernels.hpp"
 #include "tiff.hpp"
@@ -106,6 +107,10 @@ template <class F> int guard(F&& f) {
 }  // namespace cdn
 
 using namespace cdn;
+
+struct cudaDecon_batch {
+  Batch b;
+};
 
 struct cudaDecon_rl {
   hipStream_t st = nullptr;
@@ -673,5 +678,71 @@ int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms) {
 }
 
 void cudaDecon_rl_destroy(cudaDecon_rl* h) { delete h; }
+
+// ---- batch of stacks ------------------------------------------------------
+cudaDecon_batch* cudaDecon_batch_create(int n, const int* dims_zyx, const float* otf, int otf_width, int otf_height,
+                                        int otf_depth, const float* kscales, int nstreams) {
+  cudaDecon_batch* out = nullptr;
+  guard([&]() -> int {
+    if (!dims_zyx || !otf || !kscales) fail("cudaDecon_batch_create: null argument");
+    auto h = std::make_unique<cudaDecon_batch>();
+    OtfTable T;
+    T.set_from_image(otf, otf_width, otf_height, otf_depth, 1.f, 1.f);
+    h->b.setup(n, dims_zyx, T, kscales, nstreams);
+    out = h.release();
+    return 1;
+  });
+  return out;
+}
+
+int cudaDecon_batch_set_raw(cudaDecon_batch* h, int i, const float* raw) {
+  return guard([&]() -> int {
+    if (!h || !raw || i < 0 || i >= (int)h->b.engines.size()) fail("cudaDecon_batch_set_raw: bad argument");
+    h->b.engines[i]->load_raw(raw, false, h->b.streams[0]);
+    CDN_HIP(hipStreamSynchronize(h->b.streams[0]));
+    return 1;
+  });
+}
+
+int cudaDecon_batch_set_raw_block(cudaDecon_batch* h, int i, const float* block, int bx, int by, int bz, int ox,
+                                  int oy, int oz, float background) {
+  return guard([&]() -> int {
+    if (!h || !block || i < 0 || i >= (int)h->b.engines.size()) fail("cudaDecon_batch_set_raw_block: bad argument");
+    RLEngine& e = *h->b.engines[i];
+    hipStream_t st = h->b.streams[0];
+    const Fft3& F = e.fft;
+    if (bx <= 0 || by <= 0 || bz <= 0 || ox < 0 || oy < 0 || oz < 0 || ox + bx > F.nx || oy + by > F.ny ||
+        oz + bz > F.nz)
+      fail("block outside the stack");
+    fill(st, e.raw.as<float>(), F.nvox, background);
+    for (int z = 0; z < bz; ++z)
+      CDN_HIP(hipMemcpy2DAsync(e.raw.as<float>() + ((size_t)(oz + z) * F.ny + oy) * F.nx + ox, sizeof(float) * F.nx,
+                               block + (size_t)z * bx * by, sizeof(float) * bx, sizeof(float) * bx, by,
+                               hipMemcpyHostToDevice, st));
+    e.reset(st);
+    CDN_HIP(hipStreamSynchronize(st));
+    return 1;
+  });
+}
+
+int cudaDecon_batch_run(cudaDecon_batch* h, int iters, int mode, double* ms) {
+  return guard([&]() -> int {
+    if (!h || iters <= 0 || mode < 0 || mode > 2) fail("cudaDecon_batch_run: bad argument");
+    const double t = h->b.run(iters, mode);
+    if (ms) *ms = t;
+    return 1;
+  });
+}
+
+int cudaDecon_batch_get(cudaDecon_batch* h, int i, float* X) {
+  return guard([&]() -> int {
+    if (!h || !X || i < 0 || i >= (int)h->b.engines.size()) fail("cudaDecon_batch_get: bad argument");
+    RLEngine& e = *h->b.engines[i];
+    CDN_HIP(hipMemcpy(X, e.result(), sizeof(float) * e.fft.nvox, hipMemcpyDeviceToHost));
+    return 1;
+  });
+}
+
+void cudaDecon_batch_destroy(cudaDecon_batch* h) { delete h; }
 
 }  // extern "C"

@@ -187,6 +187,8 @@ void RLEngine::reset(hipStream_t st) {
   prev = xb.as<float>();
 }
 
+void RLEngine::prepare(int n) { lam.ensure(sizeof(double) * 4 * (size_t)std::max(n, 1)); }
+
 void RLEngine::load_seed(const float* in, bool on_device, hipStream_t st) {
   const size_t vb = sizeof(float) * (size_t)fft.nvox;
   CDN_HIP(hipMemcpyAsync(y.p, in, vb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));

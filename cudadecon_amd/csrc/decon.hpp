@@ -78,6 +78,9 @@ struct RLEngine {
   void load_seed(const float* in, bool on_device, hipStream_t st);
   // X_0 := R again (restart from the staged stack)
   void reset(hipStream_t st);
+  // size every buffer for calls of up to n iterations (no allocation inside
+  // iterate afterwards, e.g. during stream capture)
+  void prepare(int n);
   // Run iterations k = k0 .. k0+n-1 continuing the resident state (k0 > 0
   // requires the previous call to have ended at k0-1); seeded => Y_0 from the
   // seed buffer.  lambdas (optional, host, n entries) receives the factor

@@ -140,6 +140,22 @@ int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6);
 int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms);
 void cudaDecon_rl_destroy(cudaDecon_rl* h);
 
+/* Batch of independent stacks of uneven shape, each with its own RL engine.
+ * dims_zyx: 3 ints (nz, ny, nx) per stack; kscales: 3 floats per stack.
+ * run: `iters` iterations of every stack from its staged stack, mode 0 = one
+ * stack at a time on one stream, 1 = round-robin over nstreams streams,
+ * 2 = the mode-1 schedule captured once into a hipGraph and replayed;
+ * *ms receives the device time of the whole batch. */
+typedef struct cudaDecon_batch cudaDecon_batch;
+cudaDecon_batch* cudaDecon_batch_create(int n, const int* dims_zyx, const float* otf, int otf_width, int otf_height,
+                                        int otf_depth, const float* kscales, int nstreams);
+int cudaDecon_batch_set_raw(cudaDecon_batch* h, int i, const float* raw);
+int cudaDecon_batch_set_raw_block(cudaDecon_batch* h, int i, const float* block, int bx, int by, int bz, int ox,
+                                  int oy, int oz, float background);
+int cudaDecon_batch_run(cudaDecon_batch* h, int iters, int mode, double* ms);
+int cudaDecon_batch_get(cudaDecon_batch* h, int i, float* X);
+void cudaDecon_batch_destroy(cudaDecon_batch* h);
+
 /* Synchronise the current device. */
 int cudaDecon_device_sync(void);
 
