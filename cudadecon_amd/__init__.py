@@ -20,7 +20,7 @@ import numpy as np
 
 __all__ = [
     "lib", "lib_path", "RLEngine", "rl_init", "rl_decon", "rl_cleanup", "deskew", "affine", "affine_ra",
-    "fft_r2c", "fft_c2r", "filter_volume", "output_shape", "last_error",
+    "fft_r2c", "fft_c2r", "filter_volume", "output_shape", "last_error", "Batch", "WarmNet",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -85,6 +85,23 @@ SIGNATURES = {
     "cudaDecon_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "cudaDecon_batch_get": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "cudaDecon_batch_destroy": (None, [C.c_void_p]),
+    "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
+    "cudaDecon_rl_distance": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cudaDecon_net_create": (C.c_void_p, [C.c_ulonglong]),
+    "cudaDecon_net_num_params": (C.c_long, []),
+    "cudaDecon_net_get_params": (C.c_int, [C.c_void_p, _f]),
+    "cudaDecon_net_set_params": (C.c_int, [C.c_void_p, _f]),
+    "cudaDecon_net_forward": (C.c_int, [C.c_void_p, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f]),
+    "cudaDecon_net_fetch": (C.c_long, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cudaDecon_net_loss_grad": (C.c_int, [C.c_void_p, _f, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                          C.c_void_p]),
+    "cudaDecon_net_step": (C.c_int, [C.c_void_p, _f, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                     C.c_void_p]),
+    "cudaDecon_net_add_samples": (C.c_int, [C.c_void_p, _f, _f, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cudaDecon_net_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_ulonglong, C.c_void_p,
+                                      C.c_void_p]),
+    "cudaDecon_net_destroy": (None, [C.c_void_p]),
+    "cudaDecon_rl_warm_start": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cudaDecon_device_sync": (C.c_int, []),
 }
 
@@ -325,6 +342,23 @@ class RLEngine:
         _check(lib().cudaDecon_rl_copy_probe(self.h, int(n), ms.ctypes.data), "cudaDecon_rl_copy_probe")
         return float(ms[0])
 
+    def snapshot(self):
+        """Keep a device copy of the current X as the reference of distance()."""
+        _check(lib().cudaDecon_rl_snapshot(self.h), "cudaDecon_rl_snapshot")
+
+    def distance(self):
+        """||X - snapshot|| / ||snapshot|| computed on the device."""
+        r = np.zeros(1, np.float64)
+        _check(lib().cudaDecon_rl_distance(self.h, r.ctypes.data), "cudaDecon_rl_distance")
+        return float(r[0])
+
+    def warm_start(self, net):
+        """Seed Y_0 with the network's output on the staged stack; returns the
+        device ms of the network.  Follow with iterate(n, seeded=True)."""
+        ms = np.zeros(1, np.float64)
+        _check(lib().cudaDecon_rl_warm_start(self.h, net.h, ms.ctypes.data), "cudaDecon_rl_warm_start")
+        return float(ms[0])
+
     def close(self):
         if self.h:
             lib().cudaDecon_rl_destroy(self.h)
@@ -381,6 +415,102 @@ class Batch:
     def close(self):
         if self.h:
             lib().cudaDecon_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WarmNet:
+    """Warm-start network (cudaDecon_net_* handle): BF16 3-D U-Net on the
+    matrix cores whose output seeds Y_0 of RL-Biggs-Andrews."""
+
+    FETCH = ("e1", "p1", "e2", "p2", "e3", "d3", "a2", "d2", "dy", "dPre5", "dA2", "dPre4", "dE3", "dPre3", "dP2",
+             "dPre2", "dP1", "dPre1")
+
+    def __init__(self, seed=0, params=None):
+        self.h = lib().cudaDecon_net_create(int(seed))
+        if not self.h:
+            raise RuntimeError(f"cudaDecon_net_create failed: {last_error()}")
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def num_params():
+        return int(lib().cudaDecon_net_num_params())
+
+    def get_params(self):
+        p = np.empty(self.num_params(), np.float32)
+        _check(lib().cudaDecon_net_get_params(self.h, p), "cudaDecon_net_get_params")
+        return p
+
+    def set_params(self, p):
+        p = _f32(p).reshape(-1)
+        if p.size != self.num_params():
+            raise ValueError(f"expected {self.num_params()} parameters, got {p.size}")
+        _check(lib().cudaDecon_net_set_params(self.h, p), "cudaDecon_net_set_params")
+
+    @staticmethod
+    def _batch(raw):
+        r = _f32(raw)
+        if r.ndim == 3:
+            r = r[None]
+        return np.ascontiguousarray(r)
+
+    def forward(self, raw, seed=False):
+        """y (normalised output), or the RL seed when seed=True; raw (nb,nz,ny,nx) or (nz,ny,nx)."""
+        r = self._batch(raw)
+        out = np.empty_like(r)
+        nb, nz, ny, nx = r.shape
+        _check(lib().cudaDecon_net_forward(self.h, r, nb, nz, ny, nx, int(bool(seed)), out), "cudaDecon_net_forward")
+        return out.reshape(np.shape(raw))
+
+    def fetch(self, name):
+        which = self.FETCH.index(name)
+        n = lib().cudaDecon_net_fetch(self.h, which, None)
+        if n < 0:
+            raise RuntimeError(last_error())
+        out = np.empty(n, np.float32)
+        if lib().cudaDecon_net_fetch(self.h, which, out.ctypes.data) != n:
+            raise RuntimeError(last_error())
+        return out
+
+    def loss_grad(self, raw, target):
+        r, t = self._batch(raw), self._batch(target)
+        nb, nz, ny, nx = r.shape
+        loss = np.zeros(1, np.float64)
+        g = np.empty(self.num_params(), np.float32)
+        _check(lib().cudaDecon_net_loss_grad(self.h, r, t, nb, nz, ny, nx, loss.ctypes.data, g.ctypes.data),
+               "cudaDecon_net_loss_grad")
+        return float(loss[0]), g
+
+    def step(self, raw, target, lr):
+        r, t = self._batch(raw), self._batch(target)
+        nb, nz, ny, nx = r.shape
+        loss = np.zeros(1, np.float64)
+        _check(lib().cudaDecon_net_step(self.h, r, t, nb, nz, ny, nx, float(lr), loss.ctypes.data),
+               "cudaDecon_net_step")
+        return float(loss[0])
+
+    def add_samples(self, raw, target):
+        r, t = self._batch(raw), self._batch(target)
+        nb, nz, ny, nx = r.shape
+        _check(lib().cudaDecon_net_add_samples(self.h, r, t, nb, nz, ny, nx), "cudaDecon_net_add_samples")
+
+    def train(self, steps, batch, lr, seed=0):
+        """(losses per step, device ms)."""
+        losses = np.zeros(max(int(steps), 1), np.float32)
+        ms = np.zeros(1, np.float64)
+        _check(lib().cudaDecon_net_train(self.h, int(steps), int(batch), float(lr), int(seed), losses.ctypes.data,
+                                         ms.ctypes.data), "cudaDecon_net_train")
+        return losses[:steps], float(ms[0])
+
+    def close(self):
+        if self.h:
+            lib().cudaDecon_net_destroy(self.h)
             self.h = None
 
     def __del__(self):

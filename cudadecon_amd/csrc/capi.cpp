@@ -8,6 +8,7 @@
 
 #include "batch.hpp"
 #include "decon.hpp"
+#include "net.hpp"
 #include "rl_kernels.hpp"
 #include "tiff.hpp"
 
@@ -112,10 +113,19 @@ struct cudaDecon_batch {
   Batch b;
 };
 
+struct cudaDecon_net {
+  hipStream_t st = nullptr;
+  WarmNet net;
+  ~cudaDecon_net() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
 struct cudaDecon_rl {
   hipStream_t st = nullptr;
   RLEngine eng;
   int k_next = 0;
+  DevBuf red, ref;  // reduction partials of the diagnostics (not the engine's), snapshot of X
   ~cudaDecon_rl() {
     if (st) (void)hipStreamDestroy(st);
   }
@@ -650,9 +660,32 @@ int cudaDecon_rl_get_block(cudaDecon_rl* h, float* out, int bx, int by, int bz, 
 int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6) {
   return guard([&]() -> int {
     if (!h || !out6) fail("cudaDecon_rl_stats: null argument");
-    h->eng.part.ensure(std::max(h->eng.part.bytes, sizeof(double) * 3 * 1024));
-    stats(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, h->eng.part.as<double>(), out6);
-    stats(h->st, h->eng.result(), h->eng.fft.nvox, h->eng.part.as<double>(), out6 + 3);
+    h->red.ensure(sizeof(double) * 3 * 1024);
+    stats(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, h->red.as<double>(), out6);
+    stats(h->st, h->eng.result(), h->eng.fft.nvox, h->red.as<double>(), out6 + 3);
+    return 1;
+  });
+}
+
+int cudaDecon_rl_snapshot(cudaDecon_rl* h) {
+  return guard([&]() -> int {
+    if (!h) fail("cudaDecon_rl_snapshot: null handle");
+    const size_t b = sizeof(float) * (size_t)h->eng.fft.nvox;
+    h->ref.ensure(b);
+    CDN_HIP(hipMemcpyAsync(h->ref.p, h->eng.result(), b, hipMemcpyDeviceToDevice, h->st));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    return 1;
+  });
+}
+
+int cudaDecon_rl_distance(cudaDecon_rl* h, double* rel) {
+  return guard([&]() -> int {
+    if (!h || !rel) fail("cudaDecon_rl_distance: null argument");
+    if (!h->ref.p) fail("cudaDecon_rl_distance: no snapshot taken");
+    h->red.ensure(sizeof(double) * 3 * 1024);
+    double o[2];
+    diff_norm(h->st, h->eng.result(), h->ref.as<float>(), h->eng.fft.nvox, h->red.as<double>(), o);
+    *rel = o[1] > 0 ? std::sqrt(o[0] / o[1]) : std::sqrt(o[0]);
     return 1;
   });
 }
@@ -744,5 +777,168 @@ int cudaDecon_batch_get(cudaDecon_batch* h, int i, float* X) {
 }
 
 void cudaDecon_batch_destroy(cudaDecon_batch* h) { delete h; }
+
+// ---- warm-start network ---------------------------------------------------
+cudaDecon_net* cudaDecon_net_create(unsigned long long seed) {
+  cudaDecon_net* out = nullptr;
+  guard([&]() -> int {
+    auto h = std::make_unique<cudaDecon_net>();
+    h->st = make_stream();
+    h->net.init(seed, h->st);
+    out = h.release();
+    return 1;
+  });
+  return out;
+}
+
+long cudaDecon_net_num_params(void) { return WarmNet::num_params(); }
+
+int cudaDecon_net_get_params(cudaDecon_net* h, float* params) {
+  return guard([&]() -> int {
+    if (!h || !params) fail("cudaDecon_net_get_params: null argument");
+    h->net.get_params(params, h->st);
+    return 1;
+  });
+}
+
+int cudaDecon_net_set_params(cudaDecon_net* h, const float* params) {
+  return guard([&]() -> int {
+    if (!h || !params) fail("cudaDecon_net_set_params: null argument");
+    h->net.set_params(params, h->st);
+    return 1;
+  });
+}
+
+namespace {
+// stage a host batch (and optional target) in the network's batch buffers
+void stage_batch(cudaDecon_net* h, const float* raw, const float* tgt, int nb, int nz, int ny, int nx, bool training) {
+  if (!raw) fail("warm-start network: null input");
+  h->net.geometry(nb, nz, ny, nx, training);
+  const size_t n = (size_t)nb * nz * ny * nx;
+  h->net.b_raw.ensure(sizeof(float) * n);
+  CDN_HIP(hipMemcpyAsync(h->net.b_raw.p, raw, sizeof(float) * n, hipMemcpyHostToDevice, h->st));
+  if (tgt) {
+    h->net.b_tgt.ensure(sizeof(float) * n);
+    CDN_HIP(hipMemcpyAsync(h->net.b_tgt.p, tgt, sizeof(float) * n, hipMemcpyHostToDevice, h->st));
+  }
+}
+}  // namespace
+
+int cudaDecon_net_forward(cudaDecon_net* h, const float* raw, int nb, int nz, int ny, int nx, int mode, float* out) {
+  return guard([&]() -> int {
+    if (!h || !out || mode < 0 || mode > 1) fail("cudaDecon_net_forward: bad argument");
+    stage_batch(h, raw, nullptr, nb, nz, ny, nx, false);
+    const size_t n = (size_t)nb * nz * ny * nx;
+    DevBuf o;
+    o.ensure(sizeof(float) * n);
+    h->net.forward(h->st, h->net.b_raw.as<float>());
+    h->net.head(h->st, h->net.b_raw.as<float>(), mode, o.as<float>());
+    CDN_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->st));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    return 1;
+  });
+}
+
+long cudaDecon_net_fetch(cudaDecon_net* h, int which, float* out) {
+  long n = -1;
+  guard([&]() -> int {
+    if (!h) fail("cudaDecon_net_fetch: null handle");
+    n = h->net.fetch(which, out, h->st);
+    return 1;
+  });
+  return n;
+}
+
+int cudaDecon_net_loss_grad(cudaDecon_net* h, const float* raw, const float* target, int nb, int nz, int ny, int nx,
+                            double* loss, float* grad) {
+  return guard([&]() -> int {
+    if (!h || !target) fail("cudaDecon_net_loss_grad: null argument");
+    stage_batch(h, raw, target, nb, nz, ny, nx, true);
+    WarmNet& N = h->net;
+    N.forward(h->st, N.b_raw.as<float>());
+    N.backward(h->st, N.b_raw.as<float>(), N.b_tgt.as<float>(), N.loss.as<float>());
+    float l = 0.f;
+    CDN_HIP(hipMemcpyAsync(&l, N.loss.p, sizeof(float), hipMemcpyDeviceToHost, h->st));
+    if (grad) CDN_HIP(hipMemcpyAsync(grad, N.Gr.p, sizeof(float) * N.nparam, hipMemcpyDeviceToHost, h->st));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    if (loss) *loss = l;
+    return 1;
+  });
+}
+
+int cudaDecon_net_step(cudaDecon_net* h, const float* raw, const float* target, int nb, int nz, int ny, int nx,
+                       float lr, double* loss) {
+  return guard([&]() -> int {
+    if (!h || !target) fail("cudaDecon_net_step: null argument");
+    stage_batch(h, raw, target, nb, nz, ny, nx, true);
+    WarmNet& N = h->net;
+    N.forward(h->st, N.b_raw.as<float>());
+    N.backward(h->st, N.b_raw.as<float>(), N.b_tgt.as<float>(), N.loss.as<float>());
+    N.adam(h->st, lr);
+    float l = 0.f;
+    CDN_HIP(hipMemcpyAsync(&l, N.loss.p, sizeof(float), hipMemcpyDeviceToHost, h->st));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    if (loss) *loss = l;
+    return 1;
+  });
+}
+
+int cudaDecon_net_add_samples(cudaDecon_net* h, const float* raw, const float* target, int count, int nz, int ny,
+                              int nx) {
+  return guard([&]() -> int {
+    if (!h || !raw || !target || count <= 0) fail("cudaDecon_net_add_samples: bad argument");
+    if (nz % 4 || ny % 4 || nx % 4 || nz < 4 || ny < 4 || nx < 4)
+      fail("cudaDecon_net_add_samples: every dimension must be a positive multiple of 4");
+    h->net.add_samples(raw, target, count, nz, ny, nx, h->st);
+    return 1;
+  });
+}
+
+int cudaDecon_net_train(cudaDecon_net* h, int steps, int batch, float lr, unsigned long long seed, float* losses,
+                        double* elapsed_ms) {
+  return guard([&]() -> int {
+    if (!h || steps < 0) fail("cudaDecon_net_train: bad argument");
+    hipEvent_t e0, e1;
+    CDN_HIP(hipEventCreate(&e0));
+    CDN_HIP(hipEventCreate(&e1));
+    CDN_HIP(hipEventRecord(e0, h->st));
+    h->net.train(steps, batch, lr, seed, h->st, losses);
+    CDN_HIP(hipEventRecord(e1, h->st));
+    CDN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CDN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    return 1;
+  });
+}
+
+void cudaDecon_net_destroy(cudaDecon_net* h) { delete h; }
+
+int cudaDecon_rl_warm_start(cudaDecon_rl* h, cudaDecon_net* net, double* elapsed_ms) {
+  return guard([&]() -> int {
+    if (!h || !net) fail("cudaDecon_rl_warm_start: null argument");
+    const Fft3& F = h->eng.fft;
+    WarmNet& N = net->net;
+    N.geometry(1, F.nz, F.ny, F.nx, false);
+    CDN_HIP(hipStreamSynchronize(net->st));  // parameter uploads of the network's stream
+    hipEvent_t e0, e1;
+    CDN_HIP(hipEventCreate(&e0));
+    CDN_HIP(hipEventCreate(&e1));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    CDN_HIP(hipEventRecord(e0, h->st));
+    N.forward(h->st, h->eng.raw.as<float>());
+    N.head(h->st, h->eng.raw.as<float>(), 1, h->eng.y.as<float>());
+    CDN_HIP(hipEventRecord(e1, h->st));
+    CDN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CDN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    return 1;
+  });
+}
 
 }  // extern "C"

@@ -1,0 +1,856 @@
+// Kernels of the warm-start network: a 3-level 3-D U-Net (16/32/64 channels,
+// 3x3x3 convolutions, ReLU, average pooling, nearest upsampling with additive
+// skips, a 1x1x1 head and a skip add to the normalised input stack).
+//
+// Activations are BF16, channels-last ([sample][z][y][x][channel]); every
+// 3x3x3 convolution is an implicit GEMM on the CDNA4 matrix cores
+// (v_mfma 16x16x32 bf16, or 16x16x16 bf16 for 16 input channels) with FP32
+// accumulation.  Weights are staged once per workgroup in LDS in the order
+// the MFMA A operand is read (one contiguous 8- or 16-byte fragment per lane).
+//
+// MFMA operand layout (checked on gfx950): for a 16 x 16 x K product with
+// F = K/4 values per lane, lane l = 16q + r holds A[m = r][k = F q + i],
+// B[k = F q + i][n = r] (i < F) and receives D[m = 4q + i][n = r] (i < 4).
+// Convolutions use A = weights (m = output channel, k = input channel),
+// B = activations (k = input channel, n = voxel along x), so each lane
+// stores 4 consecutive output channels of one voxel.
+#include "common.hpp"
+#include "net_kernels.hpp"
+
+#include <algorithm>
+
+namespace cdn {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+// round to nearest even (inputs are finite)
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ unsigned pack2(float a, float b) { return (unsigned)f2bf(a) | ((unsigned)f2bf(b) << 16); }
+__device__ __forceinline__ float lo(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float hi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+
+template <int KS> struct Mfma;
+template <> struct Mfma<32> {
+  typedef u32x4 frag;
+  __device__ static f32x4 run(frag a, frag b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+  }
+};
+template <> struct Mfma<16> {
+  typedef u32x2 frag;
+  __device__ static f32x4 run(frag a, frag b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, a), __builtin_bit_cast(s16x4, b), c,
+                                                     0, 0, 0);
+  }
+};
+
+__device__ __forceinline__ void tap_offsets(int t, int& dz, int& dy, int& dx) {
+  dz = t / 9 - 1;
+  dy = (t / 3) % 3 - 1;
+  dx = t % 3 - 1;
+}
+
+// Store 4 consecutive channels (acc + bias, optional ReLU) of one voxel.
+template <bool RELU>
+__device__ __forceinline__ void store4(bf16_t* p, f32x4 a, const float* b) {
+  float v0 = a[0] + b[0], v1 = a[1] + b[1], v2 = a[2] + b[2], v3 = a[3] + b[3];
+  if (RELU) {
+    v0 = fmaxf(v0, 0.f);
+    v1 = fmaxf(v1, 0.f);
+    v2 = fmaxf(v2, 0.f);
+    v3 = fmaxf(v3, 0.f);
+  }
+  u32x2 w;
+  w[0] = pack2(v0, v1);
+  w[1] = pack2(v2, v3);
+  *reinterpret_cast<u32x2*>(p) = w;
+}
+
+// ---------------------------------------------------------------------------
+// 3x3x3 convolution, CIN in {16, 32, 64}, zero padding, per-sample borders.
+// One wave computes 16*MB consecutive voxels of one (sample, z, y) row for
+// all COUT channels.  Grid-stride over (row, x-chunk) work items.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT, int MB, int THREADS, bool RELU, bool BIAS>
+__global__ __launch_bounds__(THREADS) void k_conv(const bf16_t* __restrict__ in, const bf16_t* __restrict__ wf,
+                                                  const float* __restrict__ bias, bf16_t* __restrict__ out, Vol v) {
+  constexpr int KS = CIN == 16 ? 16 : 32, NS = CIN / KS, NBO = COUT / 16, FR = KS / 4;
+  constexpr int WN = 27 * CIN * COUT, WAVES = THREADS / 64;
+  typedef typename Mfma<KS>::frag frag;
+  __shared__ __attribute__((aligned(16))) bf16_t sw[WN];
+  for (int i = threadIdx.x * 8; i < WN; i += THREADS * 8)
+    *reinterpret_cast<u32x4*>(sw + i) = *reinterpret_cast<const u32x4*>(wf + i);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  float bl[NBO][4];
+#pragma unroll
+  for (int j = 0; j < NBO; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bl[j][i] = BIAS ? bias[16 * j + 4 * q + i] : 0.f;
+  const int xchunks = (v.nx + 16 * MB - 1) / (16 * MB);
+  const long rows = (long)v.nb * v.nz * v.ny;
+  const long items = rows * xchunks;
+  for (long it = (long)blockIdx.x * WAVES + (threadIdx.x >> 6); it < items; it += (long)gridDim.x * WAVES) {
+    const long row = it / xchunks;
+    const int x0 = (int)(it - row * xchunks) * 16 * MB;
+    const int y = (int)(row % v.ny);
+    const long bzi = row / v.ny;
+    const int z = (int)(bzi % v.nz);
+    const long sbase = (bzi - z) * v.ny;
+    f32x4 acc[NBO][MB];
+#pragma unroll
+    for (int j = 0; j < NBO; ++j)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) acc[j][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int zz = z + dz;
+      if (zz < 0 || zz >= v.nz) continue;
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= v.ny) continue;
+        const bf16_t* src = in + (sbase + (long)zz * v.ny + yy) * (long)v.nx * CIN;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+#pragma unroll
+          for (int s = 0; s < NS; ++s) {
+            frag b[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+              const int x = x0 + 16 * mb + r + dx;
+              if (x >= 0 && x < v.nx)
+                b[mb] = *reinterpret_cast<const frag*>(src + (long)x * CIN + s * KS + q * FR);
+              else
+                b[mb] = frag(0u);
+            }
+#pragma unroll
+            for (int j = 0; j < NBO; ++j) {
+              const frag a = *reinterpret_cast<const frag*>(sw + (((t * NS + s) * NBO + j) * 64 + lane) * FR);
+#pragma unroll
+              for (int mb = 0; mb < MB; ++mb) acc[j][mb] = Mfma<KS>::run(a, b[mb], acc[j][mb]);
+            }
+          }
+        }
+      }
+    }
+    bf16_t* dst = out + row * (long)v.nx * COUT;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const int x = x0 + 16 * mb + r;
+      if (x < v.nx)
+#pragma unroll
+        for (int j = 0; j < NBO; ++j) store4<RELU>(dst + (long)x * COUT + 16 * j + 4 * q, acc[j][mb], bl[j]);
+    }
+  }
+}
+
+// First layer: 1 -> 16 channels from the FP32 stack scaled by 1/s per sample;
+// K runs over the 27 taps (padded to 32).
+template <int MB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_conv_in1(const float* __restrict__ raw, const float* __restrict__ inv_s,
+                                                      const bf16_t* __restrict__ wf, const float* __restrict__ bias,
+                                                      bf16_t* __restrict__ out, Vol v) {
+  constexpr int WAVES = THREADS / 64;
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const u32x4 a = *reinterpret_cast<const u32x4*>(wf + lane * 8);
+  float bl[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bl[i] = bias[4 * q + i];
+  int tdz[8], tdy[8], tdx[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) tap_offsets(8 * q + i, tdz[i], tdy[i], tdx[i]);
+  const int xchunks = (v.nx + 16 * MB - 1) / (16 * MB);
+  const long rows = (long)v.nb * v.nz * v.ny;
+  const long items = rows * xchunks;
+  for (long it = (long)blockIdx.x * WAVES + (threadIdx.x >> 6); it < items; it += (long)gridDim.x * WAVES) {
+    const long row = it / xchunks;
+    const int x0 = (int)(it - row * xchunks) * 16 * MB;
+    const int y = (int)(row % v.ny);
+    const long bzi = row / v.ny;
+    const int z = (int)(bzi % v.nz);
+    const long b = bzi / v.nz;
+    const float sc = inv_s[b];
+    const float* smp = raw + b * (long)v.nz * v.ny * v.nx;
+    f32x4 acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      float u[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int zz = z + tdz[i], yy = y + tdy[i], xx = x0 + 16 * mb + r + tdx[i];
+        const bool ok = 8 * q + i < 27 && zz >= 0 && zz < v.nz && yy >= 0 && yy < v.ny && xx >= 0 && xx < v.nx;
+        u[i] = ok ? smp[((long)zz * v.ny + yy) * v.nx + xx] * sc : 0.f;
+      }
+      u32x4 bfr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) bfr[i] = pack2(u[2 * i], u[2 * i + 1]);
+      acc[mb] = Mfma<32>::run(a, bfr, f32x4{0.f, 0.f, 0.f, 0.f});
+    }
+    bf16_t* dst = out + row * (long)v.nx * 16;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const int x = x0 + 16 * mb + r;
+      if (x < v.nx) store4<true>(dst + (long)x * 16 + 4 * q, acc[mb], bl);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pooling / upsampling (level L has dims v; the coarser level has v/2)
+// ---------------------------------------------------------------------------
+// p[w] = mean of the 8 children of w; one thread per (coarse voxel, 8 channels)
+template <int C>
+__global__ void k_pool(const bf16_t* __restrict__ in, bf16_t* __restrict__ out, Vol fine) {
+  constexpr int G = C / 8;
+  const int cz = fine.nz / 2, cy = fine.ny / 2, cx = fine.nx / 2;
+  const long n = (long)fine.nb * cz * cy * cx * G;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int g = (int)(i % G);
+    long w = i / G;
+    const int x = (int)(w % cx);
+    w /= cx;
+    const int y = (int)(w % cy);
+    w /= cy;
+    const int z = (int)(w % cz);
+    const long b = w / cz;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const long f = ((b * fine.nz + 2 * z + (c >> 2)) * fine.ny + 2 * y + ((c >> 1) & 1)) * fine.nx + 2 * x + (c & 1);
+      const u32x4 d = *reinterpret_cast<const u32x4*>(in + f * C + 8 * g);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] += lo(d[k]);
+        s[2 * k + 1] += hi(d[k]);
+      }
+    }
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = pack2(s[2 * k] * 0.125f, s[2 * k + 1] * 0.125f);
+    const long ow = ((b * cz + z) * cy + y) * (long)cx + x;
+    *reinterpret_cast<u32x4*>(out + ow * C + 8 * g) = o;
+  }
+}
+
+// out[v] = coarse[v/2] + skip[v]
+template <int C>
+__global__ void k_upadd(const bf16_t* __restrict__ coarse, const bf16_t* __restrict__ skip, bf16_t* __restrict__ out,
+                        Vol fine) {
+  constexpr int G = C / 8;
+  const long n = (long)fine.nb * fine.nz * fine.ny * fine.nx * G;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int g = (int)(i % G);
+    long w = i / G;
+    const long vox = w;
+    const int x = (int)(w % fine.nx);
+    w /= fine.nx;
+    const int y = (int)(w % fine.ny);
+    w /= fine.ny;
+    const int z = (int)(w % fine.nz);
+    const long b = w / fine.nz;
+    const long cw = ((b * (fine.nz / 2) + z / 2) * (fine.ny / 2) + y / 2) * (long)(fine.nx / 2) + x / 2;
+    const u32x4 a = *reinterpret_cast<const u32x4*>(coarse + cw * C + 8 * g);
+    const u32x4 s = *reinterpret_cast<const u32x4*>(skip + vox * C + 8 * g);
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = pack2(lo(a[k]) + lo(s[k]), hi(a[k]) + hi(s[k]));
+    *reinterpret_cast<u32x4*>(out + vox * C + 8 * g) = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Head: y = u + bh + sum_c Wh[c] (d2[v/2][c] + e1[v][c]),  u = raw / s.
+// MODE 0: write y;  MODE 1: write max(y, 0) * s (the RL seed);
+// MODE 2 (training): residual against target/s, dL/dy = 2 res / n_total,
+// per-block partials of [sum res^2, dWh[16], dbh].
+// ---------------------------------------------------------------------------
+constexpr int HEAD_THREADS = 256;
+constexpr int HEAD_PART = 18;
+
+template <int MODE>
+__global__ __launch_bounds__(HEAD_THREADS) void k_head(const float* __restrict__ raw,
+                                                       const float* __restrict__ inv_s,
+                                                       const bf16_t* __restrict__ e1, const bf16_t* __restrict__ d2,
+                                                       const float* __restrict__ hp, const float* __restrict__ tgt,
+                                                       float* __restrict__ out, float* __restrict__ part, Vol v,
+                                                       float inv_total) {
+  float wh[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) wh[c] = hp[c];
+  const float bh = hp[16];
+  float acc[HEAD_PART];
+#pragma unroll
+  for (int k = 0; k < HEAD_PART; ++k) acc[k] = 0.f;
+  const long per = (long)v.nz * v.ny * v.nx;
+  const long n = per * v.nb;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    long w = i;
+    const int x = (int)(w % v.nx);
+    w /= v.nx;
+    const int y = (int)(w % v.ny);
+    w /= v.ny;
+    const int z = (int)(w % v.nz);
+    const long b = w / v.nz;
+    const long cw = ((b * (v.nz / 2) + z / 2) * (v.ny / 2) + y / 2) * (long)(v.nx / 2) + x / 2;
+    const u32x4* pe = reinterpret_cast<const u32x4*>(e1 + i * 16);
+    const u32x4* pd = reinterpret_cast<const u32x4*>(d2 + cw * 16);
+    float a1[16];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const u32x4 ee = pe[h], dd = pd[h];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        a1[8 * h + 2 * k] = lo(ee[k]) + lo(dd[k]);
+        a1[8 * h + 2 * k + 1] = hi(ee[k]) + hi(dd[k]);
+      }
+    }
+    const float sc = inv_s[b];
+    float yv = raw[i] * sc + bh;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) yv += wh[c] * a1[c];
+    if (MODE == 0) out[i] = yv;
+    if (MODE == 1) out[i] = fmaxf(yv, 0.f) / sc;
+    if (MODE == 2) {
+      const float res = yv - tgt[i] * sc;
+      const float g = 2.f * res * inv_total;
+      out[i] = g;
+      acc[0] += res * res;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) acc[1 + c] += g * a1[c];
+      acc[17] += g;
+    }
+  }
+  if (MODE == 2) {
+    __shared__ float red[HEAD_THREADS];
+    for (int k = 0; k < HEAD_PART; ++k) {
+      red[threadIdx.x] = acc[k];
+      __syncthreads();
+      for (int s = HEAD_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) part[blockIdx.x * HEAD_PART + k] = red[0];
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backward elementwise kernels (one thread per voxel and 8 channels)
+// ---------------------------------------------------------------------------
+// Encoder level: dPre = (skip + up(dP)/8) * [e > 0]; skip is a tensor, or the
+// rank-1 head gradient dout[v] * Wh[c] when skip == nullptr.  dP may be null.
+template <int C>
+__global__ void k_grad_enc(const bf16_t* __restrict__ e, const bf16_t* __restrict__ skip,
+                           const float* __restrict__ dout, const float* __restrict__ wh,
+                           const bf16_t* __restrict__ dP, bf16_t* __restrict__ dpre, Vol v) {
+  constexpr int G = C / 8;
+  const long n = (long)v.nb * v.nz * v.ny * v.nx * G;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int g = (int)(i % G);
+    const long vox = i / G;
+    float s[8];
+    if (skip) {
+      const u32x4 d = *reinterpret_cast<const u32x4*>(skip + vox * C + 8 * g);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] = lo(d[k]);
+        s[2 * k + 1] = hi(d[k]);
+      }
+    } else {
+      const float go = dout[vox];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] = go * wh[8 * g + k];
+    }
+    if (dP) {
+      long w = vox;
+      const int x = (int)(w % v.nx);
+      w /= v.nx;
+      const int y = (int)(w % v.ny);
+      w /= v.ny;
+      const int z = (int)(w % v.nz);
+      const long b = w / v.nz;
+      const long cw = ((b * (v.nz / 2) + z / 2) * (v.ny / 2) + y / 2) * (long)(v.nx / 2) + x / 2;
+      const u32x4 d = *reinterpret_cast<const u32x4*>(dP + cw * C + 8 * g);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] += 0.125f * lo(d[k]);
+        s[2 * k + 1] += 0.125f * hi(d[k]);
+      }
+    }
+    const u32x4 m = *reinterpret_cast<const u32x4*>(e + vox * C + 8 * g);
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = pack2(lo(m[k]) > 0.f ? s[2 * k] : 0.f, hi(m[k]) > 0.f ? s[2 * k + 1] : 0.f);
+    *reinterpret_cast<u32x4*>(dpre + vox * C + 8 * g) = o;
+  }
+}
+
+// Decoder level (coarse dims v): dPre[w] = (sum over the 8 children of the
+// finer gradient) * [d > 0].  The finer gradient is a tensor, or the rank-1
+// head gradient dout * Wh when fine == nullptr.
+template <int C>
+__global__ void k_grad_dec(const bf16_t* __restrict__ d, const bf16_t* __restrict__ fine,
+                           const float* __restrict__ dout, const float* __restrict__ wh, bf16_t* __restrict__ dpre,
+                           Vol v) {
+  constexpr int G = C / 8;
+  const long n = (long)v.nb * v.nz * v.ny * v.nx * G;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int g = (int)(i % G);
+    const long vox = i / G;
+    long w = vox;
+    const int x = (int)(w % v.nx);
+    w /= v.nx;
+    const int y = (int)(w % v.ny);
+    w /= v.ny;
+    const int z = (int)(w % v.nz);
+    const long b = w / v.nz;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float so = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const long f = ((b * 2 * v.nz + 2 * z + (c >> 2)) * (2 * v.ny) + 2 * y + ((c >> 1) & 1)) * (long)(2 * v.nx) +
+                     2 * x + (c & 1);
+      if (fine) {
+        const u32x4 q = *reinterpret_cast<const u32x4*>(fine + f * C + 8 * g);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          s[2 * k] += lo(q[k]);
+          s[2 * k + 1] += hi(q[k]);
+        }
+      } else {
+        so += dout[f];
+      }
+    }
+    if (!fine)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] = so * wh[8 * g + k];
+    const u32x4 m = *reinterpret_cast<const u32x4*>(d + vox * C + 8 * g);
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = pack2(lo(m[k]) > 0.f ? s[2 * k] : 0.f, hi(m[k]) > 0.f ? s[2 * k + 1] : 0.f);
+    *reinterpret_cast<u32x4*>(dpre + vox * C + 8 * g) = o;
+  }
+}
+
+// Per-block channel sums of a [n][C] BF16 tensor (bias gradients):
+// part[block][C].  Threads own a fixed channel group; LDS tree per group.
+template <int C>
+__global__ __launch_bounds__(256) void k_chan_sum(const bf16_t* __restrict__ t, long nvox, float* __restrict__ part) {
+  constexpr int G = C / 8;
+  const int g = threadIdx.x % G;
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const long n = nvox * G;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const u32x4 d = *reinterpret_cast<const u32x4*>(t + (i / G) * C + 8 * g);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      s[2 * k] += lo(d[k]);
+      s[2 * k + 1] += hi(d[k]);
+    }
+  }
+  __shared__ float red[256][9];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[threadIdx.x][k] = s[k];
+  __syncthreads();
+  // 256 is a multiple of G: threads tid, tid+G, ... share a channel group
+  for (int stride = 128; stride >= G; stride >>= 1) {
+    if ((int)threadIdx.x < stride)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) red[threadIdx.x][k] += red[threadIdx.x + stride][k];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < G)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) part[blockIdx.x * (long)C + 8 * threadIdx.x + k] = red[threadIdx.x][k];
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[t][ci][co] = sum_v in[v + t][ci] * dPre[v][co].
+// MFMA with K = 32 voxels along x: A = dPre^T (m = co), B = shifted input
+// (n = ci).  Wave (tap, chunk) accumulates over work items chunk, chunk+G, ...
+// and writes part[chunk][t][ci][co].
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad(const bf16_t* __restrict__ in, const bf16_t* __restrict__ gr, Vol v,
+                                               int G, float* __restrict__ part) {
+  constexpr int NBO = COUT / 16, NBI = CIN / 16;
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wv >= 27 * G) return;
+  const int t = wv % 27, chunk = wv / 27;
+  int dz, dy, dx;
+  tap_offsets(t, dz, dy, dx);
+  f32x4 acc[NBO][NBI];
+#pragma unroll
+  for (int j = 0; j < NBO; ++j)
+#pragma unroll
+    for (int p = 0; p < NBI; ++p) acc[j][p] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int xch = (v.nx + 31) / 32;
+  const long items = (long)v.nb * v.nz * v.ny * xch;
+  for (long it = chunk; it < items; it += G) {
+    const long row = it / xch;
+    const int x0 = (int)(it - row * xch) * 32;
+    const int y = (int)(row % v.ny);
+    const long bzi = row / v.ny;
+    const int z = (int)(bzi % v.nz);
+    const int zz = z + dz, yy = y + dy;
+    if (zz < 0 || zz >= v.nz || yy < 0 || yy >= v.ny) continue;
+    const long srow = (bzi - z + zz) * v.ny + yy;
+    const bf16_t* gp = gr + row * (long)v.nx * COUT;
+    const bf16_t* ip = in + srow * (long)v.nx * CIN;
+    u32x4 a[NBO], b[NBI];
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      const int x = x0 + 8 * q + i;
+      const bool g0 = x < v.nx, g1 = x + 1 < v.nx;
+      const bool i0 = g0 && x + dx >= 0 && x + dx < v.nx, i1 = g1 && x + 1 + dx >= 0 && x + 1 + dx < v.nx;
+#pragma unroll
+      for (int j = 0; j < NBO; ++j) {
+        const unsigned v0 = g0 ? gp[(long)x * COUT + 16 * j + r] : 0u;
+        const unsigned v1 = g1 ? gp[(long)(x + 1) * COUT + 16 * j + r] : 0u;
+        a[j][i / 2] = v0 | (v1 << 16);
+      }
+#pragma unroll
+      for (int p = 0; p < NBI; ++p) {
+        const unsigned v0 = i0 ? ip[(long)(x + dx) * CIN + 16 * p + r] : 0u;
+        const unsigned v1 = i1 ? ip[(long)(x + 1 + dx) * CIN + 16 * p + r] : 0u;
+        b[p][i / 2] = v0 | (v1 << 16);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NBO; ++j)
+#pragma unroll
+      for (int p = 0; p < NBI; ++p) acc[j][p] = Mfma<32>::run(a[j], b[p], acc[j][p]);
+  }
+  float* dst = part + ((long)chunk * 27 + t) * CIN * COUT;
+#pragma unroll
+  for (int j = 0; j < NBO; ++j)
+#pragma unroll
+    for (int p = 0; p < NBI; ++p)
+      *reinterpret_cast<f32x4*>(dst + (16 * p + r) * COUT + 16 * j + 4 * q) = acc[j][p];
+}
+
+// First layer weight gradient: n runs over the taps (two blocks of 16, taps
+// >= 27 dropped), the input is bf16(raw / s) as in the forward pass.
+__global__ __launch_bounds__(256) void k_wgrad_in1(const float* __restrict__ raw, const float* __restrict__ inv_s,
+                                                   const bf16_t* __restrict__ gr, Vol v, int G,
+                                                   float* __restrict__ part) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int chunk = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (chunk >= G) return;
+  int tdz[2], tdy[2], tdx[2];
+  tap_offsets(r, tdz[0], tdy[0], tdx[0]);
+  tap_offsets(16 + r, tdz[1], tdy[1], tdx[1]);
+  const bool tv1 = 16 + r < 27;
+  f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+  const int xch = (v.nx + 31) / 32;
+  const long items = (long)v.nb * v.nz * v.ny * xch;
+  for (long it = chunk; it < items; it += G) {
+    const long row = it / xch;
+    const int x0 = (int)(it - row * xch) * 32;
+    const int y = (int)(row % v.ny);
+    const long bzi = row / v.ny;
+    const int z = (int)(bzi % v.nz);
+    const long b = bzi / v.nz;
+    const float sc = inv_s[b];
+    const float* smp = raw + b * (long)v.nz * v.ny * v.nx;
+    const bf16_t* gp = gr + row * (long)v.nx * 16;
+    u32x4 a, bb[2];
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      const int x = x0 + 8 * q + i;
+      const unsigned g0 = x < v.nx ? gp[(long)x * 16 + r] : 0u;
+      const unsigned g1 = x + 1 < v.nx ? gp[(long)(x + 1) * 16 + r] : 0u;
+      a[i / 2] = g0 | (g1 << 16);
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        float u[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int xo = x + h, zz = z + tdz[p], yy = y + tdy[p], xx = xo + tdx[p];
+          const bool ok = (p == 0 || tv1) && xo < v.nx && zz >= 0 && zz < v.nz && yy >= 0 && yy < v.ny && xx >= 0 &&
+                          xx < v.nx;
+          u[h] = ok ? smp[((long)zz * v.ny + yy) * v.nx + xx] * sc : 0.f;
+        }
+        bb[p][i / 2] = pack2(u[0], u[1]);
+      }
+    }
+    acc[0] = Mfma<32>::run(a, bb[0], acc[0]);
+    acc[1] = Mfma<32>::run(a, bb[1], acc[1]);
+  }
+  // D[m = co = 4q + i][n = tap = 16p + r] -> part[chunk][tap][co]
+  float* dst = part + (long)chunk * 27 * 16;
+  *reinterpret_cast<f32x4*>(dst + r * 16 + 4 * q) = acc[0];
+  if (tv1) *reinterpret_cast<f32x4*>(dst + (16 + r) * 16 + 4 * q) = acc[1];
+}
+
+// out[c] = sum_g part[g][c] (double, fixed order); accumulate into out or set
+__global__ void k_reduce_rows(const float* __restrict__ part, int rows, long cols, float* __restrict__ out,
+                              double scale) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < cols; c += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int g = 0; g < rows; ++g) s += part[(long)g * cols + c];
+    out[c] = (float)(s * scale);
+  }
+}
+
+// per-sample sums of the FP32 stack: part[b][blk]
+constexpr int SUM_THREADS = 256;
+__global__ __launch_bounds__(SUM_THREADS) void k_sample_sum(const float* __restrict__ raw, long per, int blocks,
+                                                            double* __restrict__ part) {
+  const int b = blockIdx.x / blocks, k = blockIdx.x % blocks;
+  const float* p = raw + (long)b * per;
+  double s = 0.0;
+  for (long i = (long)k * SUM_THREADS + threadIdx.x; i < per; i += (long)blocks * SUM_THREADS) s += p[i];
+  __shared__ double red[SUM_THREADS];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = SUM_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// inv_s[b] = n / sum (the inverse of the sample mean; 1 for an all-zero sample)
+__global__ void k_inv_scale(const double* __restrict__ part, int nb, int blocks, long per, float* __restrict__ inv_s) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s = 0.0;
+  for (int k = 0; k < blocks; ++k) s += part[b * blocks + k];
+  inv_s[b] = s > 0.0 ? (float)((double)per / s) : 1.f;
+}
+
+// Loss and head gradient from the head partials: loss = sum res^2 / n_total,
+// grad[head_off + c] = sum of the dWh partials, grad[head_off + 16] = dbh.
+__global__ void k_head_reduce(const float* __restrict__ part, int blocks, double inv_total, float* __restrict__ grad,
+                              long head_off, float* __restrict__ loss) {
+  const int k = threadIdx.x;
+  if (k >= HEAD_PART) return;
+  double s = 0.0;
+  for (int b = 0; b < blocks; ++b) s += part[(long)b * HEAD_PART + k];
+  if (k == 0)
+    *loss = (float)(s * inv_total);
+  else
+    grad[head_off + k - 1] = (float)s;
+}
+
+// Packed BF16 MFMA A operand of one 3x3x3 layer from the FP32 master weights
+// W[t][ci][co] (ci < cin, co < cout).  For the backward-data pass (flip) the
+// layer is the adjoint: W'[t][co][ci] = W[26 - t][ci][co] with the channel
+// counts swapped.  Layout: [t][k-step][co-block][lane][F], lane = 16q + r
+// holding W[ci = step*KS + F q + i][co = 16 block + r]; for a single input
+// channel K runs over the taps ([lane][8], tap = 8q + i, zero for tap >= 27).
+__global__ void k_pack(const float* __restrict__ W, int cin, int cout, int flip, bf16_t* __restrict__ out) {
+  const int ci_n = flip ? cout : cin, co_n = flip ? cin : cout;  // the packed layer's channels
+  const long n = 27L * ci_n * co_n + (ci_n == 1 ? 5L * co_n : 0L);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    float val;
+    if (ci_n == 1) {
+      const int i = (int)(e % 8), lane = (int)((e / 8) % 64), j = (int)(e / 512);
+      const int tap = 8 * (lane >> 4) + i, co = 16 * j + (lane & 15);
+      val = tap < 27 ? W[tap * co_n + co] : 0.f;
+    } else {
+      const int KS = ci_n == 16 ? 16 : 32, F = KS / 4, NS = ci_n / KS, NBO = co_n / 16;
+      long k = e;
+      const int i = (int)(k % F);
+      k /= F;
+      const int lane = (int)(k % 64);
+      k /= 64;
+      const int j = (int)(k % NBO);
+      k /= NBO;
+      const int s = (int)(k % NS);
+      const int t = (int)(k / NS);
+      const int ci = s * KS + F * (lane >> 4) + i, co = 16 * j + (lane & 15);
+      val = flip ? W[((long)(26 - t) * cin + co) * cout + ci] : W[((long)t * cin + ci) * cout + co];
+    }
+    out[e] = f2bf(val);
+  }
+}
+
+// Adam (bias-corrected step size folded into lr_t; c2 = 1 / (1 - beta2^t))
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ vv, long n, float lr_t, float b1, float b2, float eps, float c2) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * vv[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    vv[i] = vi;
+    p[i] -= lr_t * mi / (sqrtf(vi * c2) + eps);
+  }
+}
+
+int grid_for(long n, int threads, int cap = 4096) {
+  long g = (n + threads - 1) / threads;
+  return (int)std::max(1L, std::min<long>(g, cap));
+}
+
+template <int CIN, int COUT, bool RELU, bool BIAS>
+void conv_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
+  constexpr int MB = COUT == 64 ? 2 : 4;  // 4 keeps COUT = 64 within 128 VGPRs only with spills
+  constexpr int THREADS = 27 * CIN * COUT * 2 > 64 * 1024 ? 1024 : 256;
+  const long items = (long)v.nb * v.nz * v.ny * ((v.nx + 16 * MB - 1) / (16 * MB));
+  const int waves = THREADS / 64;
+  const int cap = THREADS == 1024 ? 256 : 1024;
+  const int grid = (int)std::max(1L, std::min<long>((items + waves - 1) / waves, cap));
+  hipLaunchKernelGGL((k_conv<CIN, COUT, MB, THREADS, RELU, BIAS>), dim3(grid), dim3(THREADS), 0, st, in, wf, bias,
+                     out, v);
+}
+
+template <int CIN, int COUT>
+void wgrad_launch(hipStream_t st, const bf16_t* in, const bf16_t* g, Vol v, int G, float* part) {
+  hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3((27 * G + 3) / 4), dim3(256), 0, st, in, g, v, G, part);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void net_conv(hipStream_t st, int cin, int cout, bool relu, const bf16_t* in, const bf16_t* wf, const float* bias,
+              bf16_t* out, Vol v) {
+  const int key = cin * 1000 + cout;
+  if (relu) {
+    switch (key) {
+      case 16032: return conv_launch<16, 32, true, true>(st, in, wf, bias, out, v);
+      case 32064: return conv_launch<32, 64, true, true>(st, in, wf, bias, out, v);
+      case 64032: return conv_launch<64, 32, true, true>(st, in, wf, bias, out, v);
+      case 32016: return conv_launch<32, 16, true, true>(st, in, wf, bias, out, v);
+    }
+  } else {
+    switch (key) {
+      case 16032: return conv_launch<16, 32, false, false>(st, in, wf, bias, out, v);
+      case 32064: return conv_launch<32, 64, false, false>(st, in, wf, bias, out, v);
+      case 64032: return conv_launch<64, 32, false, false>(st, in, wf, bias, out, v);
+      case 32016: return conv_launch<32, 16, false, false>(st, in, wf, bias, out, v);
+    }
+  }
+  fail("net_conv: unsupported channel pair");
+}
+
+void net_conv_in1(hipStream_t st, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,
+                  bf16_t* out, Vol v) {
+  constexpr int MB = 4, THREADS = 256;
+  const long items = (long)v.nb * v.nz * v.ny * ((v.nx + 16 * MB - 1) / (16 * MB));
+  const int grid = (int)std::max(1L, std::min<long>((items + 3) / 4, 2048));
+  hipLaunchKernelGGL((k_conv_in1<MB, THREADS>), dim3(grid), dim3(THREADS), 0, st, raw, inv_s, wf, bias, out, v);
+}
+
+void net_pool(hipStream_t st, int c, const bf16_t* in, bf16_t* out, Vol fine) {
+  const long n = (long)fine.nb * (fine.nz / 2) * (fine.ny / 2) * (fine.nx / 2) * (c / 8);
+  const int grid = grid_for(n, 256);
+  if (c == 16) hipLaunchKernelGGL(k_pool<16>, dim3(grid), dim3(256), 0, st, in, out, fine);
+  else if (c == 32) hipLaunchKernelGGL(k_pool<32>, dim3(grid), dim3(256), 0, st, in, out, fine);
+  else fail("net_pool: channels");
+}
+
+void net_upadd(hipStream_t st, int c, const bf16_t* coarse, const bf16_t* skip, bf16_t* out, Vol fine) {
+  const long n = (long)fine.nb * fine.nz * fine.ny * fine.nx * (c / 8);
+  const int grid = grid_for(n, 256);
+  if (c == 32) hipLaunchKernelGGL(k_upadd<32>, dim3(grid), dim3(256), 0, st, coarse, skip, out, fine);
+  else if (c == 16) hipLaunchKernelGGL(k_upadd<16>, dim3(grid), dim3(256), 0, st, coarse, skip, out, fine);
+  else fail("net_upadd: channels");
+}
+
+int net_head_blocks(Vol v) { return grid_for((long)v.nb * v.nz * v.ny * v.nx, HEAD_THREADS, 1024); }
+
+void net_head(hipStream_t st, int mode, const float* raw, const float* inv_s, const bf16_t* e1, const bf16_t* d2,
+              const float* hp, const float* tgt, float* out, float* part, Vol v) {
+  const int grid = net_head_blocks(v);
+  const float inv_total = 1.f / (float)((double)v.nb * v.nz * v.ny * v.nx);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_head<0>, dim3(grid), dim3(HEAD_THREADS), 0, st, raw, inv_s, e1, d2, hp, tgt, out, part, v,
+                       inv_total);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_head<1>, dim3(grid), dim3(HEAD_THREADS), 0, st, raw, inv_s, e1, d2, hp, tgt, out, part, v,
+                       inv_total);
+  else
+    hipLaunchKernelGGL(k_head<2>, dim3(grid), dim3(HEAD_THREADS), 0, st, raw, inv_s, e1, d2, hp, tgt, out, part, v,
+                       inv_total);
+}
+
+void net_head_reduce(hipStream_t st, const float* part, int blocks, double inv_total, float* grad, long head_off,
+                     float* loss) {
+  hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, st, part, blocks, inv_total, grad, head_off, loss);
+}
+
+void net_grad_enc(hipStream_t st, int c, const bf16_t* e, const bf16_t* skip, const float* dout, const float* wh,
+                  const bf16_t* dP, bf16_t* dpre, Vol v) {
+  const int grid = grid_for((long)v.nb * v.nz * v.ny * v.nx * (c / 8), 256);
+  if (c == 16) hipLaunchKernelGGL(k_grad_enc<16>, dim3(grid), dim3(256), 0, st, e, skip, dout, wh, dP, dpre, v);
+  else if (c == 32) hipLaunchKernelGGL(k_grad_enc<32>, dim3(grid), dim3(256), 0, st, e, skip, dout, wh, dP, dpre, v);
+  else if (c == 64) hipLaunchKernelGGL(k_grad_enc<64>, dim3(grid), dim3(256), 0, st, e, skip, dout, wh, dP, dpre, v);
+  else fail("net_grad_enc: channels");
+}
+
+void net_grad_dec(hipStream_t st, int c, const bf16_t* d, const bf16_t* fine, const float* dout, const float* wh,
+                  bf16_t* dpre, Vol coarse) {
+  const int grid = grid_for((long)coarse.nb * coarse.nz * coarse.ny * coarse.nx * (c / 8), 256);
+  if (c == 16) hipLaunchKernelGGL(k_grad_dec<16>, dim3(grid), dim3(256), 0, st, d, fine, dout, wh, dpre, coarse);
+  else if (c == 32) hipLaunchKernelGGL(k_grad_dec<32>, dim3(grid), dim3(256), 0, st, d, fine, dout, wh, dpre, coarse);
+  else fail("net_grad_dec: channels");
+}
+
+void net_chan_sum(hipStream_t st, int c, const bf16_t* t, long nvox, int blocks, float* part) {
+  if (c == 16) hipLaunchKernelGGL(k_chan_sum<16>, dim3(blocks), dim3(256), 0, st, t, nvox, part);
+  else if (c == 32) hipLaunchKernelGGL(k_chan_sum<32>, dim3(blocks), dim3(256), 0, st, t, nvox, part);
+  else if (c == 64) hipLaunchKernelGGL(k_chan_sum<64>, dim3(blocks), dim3(256), 0, st, t, nvox, part);
+  else fail("net_chan_sum: channels");
+}
+
+void net_wgrad(hipStream_t st, int cin, int cout, const bf16_t* in, const bf16_t* g, Vol v, int G, float* part) {
+  switch (cin * 1000 + cout) {
+    case 16032: return wgrad_launch<16, 32>(st, in, g, v, G, part);
+    case 32064: return wgrad_launch<32, 64>(st, in, g, v, G, part);
+    case 64032: return wgrad_launch<64, 32>(st, in, g, v, G, part);
+    case 32016: return wgrad_launch<32, 16>(st, in, g, v, G, part);
+  }
+  fail("net_wgrad: unsupported channel pair");
+}
+
+void net_wgrad_in1(hipStream_t st, const float* raw, const float* inv_s, const bf16_t* g, Vol v, int G, float* part) {
+  hipLaunchKernelGGL(k_wgrad_in1, dim3((G + 3) / 4), dim3(256), 0, st, raw, inv_s, g, v, G, part);
+}
+
+void net_reduce_rows(hipStream_t st, const float* part, int rows, long cols, float* out) {
+  hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(cols, 256, 1024)), dim3(256), 0, st, part, rows, cols, out, 1.0);
+}
+
+int net_sum_blocks() { return 64; }
+
+void net_inv_scale(hipStream_t st, const float* raw, Vol v, double* part, float* inv_s) {
+  const long per = (long)v.nz * v.ny * v.nx;
+  const int blocks = net_sum_blocks();
+  hipLaunchKernelGGL(k_sample_sum, dim3(v.nb * blocks), dim3(SUM_THREADS), 0, st, raw, per, blocks, part);
+  hipLaunchKernelGGL(k_inv_scale, dim3((v.nb + 63) / 64), dim3(64), 0, st, part, v.nb, blocks, per, inv_s);
+}
+
+void net_pack(hipStream_t st, const float* W, int cin, int cout, int flip, bf16_t* out) {
+  const int ci_n = flip ? cout : cin;
+  const long n = 27L * cin * cout + (ci_n == 1 ? 5L * cout : 0L);
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256, 1024)), dim3(256), 0, st, W, cin, cout, flip, out);
+}
+
+void net_adam(hipStream_t st, float* p, const float* g, float* m, float* v, long n, float lr_t, float b1, float b2,
+              float eps, float c2) {
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(n, 256, 1024)), dim3(256), 0, st, p, g, m, v, n, lr_t, b1, b2, eps, c2);
+}
+
+}  // namespace cdn

@@ -325,6 +325,50 @@ void stats(hipStream_t st, const float* img, long n, double* d_part, double out[
   }
 }
 
+// sum (a - b)^2 and sum b^2 (FP64) per block
+__global__ void __launch_bounds__(BS) k_diff_part(const float* __restrict__ a, const float* __restrict__ b, long n,
+                                                  double* part) {
+  __shared__ double r[2][BS / 64];
+  double d2 = 0, b2 = 0;
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    const double bv = b[i], dv = (double)a[i] - bv;
+    d2 += dv * dv;
+    b2 += bv * bv;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    d2 += __shfl_xor(d2, o, 64);
+    b2 += __shfl_xor(b2, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    r[0][threadIdx.x >> 6] = d2;
+    r[1][threadIdx.x >> 6] = b2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0, y = 0;
+    for (int w = 0; w < BS / 64; ++w) {
+      x += r[0][w];
+      y += r[1][w];
+    }
+    part[2 * blockIdx.x] = x;
+    part[2 * blockIdx.x + 1] = y;
+  }
+}
+void diff_norm(hipStream_t st, const float* a, const float* b, long n, double* d_part, double out[2]) {
+  const int nb = (int)std::min<long>(RED_BLOCKS, (n + BS - 1) / BS);
+  hipLaunchKernelGGL(k_diff_part, dim3(nb), dim3(BS), 0, st, a, b, n, d_part);
+  CDN_HIP(hipGetLastError());
+  std::vector<double> h(2 * (size_t)nb);
+  CDN_HIP(hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  CDN_HIP(hipStreamSynchronize(st));
+  out[0] = out[1] = 0;
+  for (int k = 0; k < nb; ++k) {
+    out[0] += h[2 * k];
+    out[1] += h[2 * k + 1];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Geometry
 // ---------------------------------------------------------------------------

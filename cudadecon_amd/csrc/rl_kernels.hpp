@@ -18,6 +18,8 @@ void zblend(hipStream_t st, float* img, int nx, int ny, int nz, int nZblend);
 void fill(hipStream_t st, float* p, long n, float v);
 // sum, min, max of n floats (FP64 sum); d_part holds >= 3 * 1024 doubles.  Synchronises.
 void stats(hipStream_t st, const float* img, long n, double* d_part, double out[3]);
+// sum (a-b)^2 and sum b^2 of n floats (FP64); d_part holds >= 2 * 1024 doubles.  Synchronises.
+void diff_norm(hipStream_t st, const float* a, const float* b, long n, double* d_part, double out[2]);
 // bytes must be a multiple of 16
 void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes);
 void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam);

@@ -156,6 +156,55 @@ int cudaDecon_batch_run(cudaDecon_batch* h, int iters, int mode, double* ms);
 int cudaDecon_batch_get(cudaDecon_batch* h, int i, float* X);
 void cudaDecon_batch_destroy(cudaDecon_batch* h);
 
+/* Reference copy of the engine's current X (snapshot) and the relative L2
+ * distance ||X - ref|| / ||ref|| of the current X to it (FP64 sums). */
+int cudaDecon_rl_snapshot(cudaDecon_rl* h);
+int cudaDecon_rl_distance(cudaDecon_rl* h, double* rel);
+
+/* Learned warm start: a 3-level 3-D U-Net (16/32/64 channels, 3x3x3
+ * convolutions, ReLU, average pooling, nearest upsampling with additive
+ * skips, 1x1x1 head plus a skip add of the input) in BF16 on the matrix
+ * cores with FP32 accumulation.  Input u = stack / mean(stack) per sample;
+ * output y (normalised); RL seed = max(y, 0) * mean(stack).  Volumes are
+ * (nb, nz, ny, nx) float arrays with nz, ny, nx multiples of 4.
+ * Parameters are one FP32 vector: per 3x3x3 layer W[27][cin][cout] then
+ * b[cout] (layers 1->16, 16->32, 32->64, 64->32, 32->16), then the head
+ * Wh[16], bh. */
+typedef struct cudaDecon_net cudaDecon_net;
+/* He-normal weights from `seed`, zero biases and head (output = input). */
+cudaDecon_net* cudaDecon_net_create(unsigned long long seed);
+long cudaDecon_net_num_params(void);
+int cudaDecon_net_get_params(cudaDecon_net* h, float* params);
+/* also clears the Adam state */
+int cudaDecon_net_set_params(cudaDecon_net* h, const float* params);
+/* mode 0: y, mode 1: the RL seed; out has the shape of raw */
+int cudaDecon_net_forward(cudaDecon_net* h, const float* raw, int nb, int nz, int ny, int nx, int mode, float* out);
+/* Copy an internal tensor of the last forward / loss_grad call to host as
+ * float (channels-last).  which: 0 e1, 1 pool(e1), 2 e2, 3 pool(e2), 4 e3,
+ * 5 d3, 6 up(d3)+e2, 7 d2, 8 dL/dy, 9.. pre-activation gradients of the
+ * layers and the gradients flowing between them (see the implementation).
+ * Returns the element count (out may be null to ask), -1 on error. */
+long cudaDecon_net_fetch(cudaDecon_net* h, int which, float* out);
+/* loss = mean((y - target / mean(raw))^2) and its gradient w.r.t. every
+ * parameter (num_params floats) */
+int cudaDecon_net_loss_grad(cudaDecon_net* h, const float* raw, const float* target, int nb, int nz, int ny, int nx,
+                            double* loss, float* grad);
+/* one Adam step (lr, betas 0.9 / 0.999, eps 1e-8) on a host batch */
+int cudaDecon_net_step(cudaDecon_net* h, const float* raw, const float* target, int nb, int nz, int ny, int nx,
+                       float lr, double* loss);
+/* append `count` (stack, target) pairs of one shape to the device training set */
+int cudaDecon_net_add_samples(cudaDecon_net* h, const float* raw, const float* target, int count, int nz, int ny,
+                              int nx);
+/* `steps` Adam steps on random batches of the training set; losses (steps
+ * floats, nullable); elapsed_ms (nullable) the device time */
+int cudaDecon_net_train(cudaDecon_net* h, int steps, int batch, float lr, unsigned long long seed, float* losses,
+                        double* elapsed_ms);
+void cudaDecon_net_destroy(cudaDecon_net* h);
+/* Seed Y_0 of an RL engine with the network's output on the engine's
+ * staged stack (all on the device); then iterate with seeded = 1.
+ * elapsed_ms (nullable): device time of the network. */
+int cudaDecon_rl_warm_start(cudaDecon_rl* h, cudaDecon_net* net, double* elapsed_ms);
+
 /* Synchronise the current device. */
 int cudaDecon_device_sync(void);
 
