@@ -86,7 +86,7 @@ SIGNATURES = {
     "cudaDecon_batch_get": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "cudaDecon_batch_destroy": (None, [C.c_void_p]),
     "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
-    "cudaDecon_rl_distance": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cudaDecon_rl_distance": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cudaDecon_net_create": (C.c_void_p, [C.c_ulonglong]),
     "cudaDecon_net_num_params": (C.c_long, []),
     "cudaDecon_net_get_params": (C.c_int, [C.c_void_p, _f]),
@@ -346,10 +346,10 @@ class RLEngine:
         """Keep a device copy of the current X as the reference of distance()."""
         _check(lib().cudaDecon_rl_snapshot(self.h), "cudaDecon_rl_snapshot")
 
-    def distance(self):
-        """||X - snapshot|| / ||snapshot|| computed on the device."""
+    def distance(self, seed=False):
+        """||X - snapshot|| / ||snapshot|| (or of the Y_0 seed) computed on the device."""
         r = np.zeros(1, np.float64)
-        _check(lib().cudaDecon_rl_distance(self.h, r.ctypes.data), "cudaDecon_rl_distance")
+        _check(lib().cudaDecon_rl_distance(self.h, int(bool(seed)), r.ctypes.data), "cudaDecon_rl_distance")
         return float(r[0])
 
     def warm_start(self, net):

@@ -678,13 +678,14 @@ int cudaDecon_rl_snapshot(cudaDecon_rl* h) {
   });
 }
 
-int cudaDecon_rl_distance(cudaDecon_rl* h, double* rel) {
+int cudaDecon_rl_distance(cudaDecon_rl* h, int which, double* rel) {
   return guard([&]() -> int {
-    if (!h || !rel) fail("cudaDecon_rl_distance: null argument");
+    if (!h || !rel || which < 0 || which > 1) fail("cudaDecon_rl_distance: bad argument");
     if (!h->ref.p) fail("cudaDecon_rl_distance: no snapshot taken");
     h->red.ensure(sizeof(double) * 3 * 1024);
     double o[2];
-    diff_norm(h->st, h->eng.result(), h->ref.as<float>(), h->eng.fft.nvox, h->red.as<double>(), o);
+    const float* v = which == 0 ? h->eng.result() : h->eng.y.as<float>();
+    diff_norm(h->st, v, h->ref.as<float>(), h->eng.fft.nvox, h->red.as<double>(), o);
     *rel = o[1] > 0 ? std::sqrt(o[0] / o[1]) : std::sqrt(o[0]);
     return 1;
   });

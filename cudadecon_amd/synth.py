@@ -121,10 +121,12 @@ CONFIGS = {
 }
 
 
-def make_stack(name: str, shape=None, beads=None):
+def make_stack(name: str, shape=None, beads=None, seed=None):
     """uint16 raw stack and the PSF of a named workload (optionally resized;
-    bead count then scales with the volume)."""
+    bead count then scales with the volume; optionally another seed)."""
     c = dict(CONFIGS[name])
+    if seed is not None:
+        c["seed"] = int(seed)
     if shape is not None:
         vol0 = np.prod(c["shape"])
         c["beads"] = max(1, int(round(c["beads"] * np.prod(shape) / vol0))) if beads is None else beads

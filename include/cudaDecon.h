@@ -157,9 +157,10 @@ int cudaDecon_batch_get(cudaDecon_batch* h, int i, float* X);
 void cudaDecon_batch_destroy(cudaDecon_batch* h);
 
 /* Reference copy of the engine's current X (snapshot) and the relative L2
- * distance ||X - ref|| / ||ref|| of the current X to it (FP64 sums). */
+ * distance ||V - ref|| / ||ref|| (FP64 sums) of V = the current X
+ * (which = 0) or the Y_0 seed buffer (which = 1). */
 int cudaDecon_rl_snapshot(cudaDecon_rl* h);
-int cudaDecon_rl_distance(cudaDecon_rl* h, double* rel);
+int cudaDecon_rl_distance(cudaDecon_rl* h, int which, double* rel);
 
 /* Learned warm start: a 3-level 3-D U-Net (16/32/64 channels, 3x3x3
  * convolutions, ReLU, average pooling, nearest upsampling with additive
