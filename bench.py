@@ -103,6 +103,31 @@ def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int):
     }
 
 
+def warm_start_eval(cdn, iters=50):
+    """Learned warm start vs plain RL on a 256x1024x1024 light-sheet stack
+    (made as the 128x512x512 workload, seed 3, not a training seed): relative
+    L2 to the plain 50-iteration result after k iterations, device times, and
+    the time each needs to reach a residual."""
+    from cudadecon_amd import warmstart as ws
+    from cudadecon_amd.otf import make_otf
+    p, meta = ws.load_params()
+    net = cdn.WarmNet(params=p)
+    t0 = time.perf_counter()
+    R, psf = ws.light_sheet_stack(ws.EVAL_STACK, ws.EVAL_SEED)
+    tab = make_otf(psf)
+    gen = time.perf_counter() - t0
+    res = ws.evaluate(net, R, tab, ws.kscale_for(R.shape, tab), iters=iters)
+    net.close()
+    curves = res.pop("curves")
+    res["curves_every_5"] = {k: v[::5] for k, v in curves.items()}
+    res["stack_zyx"] = list(ws.EVAL_STACK)
+    res["reference"] = f"plain RL-BA after {iters} iterations (relative L2)"
+    res["weights"] = {k: meta[k] for k in ("arch", "patches", "steps", "train_ms", "validation_seed_rel")
+                      if k in meta}
+    res["setup_s"] = round(gen, 1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +141,7 @@ def main():
     ap.add_argument("--no-e2e", action="store_true", help="skip the RL_interface end-to-end timing")
     ap.add_argument("--e2e-iters", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-stack batch measurement")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1 "
@@ -178,6 +204,7 @@ def main():
                "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
     batch = None if args.no_batch else batch_cfg2(cdn, tab)
+    warm = None if args.no_warm else warm_start_eval(cdn)
 
     Rb = 4.0 * nvox
     Pb = 8.0 * nspec
@@ -216,7 +243,8 @@ def main():
             "beads": c["beads"],
             "otf": "radially averaged 17x33 complex64 from the 33^3 Gaussian PSF",
             "hbm_resident": True,
-            "warm_start": None,
+            "warm_start": "3-level BF16 U-Net seed of Y_0, evaluated on a 256x1024x1024 light-sheet stack "
+                          "(field warm_start)",
         },
         "hbm_iteration": {
             "alg_bytes": int(b_iter),
@@ -239,6 +267,7 @@ def main():
         "passes": passes,
         "rl_interface_e2e": e2e,
         "batch_cfg2": batch,
+        "warm_start": warm,
         "setup_s": round(gen_s, 1),
     }
     if rank == 0 and not args.no_cpu_baseline:

@@ -86,6 +86,8 @@ SIGNATURES = {
     "cudaDecon_batch_get": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "cudaDecon_batch_destroy": (None, [C.c_void_p]),
     "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
+    "cudaDecon_wiener": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_float, C.c_float,
+                                   C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
     "cudaDecon_rl_distance": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cudaDecon_net_create": (C.c_void_p, [C.c_ulonglong]),
     "cudaDecon_net_num_params": (C.c_long, []),
@@ -266,6 +268,34 @@ def filter_volume(v, otf_table, kscale, conj=False):
                                   float(kscale[0]), float(kscale[1]), float(kscale[2]), int(conj)),
            "cudaDecon_filter")
     return v
+
+
+def wiener_filter(v, otf_table, kscale, dkxy, rcutoff, wiener, background=0.0):
+    """One-step Wiener filter of a (nz, ny, nx) volume (cudaDecon_wiener);
+    otf_table is the rotationally averaged (nr, nzotf) complex table."""
+    out = _f32(v).copy()
+    nz, ny, nx = out.shape
+    img = otf_image(otf_table)
+    d, h, w = img.shape
+    flat = np.ascontiguousarray(img).reshape(-1)
+    _check(lib().cudaDecon_wiener(out, nx, ny, nz, flat, w, h, float(kscale[0]), float(kscale[1]), float(kscale[2]),
+                                  float(dkxy[0]), float(dkxy[1]), float(rcutoff), float(wiener), float(background)),
+           "cudaDecon_wiener")
+    return out
+
+
+def camcor(raw_u16, camparam):
+    """Camera correction (camcor_interface_init + camcor_interface) of a
+    (nz, ny, nx) uint16 stack; camparam (3, ny, nx) = (a, b, offset)."""
+    raw = np.ascontiguousarray(raw_u16, dtype=np.uint16)
+    cam = np.ascontiguousarray(camparam, dtype=np.float32)
+    nz, ny, nx = raw.shape
+    if cam.shape != (3, ny, nx):
+        raise ValueError("camparam must be (3, ny, nx)")
+    _check(lib().camcor_interface_init(nx, ny, nz, cam.ctypes.data), "camcor_interface_init")
+    out = np.empty_like(raw)
+    _check(lib().camcor_interface(raw.ctypes.data, nx, ny, nz, out.ctypes.data), "camcor_interface")
+    return out
 
 
 class RLEngine:
@@ -460,12 +490,15 @@ class WarmNet:
             r = r[None]
         return np.ascontiguousarray(r)
 
-    def forward(self, raw, seed=False):
-        """y (normalised output), or the RL seed when seed=True; raw (nb,nz,ny,nx) or (nz,ny,nx)."""
+    def forward(self, raw, seed=False, inference=False):
+        """y (normalised output), or the RL seed when seed=True; raw (nb,nz,ny,nx)
+        or (nz,ny,nx).  inference=True runs the path of RLEngine.warm_start
+        (full-size e1 recomputed in the head, not stored)."""
         r = self._batch(raw)
         out = np.empty_like(r)
         nb, nz, ny, nx = r.shape
-        _check(lib().cudaDecon_net_forward(self.h, r, nb, nz, ny, nx, int(bool(seed)), out), "cudaDecon_net_forward")
+        mode = int(bool(seed)) + (2 if inference else 0)
+        _check(lib().cudaDecon_net_forward(self.h, r, nb, nz, ny, nx, mode, out), "cudaDecon_net_forward")
         return out.reshape(np.shape(raw))
 
     def fetch(self, name):

@@ -92,6 +92,18 @@ void rotated_dims(const float* p, unsigned nx, unsigned nz, int* nx_o, int* nz_o
   *nx_o = (int)((float)nx * p[3] + (float)nz * p[2] * p[2] / std::fabs(p[1]));
 }
 
+// camera-correction parameters of camcor_interface_init (process global, as
+// the reference's intended context, src/llspy.cpp:70-77)
+struct Camcor {
+  hipStream_t st = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+  DevBuf cam, in, out;
+};
+Camcor& camcor_state() {
+  static Camcor c;
+  return c;
+}
+
 template <class F> int guard(F&& f) {
   try {
     g_err.clear();
@@ -416,13 +428,37 @@ int Affine_interface_RA(const float* const raw_data, int nx, int ny, int nz, flo
   return affine_common(raw_data, nx, ny, nz, result, affMat, 1, dx, dy, dz);
 }
 
-int camcor_interface_init(int, int, int, const float* const) {
-  g_err = "camcor_interface_init: camera correction is not part of this library";
-  return 0;
+int camcor_interface_init(int nx, int ny, int nz, const float* const camparam) {
+  return guard([&]() -> int {
+    if (!camparam || nx <= 0 || ny <= 0 || nz <= 0) fail("camcor_interface_init: bad argument");
+    Camcor& C = camcor_state();
+    if (!C.st) C.st = make_stream();
+    const size_t b = sizeof(float) * 3 * (size_t)nx * ny;
+    C.cam.ensure(b);
+    CDN_HIP(hipMemcpyAsync(C.cam.p, camparam, b, hipMemcpyHostToDevice, C.st));
+    CDN_HIP(hipStreamSynchronize(C.st));
+    C.nx = nx;
+    C.ny = ny;
+    C.nz = nz;
+    return 1;
+  });
 }
-int camcor_interface(const unsigned short* const, int, int, int, unsigned short* const) {
-  g_err = "camcor_interface: camera correction is not part of this library";
-  return 0;
+
+int camcor_interface(const unsigned short* const raw_data, int nx, int ny, int nz, unsigned short* const result) {
+  return guard([&]() -> int {
+    Camcor& C = camcor_state();
+    if (!C.st || C.nx == 0) fail("camcor_interface called before camcor_interface_init");
+    if (!raw_data || !result || nz <= 0) fail("camcor_interface: bad argument");
+    if (nx != C.nx || ny != C.ny) fail("camcor_interface: nx, ny differ from camcor_interface_init");
+    const size_t n = (size_t)nx * ny * nz;
+    C.in.ensure(sizeof(uint16_t) * n);
+    C.out.ensure(sizeof(uint16_t) * n);
+    CDN_HIP(hipMemcpyAsync(C.in.p, raw_data, sizeof(uint16_t) * n, hipMemcpyHostToDevice, C.st));
+    camcor(C.st, C.in.as<uint16_t>(), C.cam.as<float>(), nx, ny, nz, C.out.as<uint16_t>());
+    CDN_HIP(hipMemcpyAsync(result, C.out.p, sizeof(uint16_t) * n, hipMemcpyDeviceToHost, C.st));
+    CDN_HIP(hipStreamSynchronize(C.st));
+    return 1;
+  });
 }
 
 int cudaDecon_device_sync(void) {
@@ -501,6 +537,48 @@ int cudaDecon_filter(float* vol, int nx, int ny, int nz, const float* otf, int o
       F.col(st, S, 1, false);
       launch_z_filter(nz, st, S, F.nxh, (long)ny * F.pitch, F.pitch, ny, conj, D.params(T, 1.0 / (double)F.nvox),
                       F.twz.as<float2>());
+      F.col(st, S, 1, true);
+      launch_x_c2r(nx, st, S, dv.as<float>(), (long)ny * nz, F.pitch, F.twx.as<float2>());
+      CDN_HIP(hipMemcpyAsync(vol, dv.p, sizeof(float) * F.nvox, hipMemcpyDeviceToHost, st));
+      CDN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    (void)hipStreamDestroy(st);
+    return 1;
+  });
+}
+
+int cudaDecon_wiener(float* vol, int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
+                     float kxscale, float kyscale, float kzscale, float dkx, float dky, float rcutoff,
+                     float wiener, float background) {
+  return guard([&]() -> int {
+    if (!vol || !otf) fail("cudaDecon_wiener: null argument");
+    if (!(rcutoff > 0.f)) fail("cudaDecon_wiener: rcutoff must be positive");
+    hipStream_t st = make_stream();
+    try {
+      Fft3 F;
+      F.setup(nx, ny, nz, st);
+      OtfTable T;
+      T.set_from_image(otf, otf_width, otf_height, 1, 1.f, 1.f);
+      if (T.nyotf != 1) fail("cudaDecon_wiener: needs a rotationally averaged OTF (depth 1)");
+      OtfDev D;
+      const float ks[3] = {kxscale, kyscale, kzscale};
+      D.setup(T, ks, nz, st);
+      DevBuf dv, ds;
+      dv.ensure(sizeof(float) * F.nvox);
+      ds.ensure(sizeof(float2) * F.nspec);
+      CDN_HIP(hipMemcpyAsync(dv.p, vol, sizeof(float) * F.nvox, hipMemcpyHostToDevice, st));
+      if (background != 0.f) add_scalar(st, dv.as<float>(), F.nvox, -background);
+      float2* S = ds.as<float2>();
+      launch_x_r2c(nx, st, dv.as<float>(), nullptr, nullptr, nullptr, XIN_PLAIN, S, (long)ny * nz, F.pitch,
+                   F.twx.as<float2>());
+      F.col(st, S, 1, false);
+      F.col(st, S, 2, false);
+      wiener_multiply(st, S, F.nxh, ny, nz, F.pitch, D.tz.as<float2>(), T.nxotf, kxscale, kyscale, dkx, dky,
+                      rcutoff, wiener * wiener, (float)(1.0 / (double)F.nvox));
+      F.col(st, S, 2, true);
       F.col(st, S, 1, true);
       launch_x_c2r(nx, st, S, dv.as<float>(), (long)ny * nz, F.pitch, F.twx.as<float2>());
       CDN_HIP(hipMemcpyAsync(vol, dv.p, sizeof(float) * F.nvox, hipMemcpyDeviceToHost, st));
@@ -827,13 +905,13 @@ void stage_batch(cudaDecon_net* h, const float* raw, const float* tgt, int nb, i
 
 int cudaDecon_net_forward(cudaDecon_net* h, const float* raw, int nb, int nz, int ny, int nx, int mode, float* out) {
   return guard([&]() -> int {
-    if (!h || !out || mode < 0 || mode > 1) fail("cudaDecon_net_forward: bad argument");
+    if (!h || !out || mode < 0 || mode > 3) fail("cudaDecon_net_forward: bad argument");
     stage_batch(h, raw, nullptr, nb, nz, ny, nx, false);
     const size_t n = (size_t)nb * nz * ny * nx;
     DevBuf o;
     o.ensure(sizeof(float) * n);
-    h->net.forward(h->st, h->net.b_raw.as<float>());
-    h->net.head(h->st, h->net.b_raw.as<float>(), mode, o.as<float>());
+    h->net.forward(h->st, h->net.b_raw.as<float>(), mode < 2);
+    h->net.head(h->st, h->net.b_raw.as<float>(), mode & 1, o.as<float>());
     CDN_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->st));
     CDN_HIP(hipStreamSynchronize(h->st));
     return 1;
@@ -929,7 +1007,7 @@ int cudaDecon_rl_warm_start(cudaDecon_rl* h, cudaDecon_net* net, double* elapsed
     CDN_HIP(hipEventCreate(&e1));
     CDN_HIP(hipStreamSynchronize(h->st));
     CDN_HIP(hipEventRecord(e0, h->st));
-    N.forward(h->st, h->eng.raw.as<float>());
+    N.forward(h->st, h->eng.raw.as<float>(), false);
     N.head(h->st, h->eng.raw.as<float>(), 1, h->eng.y.as<float>());
     CDN_HIP(hipEventRecord(e1, h->st));
     CDN_HIP(hipEventSynchronize(e1));

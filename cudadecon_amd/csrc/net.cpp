@@ -98,8 +98,8 @@ void WarmNet::geometry(int nb, int nz, int ny, int nx, bool training) {
   L3 = Vol{nb, nz / 4, ny / 4, nx / 4};
   const size_t n1 = (size_t)vox(L1), n2 = (size_t)vox(L2), n3 = (size_t)vox(L3), h = sizeof(bf16_t);
   inv_s.ensure(sizeof(float) * nb);
-  sum_part.ensure(sizeof(double) * (size_t)nb * net_sum_blocks());
-  e1.ensure(h * n1 * 16);
+  sum_part.ensure(sizeof(double) * (size_t)nb * net_sum_blocks(nb));
+  if (training) e1.ensure(h * n1 * 16);
   p1.ensure(h * n2 * 16);
   e2.ensure(h * n2 * 32);
   p2.ensure(h * n3 * 32);
@@ -125,11 +125,19 @@ void WarmNet::geometry(int nb, int nz, int ny, int nx, bool training) {
   }
 }
 
-void WarmNet::forward(hipStream_t st, const float* raw) {
+void WarmNet::forward(hipStream_t st, const float* raw, bool keep_e1) {
   const float* p = P.as<float>();
   net_inv_scale(st, raw, L1, sum_part.as<double>(), inv_s.as<float>());
-  net_conv_in1(st, raw, inv_s.as<float>(), wf[0].as<bf16_t>(), p + b_off[0], e1.as<bf16_t>(), L1);
-  net_pool(st, 16, e1.as<bf16_t>(), p1.as<bf16_t>(), L1);
+  if (keep_e1) {
+    e1.ensure(sizeof(bf16_t) * 16 * (size_t)vox(L1));
+    net_first(st, 0, raw, inv_s.as<float>(), wf[0].as<bf16_t>(), p + b_off[0], e1.as<bf16_t>(), nullptr, nullptr,
+              nullptr, L1);
+    net_pool(st, 16, e1.as<bf16_t>(), p1.as<bf16_t>(), L1);
+  } else {
+    net_first(st, 1, raw, inv_s.as<float>(), wf[0].as<bf16_t>(), p + b_off[0], p1.as<bf16_t>(), nullptr, nullptr,
+              nullptr, L1);
+  }
+  e1_kept = keep_e1;
   net_conv(st, 16, 32, true, p1.as<bf16_t>(), wf[1].as<bf16_t>(), p + b_off[1], e2.as<bf16_t>(), L2);
   net_pool(st, 32, e2.as<bf16_t>(), p2.as<bf16_t>(), L2);
   net_conv(st, 32, 64, true, p2.as<bf16_t>(), wf[2].as<bf16_t>(), p + b_off[2], e3.as<bf16_t>(), L3);
@@ -139,8 +147,12 @@ void WarmNet::forward(hipStream_t st, const float* raw) {
 }
 
 void WarmNet::head(hipStream_t st, const float* raw, int mode, float* out) {
-  net_head(st, mode, raw, inv_s.as<float>(), e1.as<bf16_t>(), d2.as<bf16_t>(), P.as<float>() + head_off, nullptr,
-           out, nullptr, L1);
+  if (e1_kept)
+    net_head(st, mode, raw, inv_s.as<float>(), e1.as<bf16_t>(), d2.as<bf16_t>(), P.as<float>() + head_off, nullptr,
+             out, nullptr, L1);
+  else
+    net_first(st, 2 + mode, raw, inv_s.as<float>(), wf[0].as<bf16_t>(), P.as<float>() + b_off[0], nullptr,
+              d2.as<bf16_t>(), P.as<float>() + head_off, out, L1);
 }
 
 void WarmNet::backward(hipStream_t st, const float* raw, const float* tgt, float* loss_slot) {
@@ -203,6 +215,7 @@ long WarmNet::fetch(int which, float* host, hipStream_t st) {
                    {&gA2, &L2, 32}, {&g4, &L3, 32}, {&gE3, &L3, 64}, {&g3, &L3, 64}, {&gP2, &L3, 32},
                    {&g2, &L2, 32}, {&gP1, &L2, 16}, {&g1, &L1, 16}};
   if (which < 0 || which >= (int)(sizeof(tab) / sizeof(tab[0]))) fail("cudaDecon_net_fetch: unknown tensor");
+  if (which == 0 && !e1_kept) fail("cudaDecon_net_fetch: e1 is not stored by the inference path");
   const T& t = tab[which];
   const long n = vox(*t.v) * t.c;
   if (!t.b->p || t.b->bytes < (size_t)n * (which == 8 ? 4 : 2)) fail("cudaDecon_net_fetch: tensor not computed");

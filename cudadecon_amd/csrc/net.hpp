@@ -47,10 +47,13 @@ struct WarmNet {
   void get_params(float* host, hipStream_t st) const;
   void pack(hipStream_t st);
   void geometry(int nb, int nz, int ny, int nx, bool training);
-  // activations of `raw` (device, nb samples of the current geometry)
-  void forward(hipStream_t st, const float* raw);
+  // activations of `raw` (device, nb samples of the current geometry); with
+  // keep_e1 = false (inference) the full-size e1 is not stored: the first
+  // layer writes only its pooled output and head() recomputes e1
+  void forward(hipStream_t st, const float* raw, bool keep_e1 = true);
   // mode 0: normalised output, 1: RL seed (both into out, device)
   void head(hipStream_t st, const float* raw, int mode, float* out);
+  bool e1_kept = false;
   // loss (device float at loss_slot) and the full gradient into Gr
   void backward(hipStream_t st, const float* raw, const float* tgt, float* loss_slot);
   void adam(hipStream_t st, float lr);

@@ -157,53 +157,165 @@ __global__ __launch_bounds__(THREADS) void k_conv(const bf16_t* __restrict__ in,
   }
 }
 
-// First layer: 1 -> 16 channels from the FP32 stack scaled by 1/s per sample;
-// K runs over the 27 taps (padded to 32).
-template <int MB, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_conv_in1(const float* __restrict__ raw, const float* __restrict__ inv_s,
-                                                      const bf16_t* __restrict__ wf, const float* __restrict__ bias,
-                                                      bf16_t* __restrict__ out, Vol v) {
-  constexpr int WAVES = THREADS / 64;
-  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+// First layer: 1 -> 16 channels from the FP32 stack scaled by 1/s per sample,
+// K running over the 27 taps (padded to 32).  A workgroup of 4 waves covers
+// 2 (z) x 8 (y) x 64 (x) voxels: wave w takes x0 + 16w .. x0 + 16w + 15 and
+// its 16 M-blocks are the 16 (z, y) rows.  The LDS tile holds bf16(raw / s)
+// of planes z0-1..z0+2, rows y0-1..y0+8, columns x0-1..x0+64 (0 outside the
+// volume), so every lane reads its 8 taps at fixed offsets.
+// MODE 0: write e1;  1: write pool(e1) only;  2: fused head, write y;
+// 3: fused head, write the RL seed max(y, 0) * s.  In the head modes e1 is
+// recomputed here and never stored; d2 and the head parameters hp[0..16]
+// complete y = u + bh + sum_c Wh[c] (e1[c] + d2[v/2][c]).
+constexpr int FT_X = 66, FT_Y = 10, FT_ROW = FT_Y * FT_X, FT_N = 4 * FT_ROW;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_first(const float* __restrict__ raw, const float* __restrict__ inv_s,
+                                               const bf16_t* __restrict__ wf, const float* __restrict__ bias,
+                                               bf16_t* __restrict__ out16, const bf16_t* __restrict__ d2,
+                                               const float* __restrict__ hp, float* __restrict__ out32, Vol v) {
+  __shared__ bf16_t tile[FT_N];
+  const int xb = (v.nx + 63) / 64, yb = (v.ny + 7) / 8, hz = v.nz / 2;
+  int blk = blockIdx.x;
+  const int xt = blk % xb;
+  blk /= xb;
+  const int yt = blk % yb;
+  blk /= yb;
+  const int zp = blk % hz;
+  const int b = blk / hz;
+  const int z0 = 2 * zp, y0 = 8 * yt, x0 = 64 * xt;
+  const float sc = inv_s[b];
+  const float* smp = raw + (long)b * v.nz * v.ny * v.nx;
+  // 40 tile rows of 66 columns; 16 threads per row, rows tid/16 + 16k.  All
+  // loads are issued before the first LDS store (one memory latency).
+  float val[3][5];
+#pragma unroll
+  for (int k3 = 0; k3 < 3; ++k3) {
+    const int rr = (threadIdx.x >> 4) + 16 * k3;
+    const int pp = rr / FT_Y, ry = rr - pp * FT_Y;
+    const int z = z0 - 1 + pp, y = y0 - 1 + ry;
+    const bool rok = rr < 4 * FT_Y && z >= 0 && z < v.nz && y >= 0 && y < v.ny;
+    const float* row = smp + ((long)(rok ? z : 0) * v.ny + (rok ? y : 0)) * v.nx;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int j = (threadIdx.x & 15) + 16 * k, x = x0 - 1 + j;
+      const bool ok = rok && j < FT_X && x >= 0 && x < v.nx;
+      const float f = row[ok ? x : 0];
+      val[k3][k] = ok ? f * sc : 0.f;
+    }
+  }
+#pragma unroll
+  for (int k3 = 0; k3 < 3; ++k3) {
+    const int rr = (threadIdx.x >> 4) + 16 * k3;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int j = (threadIdx.x & 15) + 16 * k;
+      if (rr < 4 * FT_Y && j < FT_X) tile[rr * FT_X + j] = f2bf(val[k3][k]);
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15, w = threadIdx.x >> 6;
   const u32x4 a = *reinterpret_cast<const u32x4*>(wf + lane * 8);
+  int off[8];
+  unsigned tm[8];  // 0xffff for a real tap, 0 for the padding taps 27..31
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int t = 8 * q + i;
+    int dz, dy, dx;
+    tap_offsets(t < 27 ? t : 26, dz, dy, dx);
+    tm[i] = t < 27 ? 0xffffu : 0u;
+    off[i] = (1 + dz) * FT_ROW + (1 + dy) * FT_X + (1 + dx) + 16 * w + r;
+  }
   float bl[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) bl[i] = bias[4 * q + i];
-  int tdz[8], tdy[8], tdx[8];
+  const int x = x0 + 16 * w + r;
+  const bool xin = x < v.nx;
+  float wh[4] = {0.f, 0.f, 0.f, 0.f}, bh = 0.f;
+  if (MODE >= 2) {
 #pragma unroll
-  for (int i = 0; i < 8; ++i) tap_offsets(8 * q + i, tdz[i], tdy[i], tdx[i]);
-  const int xchunks = (v.nx + 16 * MB - 1) / (16 * MB);
-  const long rows = (long)v.nb * v.nz * v.ny;
-  const long items = rows * xchunks;
-  for (long it = (long)blockIdx.x * WAVES + (threadIdx.x >> 6); it < items; it += (long)gridDim.x * WAVES) {
-    const long row = it / xchunks;
-    const int x0 = (int)(it - row * xchunks) * 16 * MB;
-    const int y = (int)(row % v.ny);
-    const long bzi = row / v.ny;
-    const int z = (int)(bzi % v.nz);
-    const long b = bzi / v.nz;
-    const float sc = inv_s[b];
-    const float* smp = raw + b * (long)v.nz * v.ny * v.nx;
-    f32x4 acc[MB];
+    for (int i = 0; i < 4; ++i) wh[i] = hp[4 * q + i];
+    bh = hp[16];
+  }
+  float ps[4][4];  // MODE 1: pooled sums per y pair
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      float u[8];
+  for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int zz = z + tdz[i], yy = y + tdy[i], xx = x0 + 16 * mb + r + tdx[i];
-        const bool ok = 8 * q + i < 27 && zz >= 0 && zz < v.nz && yy >= 0 && yy < v.ny && xx >= 0 && xx < v.nx;
-        u[i] = ok ? smp[((long)zz * v.ny + yy) * v.nx + xx] * sc : 0.f;
-      }
-      u32x4 bfr;
+    for (int i = 0; i < 4; ++i) ps[k][i] = 0.f;
+  // head modes: the 4 coarse d2 fragments and the 16 FP32 inputs of this
+  // lane's voxels, loaded before the MFMA loop
+  u32x2 dfr[4];
+  float uin[16];
+  if (MODE >= 2) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) bfr[i] = pack2(u[2 * i], u[2 * i + 1]);
-      acc[mb] = Mfma<32>::run(a, bfr, f32x4{0.f, 0.f, 0.f, 0.f});
+    for (int k = 0; k < 4; ++k) {
+      const int yc = (y0 >> 1) + k;
+      const bool ok = xin && 2 * yc < v.ny;
+      const long cw = (((long)b * hz + zp) * (v.ny / 2) + (ok ? yc : 0)) * (v.nx / 2) + (ok ? (x >> 1) : 0);
+      const u32x2 d = *reinterpret_cast<const u32x2*>(d2 + cw * 16 + 4 * q);
+      dfr[k] = ok ? d : u32x2(0u);
     }
-    bf16_t* dst = out + row * (long)v.nx * 16;
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      const int x = x0 + 16 * mb + r;
-      if (x < v.nx) store4<true>(dst + (long)x * 16 + 4 * q, acc[mb], bl);
+    for (int m = 0; m < 16; ++m) {
+      const int y = y0 + (m & 7);
+      const bool ok = xin && y < v.ny && q == 0;
+      const long vox = (((long)b * v.nz + z0 + (m >> 3)) * v.ny + (ok ? y : 0)) * v.nx + (ok ? x : 0);
+      const float f = raw[vox];
+      uin[m] = ok ? f * sc : 0.f;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int mz = m >> 3, my = m & 7;
+    const int base = mz * FT_ROW + my * FT_X;
+    u32x4 bf;
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      const unsigned l16 = (unsigned)tile[base + off[i]] & tm[i];
+      const unsigned h16 = (unsigned)tile[base + off[i + 1]] & tm[i + 1];
+      bf[i / 2] = l16 | (h16 << 16);
+    }
+    const f32x4 acc = Mfma<32>::run(a, bf, f32x4{0.f, 0.f, 0.f, 0.f});
+    const int z = z0 + mz, y = y0 + my;
+    const bool yin = y < v.ny;
+    const long vox = (((long)b * v.nz + z) * v.ny + y) * v.nx + x;
+    if (MODE == 0) {
+      if (xin && yin) store4<true>(out16 + vox * 16 + 4 * q, acc, bl);
+      continue;
+    }
+    float e[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) e[i] = bf2f(f2bf(fmaxf(acc[i] + bl[i], 0.f)));  // e1 as stored by MODE 0
+    if (MODE == 1) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ps[my >> 1][i] += e[i];
+      continue;
+    }
+    const u32x2 d = dfr[my >> 1];
+    const float dd[4] = {lo(d[0]), hi(d[0]), lo(d[1]), hi(d[1])};
+    float part = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part += wh[i] * (e[i] + dd[i]);
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    if (xin && yin && q == 0) {
+      const float yv = uin[m] + bh + part;
+      out32[vox] = MODE == 2 ? yv : fmaxf(yv, 0.f) / sc;
+    }
+  }
+  if (MODE == 1) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ps[k][i] += __shfl_xor(ps[k][i], 1);
+      const int yc = (y0 >> 1) + k;
+      if (xin && (r & 1) == 0 && 2 * yc < v.ny) {
+        const long cw = (((long)b * hz + zp) * (v.ny / 2) + yc) * (v.nx / 2) + (x >> 1);
+        u32x2 o;
+        o[0] = pack2(0.125f * ps[k][0], 0.125f * ps[k][1]);
+        o[1] = pack2(0.125f * ps[k][2], 0.125f * ps[k][3]);
+        *reinterpret_cast<u32x2*>(out16 + cw * 16 + 4 * q) = o;
+      }
     }
   }
 }
@@ -708,8 +820,17 @@ void conv_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float
   constexpr int THREADS = 27 * CIN * COUT * 2 > 64 * 1024 ? 1024 : 256;
   const long items = (long)v.nb * v.nz * v.ny * ((v.nx + 16 * MB - 1) / (16 * MB));
   const int waves = THREADS / 64;
-  const int cap = THREADS == 1024 ? 256 : 1024;
-  const int grid = (int)std::max(1L, std::min<long>((items + waves - 1) / waves, cap));
+  // grid-stride kernel: launch what is resident at once (no tail round)
+  static int resident = 0;
+  if (resident == 0) {
+    int per_cu = 0, dev = 0, cus = 0;
+    CDN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv<CIN, COUT, MB, THREADS, RELU, BIAS>,
+                                                         THREADS, 0));
+    CDN_HIP(hipGetDevice(&dev));
+    CDN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    resident = std::max(1, per_cu) * std::max(1, cus);
+  }
+  const int grid = (int)std::max(1L, std::min<long>((items + waves - 1) / waves, resident));
   hipLaunchKernelGGL((k_conv<CIN, COUT, MB, THREADS, RELU, BIAS>), dim3(grid), dim3(THREADS), 0, st, in, wf, bias,
                      out, v);
 }
@@ -745,12 +866,25 @@ void net_conv(hipStream_t st, int cin, int cout, bool relu, const bf16_t* in, co
   fail("net_conv: unsupported channel pair");
 }
 
-void net_conv_in1(hipStream_t st, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,
-                  bf16_t* out, Vol v) {
-  constexpr int MB = 4, THREADS = 256;
-  const long items = (long)v.nb * v.nz * v.ny * ((v.nx + 16 * MB - 1) / (16 * MB));
-  const int grid = (int)std::max(1L, std::min<long>((items + 3) / 4, 2048));
-  hipLaunchKernelGGL((k_conv_in1<MB, THREADS>), dim3(grid), dim3(THREADS), 0, st, raw, inv_s, wf, bias, out, v);
+void net_first(hipStream_t st, int mode, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,
+               bf16_t* out16, const bf16_t* d2, const float* hp, float* out32, Vol v) {
+  const long grid = (long)v.nb * (v.nz / 2) * ((v.ny + 7) / 8) * ((v.nx + 63) / 64);
+  if (grid > 0x7fffffffL) fail("net_first: volume too large");
+  switch (mode) {
+    case 0:
+      hipLaunchKernelGGL(k_first<0>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      return;
+    case 1:
+      hipLaunchKernelGGL(k_first<1>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      return;
+    case 2:
+      hipLaunchKernelGGL(k_first<2>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      return;
+    case 3:
+      hipLaunchKernelGGL(k_first<3>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      return;
+  }
+  fail("net_first: mode");
 }
 
 void net_pool(hipStream_t st, int c, const bf16_t* in, bf16_t* out, Vol fine) {
@@ -833,11 +967,11 @@ void net_reduce_rows(hipStream_t st, const float* part, int rows, long cols, flo
   hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(cols, 256, 1024)), dim3(256), 0, st, part, rows, cols, out, 1.0);
 }
 
-int net_sum_blocks() { return 64; }
+int net_sum_blocks(int nb) { return std::max(64, 2048 / std::max(nb, 1)); }
 
 void net_inv_scale(hipStream_t st, const float* raw, Vol v, double* part, float* inv_s) {
   const long per = (long)v.nz * v.ny * v.nx;
-  const int blocks = net_sum_blocks();
+  const int blocks = net_sum_blocks(v.nb);
   hipLaunchKernelGGL(k_sample_sum, dim3(v.nb * blocks), dim3(SUM_THREADS), 0, st, raw, per, blocks, part);
   hipLaunchKernelGGL(k_inv_scale, dim3((v.nb + 63) / 64), dim3(64), 0, st, part, v.nb, blocks, per, inv_s);
 }

@@ -15,9 +15,11 @@ struct Vol {
 // 3x3x3 convolution (cin in {16,32,64}); relu => bias + ReLU epilogue
 void net_conv(hipStream_t st, int cin, int cout, bool relu, const bf16_t* in, const bf16_t* wf, const float* bias,
               bf16_t* out, Vol v);
-// first layer: FP32 stack * inv_s[sample] -> 16 channels, bias + ReLU
-void net_conv_in1(hipStream_t st, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,
-                  bf16_t* out, Vol v);
+// first layer from the FP32 stack * inv_s[sample] (1 -> 16 channels, bias +
+// ReLU): mode 0 writes e1 (out16), 1 writes pool(e1) (out16, half size),
+// 2 / 3 also apply up(d2) and the head hp and write y / the RL seed (out32)
+void net_first(hipStream_t st, int mode, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,
+               bf16_t* out16, const bf16_t* d2, const float* hp, float* out32, Vol v);
 void net_pool(hipStream_t st, int c, const bf16_t* in, bf16_t* out, Vol fine);
 void net_upadd(hipStream_t st, int c, const bf16_t* coarse, const bf16_t* skip, bf16_t* out, Vol fine);
 int net_head_blocks(Vol v);
@@ -34,7 +36,7 @@ void net_chan_sum(hipStream_t st, int c, const bf16_t* t, long nvox, int blocks,
 void net_wgrad(hipStream_t st, int cin, int cout, const bf16_t* in, const bf16_t* g, Vol v, int G, float* part);
 void net_wgrad_in1(hipStream_t st, const float* raw, const float* inv_s, const bf16_t* g, Vol v, int G, float* part);
 void net_reduce_rows(hipStream_t st, const float* part, int rows, long cols, float* out);
-int net_sum_blocks();
+int net_sum_blocks(int nb);
 void net_inv_scale(hipStream_t st, const float* raw, Vol v, double* part, float* inv_s);
 void net_pack(hipStream_t st, const float* W, int cin, int cout, int flip, bf16_t* out);
 void net_adam(hipStream_t st, float* p, const float* g, float* m, float* v, long n, float lr_t, float b1, float b2,

@@ -325,6 +325,90 @@ void stats(hipStream_t st, const float* img, long n, double* d_part, double out[
   }
 }
 
+// camcor_kernel (src/camcor.cu:80-108): per pixel a, b, offset (three nx*ny
+// planes), previous plane z-1 (z = 0 takes plane nz-1, the texture's wrap
+// for the normalised coordinate (z - 0.5) / nz), and the float -> uint16
+// conversion of the reference's device code (truncation, saturating at 0
+// and 65535).  One thread walks the z column of one pixel, so each plane is
+// read once and the previous value stays in a register.
+__global__ void k_camcor(const uint16_t* __restrict__ raw, const float* __restrict__ cam, int nx, int ny, int nz,
+                         uint16_t* __restrict__ out) {
+  const long nxy = (long)nx * ny;
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= nxy) return;
+  const float a = cam[p], b = cam[nxy + p], off = cam[2 * nxy + p];
+  const float c = 0.9f * a;
+  float prev = (float)raw[(long)(nz - 1) * nxy + p];
+  for (int z = 0; z < nz; ++z) {
+    const uint16_t v = raw[(long)z * nxy + p];
+    const float e = expf(-b * (prev - off));
+    const float r = fmaf(-c, 1.f - e, (float)v - off);
+    out[(long)z * nxy + p] = r >= 65535.f ? (uint16_t)65535 : (r > 0.f ? (uint16_t)r : (uint16_t)0);
+    prev = (float)v;
+  }
+}
+void camcor(hipStream_t st, const uint16_t* raw, const float* cam, int nx, int ny, int nz, uint16_t* out) {
+  const long nxy = (long)nx * ny;
+  hipLaunchKernelGGL(k_camcor, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, st, raw, cam, nx, ny, nz, out);
+  CDN_HIP(hipGetLastError());
+}
+
+// p[i] += c
+__global__ void k_add_scalar(float* __restrict__ p, long n, float c) {
+  for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) p[i] += c;
+}
+void add_scalar(hipStream_t st, float* p, long n, float c) {
+  hipLaunchKernelGGL(k_add_scalar, dim3(nblocks(n)), dim3(BS), 0, st, p, n, c);
+  CDN_HIP(hipGetLastError());
+}
+
+// wienerfilter (src/linearDecon.cpp:116-168) on the transformed half
+// spectrum S (nz, ny, pitch): kr = |(kx dkx, ky dky)| (kx = 0..nx/2, ky
+// signed); kr <= rcut: S <- conj(H) S / (|H|^2 + w2) (1 - kr / rcut) / N,
+// else 0.  H is the rotationally averaged OTF interpolated as otfinterpolate
+// (src/linearDecon.cpp:78-114): the kz-interpolated table tz[r][kz] (the
+// engine's) and linear interpolation in kr = |(kx kxscale, ky kyscale)|,
+// zero for kr >= nr - 1.
+__global__ void k_wiener(float2* __restrict__ S, int nxh, int ny, int nz, long pitch, const float2* __restrict__ tz,
+                         int nr, float kxscale, float kyscale, float dkx, float dky, float rcut, float w2,
+                         float inv_n) {
+  const long n = (long)nz * ny * nxh;
+  for (long e = blockIdx.x * (long)BS + threadIdx.x; e < n; e += (long)gridDim.x * BS) {
+    const int c = (int)(e % nxh);
+    const long zy = e / nxh;
+    const int yi = (int)(zy % ny), kz = (int)(zy / ny);
+    const int kyi = yi > ny / 2 ? yi - ny : yi;
+    const float kx = (float)c, ky = (float)kyi;
+    const float kr = sqrtf(kx * kx * dkx * dkx + ky * ky * dky * dky);
+    float2* sp = S + ((long)kz * ny + yi) * pitch + c;
+    float2 out = make_float2(0.f, 0.f);
+    if (kr <= rcut) {
+      const float kxs = kx * kxscale, kys = ky * kyscale;
+      const float krr = sqrtf(kxs * kxs + kys * kys);
+      float2 h = make_float2(0.f, 0.f);
+      if (krr < (float)(nr - 1)) {
+        const int ir = (int)floorf(krr);
+        const float ar = krr - (float)ir;
+        const float2 a = tz[(long)ir * nz + kz], b = tz[(long)(ir + 1) * nz + kz];
+        h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
+      }
+      const float2 g = *sp;
+      const float amp2 = h.x * h.x + h.y * h.y;
+      const float f = (1.f - kr / rcut) / (amp2 + w2) * inv_n;
+      // conj(h) g
+      out = make_float2((h.x * g.x + h.y * g.y) * f, (h.x * g.y - h.y * g.x) * f);
+    }
+    *sp = out;
+  }
+}
+void wiener_multiply(hipStream_t st, float2* S, int nxh, int ny, int nz, long pitch, const float2* tz, int nr,
+                     float kxscale, float kyscale, float dkx, float dky, float rcut, float w2, float inv_n) {
+  const long n = (long)nz * ny * nxh;
+  hipLaunchKernelGGL(k_wiener, dim3(nblocks(n)), dim3(BS), 0, st, S, nxh, ny, nz, pitch, tz, nr, kxscale, kyscale,
+                     dkx, dky, rcut, w2, inv_n);
+  CDN_HIP(hipGetLastError());
+}
+
 // sum (a - b)^2 and sum b^2 (FP64) per block
 __global__ void __launch_bounds__(BS) k_diff_part(const float* __restrict__ a, const float* __restrict__ b, long n,
                                                   double* part) {

@@ -20,6 +20,12 @@ void fill(hipStream_t st, float* p, long n, float v);
 void stats(hipStream_t st, const float* img, long n, double* d_part, double out[3]);
 // sum (a-b)^2 and sum b^2 of n floats (FP64); d_part holds >= 2 * 1024 doubles.  Synchronises.
 void diff_norm(hipStream_t st, const float* a, const float* b, long n, double* d_part, double out[2]);
+// camera correction of a uint16 stack with 3 parameter planes (a, b, offset)
+void camcor(hipStream_t st, const uint16_t* raw, const float* cam, int nx, int ny, int nz, uint16_t* out);
+void add_scalar(hipStream_t st, float* p, long n, float c);
+// Wiener step on a transformed half spectrum with the kz-interpolated RA table
+void wiener_multiply(hipStream_t st, float2* S, int nxh, int ny, int nz, long pitch, const float2* tz, int nr,
+                     float kxscale, float kyscale, float dkx, float dky, float rcut, float w2, float inv_n);
 // bytes must be a multiple of 16
 void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes);
 void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam);

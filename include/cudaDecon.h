@@ -57,8 +57,12 @@ int Affine_interface(const float* const raw_data, int nx, int ny, int nz, float*
 int Affine_interface_RA(const float* const raw_data, int nx, int ny, int nz, float dx, float dy, float dz,
                         float* const result, const float* affMat);
 
-/* declared by the reference (src/linearDecon.h:298-303) but never built
- * into it; exported here as stubs that return 0. */
+/* declared by the reference (src/linearDecon.h:298-303; kernel
+ * src/camcor.cu:80-108, intended context src/llspy.cpp:70-91) but never
+ * exported by it.  init keeps the 3 parameter planes (a, b, offset; each
+ * nx*ny floats, x fastest); camcor_interface corrects a uint16 stack:
+ * out = v - offset - 0.9 a (1 - exp(-b (v_prev - offset))), v_prev the
+ * previous plane (plane nz-1 for z = 0), truncated and saturated to uint16. */
 int camcor_interface_init(int nx, int ny, int nz, const float* const camparam);
 int camcor_interface(const unsigned short* const raw_data, int nx, int ny, int nz, unsigned short* const result);
 
@@ -106,6 +110,17 @@ int cudaDecon_fft_c2r(const float* in, int nx, int ny, int nz, float* out);
  * out as its TIFF file (width, height, depth floats) with k-scales kx, ky, kz. */
 int cudaDecon_filter(float* vol, int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
                      int otf_depth, float kxscale, float kyscale, float kzscale, int conj);
+
+/* One-step Wiener filter, the CLI's alternative to RL (wienerfilter,
+ * src/linearDecon.cpp:116-168, called at :1116-1128), on the GPU, in place:
+ * v = C2R(conj(H) R2C(v - background) / (|H|^2 + wiener^2)
+ *         * (1 - kr / rcutoff)) / N  for kr = |(kx dkx, ky dky)| <= rcutoff,
+ * 0 beyond.  H: rotationally averaged OTF image (width 2 nzotf, height nr),
+ * k-scales as for RL; dkx = 1 / (dr nx), dky = 1 / (dr ny); the CLI uses
+ * rcutoff = 2 NA / wavelength. */
+int cudaDecon_wiener(float* vol, int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
+                     float kxscale, float kyscale, float kzscale, float dkx, float dky, float rcutoff,
+                     float wiener, float background);
 
 /* RL-Biggs-Andrews engine on HBM-resident state (the iteration of
  * RL_interface without pre/post-processing).  The OTF is given as its
@@ -178,7 +193,9 @@ long cudaDecon_net_num_params(void);
 int cudaDecon_net_get_params(cudaDecon_net* h, float* params);
 /* also clears the Adam state */
 int cudaDecon_net_set_params(cudaDecon_net* h, const float* params);
-/* mode 0: y, mode 1: the RL seed; out has the shape of raw */
+/* mode 0: y, mode 1: the RL seed; out has the shape of raw.  Modes 2 / 3:
+ * the same through the inference path (full-size e1 recomputed in the head
+ * instead of stored; what cudaDecon_rl_warm_start runs). */
 int cudaDecon_net_forward(cudaDecon_net* h, const float* raw, int nb, int nz, int ny, int nx, int mode, float* out);
 /* Copy an internal tensor of the last forward / loss_grad call to host as
  * float (channels-last).  which: 0 e1, 1 pool(e1), 2 e2, 3 pool(e2), 4 e3,

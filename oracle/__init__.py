@@ -417,3 +417,35 @@ def deskew_interface(raw, dz, dr, deskew_angle, output_width, extra_shift, pad_v
         else:
             xdim = int(output_width)
     return deskew(raw, fac, xdim, extra_shift, pad_val)
+
+
+def camcor(raw_u16: np.ndarray, cam: np.ndarray):
+    """camcor_kernel (src/camcor.cu:80-108) in FP64: out = v - off - 0.9 a
+    (1 - exp(-b (v_prev - off))) with v_prev the plane z-1 (plane nz-1 for
+    z = 0), truncated toward zero and saturated to [0, 65535].  cam is
+    (3, ny, nx) = (a, b, offset).  Returns (uint16 result, FP64 value before
+    the conversion)."""
+    v = raw_u16.astype(np.float64)
+    prev = np.roll(v, 1, axis=0)
+    a, b, off = (cam[i].astype(np.float64)[None] for i in range(3))
+    r = v - off - 0.9 * a * (1.0 - np.exp(-b * (prev - off)))
+    out = np.clip(np.trunc(r), 0, 65535).astype(np.uint16)
+    return out, r
+
+
+def wiener(raw: np.ndarray, raw_otf: np.ndarray, otf_dims, kscale, dkxy, rcutoff, wiener_const, background=0.0):
+    """wienerfilter + its call (src/linearDecon.cpp:116-168, 1116-1128) in
+    FP64: G = R2C(raw - background); for kr = sqrt(kx^2 dkx^2 + ky^2 dky^2) <=
+    rcutoff, G <- conj(H) G / (|H|^2 + w^2) (1 - kr / rcutoff), else 0;
+    result C2R(G) / N.  kx runs over the half axis 0..nx/2, ky signed."""
+    nz, ny, nx = raw.shape
+    H = otf_interp(raw_otf, otf_dims, (nx, ny, nz), kscale)
+    G = rfftn(raw.astype(np.float64) - background)
+    kx = np.arange(nx // 2 + 1, dtype=np.float64)
+    ky = np.arange(ny, dtype=np.float64)
+    ky = np.where(ky > ny // 2, ky - ny, ky)
+    kr = np.sqrt((kx[None, :] * dkxy[0]) ** 2 + (ky[:, None] * dkxy[1]) ** 2)
+    w2 = float(wiener_const) ** 2
+    apo = np.where(kr <= rcutoff, 1.0 - kr / rcutoff, 0.0)[None]
+    G = np.conj(H) * G / (np.abs(H) ** 2 + w2) * apo
+    return irfftn(G, nx) / raw.size

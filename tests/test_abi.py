@@ -71,9 +71,10 @@ def test_init_with_missing_otf_returns_zero(cdn, tmp_path):
     assert "cannot open" in cdn.last_error()
 
 
-def test_camcor_stubs_return_zero(cdn):
+def test_camcor_argument_errors(cdn):
     L = cdn.lib()
     assert L.camcor_interface_init(1, 1, 1, None) == 0
+    assert "bad argument" in cdn.last_error()
     assert L.camcor_interface(None, 1, 1, 1, None) == 0
 
 

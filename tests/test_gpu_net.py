@@ -65,6 +65,20 @@ def test_forward_layers_match_oracle(cdn, shape):
     assert s.min() >= 0.0
 
 
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_inference_path_matches_training_path(cdn, shape):
+    """The warm-start path (pooled first layer, e1 recomputed in the head)
+    against the path that stores e1: same BF16 values, other FP32 sum order."""
+    p = _params(2)
+    raw, _ = _stack(shape, 3)
+    net = cdn.WarmNet(params=p)
+    for seed in (False, True):
+        a = net.forward(raw, seed=seed)
+        b = net.forward(raw, seed=seed, inference=True)
+        assert _rel(b, a) <= 1e-3, (seed, _rel(b, a))
+        assert np.abs(b - a).max() <= 1e-2 * np.abs(a).max()
+
+
 @pytest.mark.parametrize("shape", SHAPES[:2], ids=lambda s: "x".join(map(str, s)))
 def test_gradients_match_oracle(cdn, shape):
     print("net grad", shape, flush=True)

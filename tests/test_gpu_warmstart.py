@@ -1,0 +1,45 @@
+"""What the trained warm-start network gains over plain RL-Biggs-Andrews.
+
+Held-out stack (64 x 512 x 512, made as the 128 x 512 x 512 light-sheet
+workload with seed 5; the weights were trained on seeds 100000+ and
+validated on 200000).  Reference: the 50-iteration plain RL result.  The
+network's seed followed by k iterations must be closer to it than plain RL
+after the same k iterations (k = 0, 2, 5, 10), and reaching the residual of
+the seed itself must take plain RL several times the network's time.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def curves(cdn):
+    from cudadecon_amd import warmstart as ws
+    from cudadecon_amd.otf import make_otf
+    p, meta = ws.load_params()
+    net = cdn.WarmNet(params=p)
+    R, psf = ws.light_sheet_stack((64, 512, 512), 5)
+    tab = make_otf(psf)
+    res = ws.evaluate(net, R, tab, ws.kscale_for(R.shape, tab), iters=50, warm_iters=12)
+    print({k: res[k] for k in ("iteration_ms", "net_ms", "plain", "warm")}, flush=True)
+    return res
+
+
+def test_warm_start_is_closer_at_equal_iterations(curves):
+    for k in ("0", "2", "5", "10"):
+        assert curves["warm"][k] <= 0.85 * curves["plain"][k], (k, curves["warm"][k], curves["plain"][k])
+
+
+def test_warm_start_reaches_its_residual_faster(curves):
+    r0 = curves["curves"]["warm"][0]
+    plain = curves["curves"]["plain"]
+    k_plain = next(k for k, v in enumerate(plain) if v <= r0)
+    t_plain = k_plain * curves["iteration_ms"]
+    assert k_plain >= 10
+    assert curves["net_ms"] <= 0.5 * t_plain, (curves["net_ms"], t_plain)
+
+
+def test_plain_curve_reaches_reference(curves):
+    plain = curves["curves"]["plain"]
+    assert plain[-1] == 0.0 and plain[0] > plain[10] > plain[40]

@@ -346,3 +346,56 @@ def test_init_geometry_cfg1(orc):
     assert g["deskewed_nx"] == 810
     assert g["out_dims"] == (791, 512, 192)
     assert abs(g["deskew_factor"] - math.cos(math.radians(31.5)) * 0.3 / 0.104) < 1e-6
+
+
+def test_camcor_closed_forms(orc):
+    """a = 0 (or b = 0) leaves v - offset; the previous plane of z = 0 is
+    plane nz-1; conversion truncates and saturates."""
+    rng = np.random.default_rng(0)
+    raw = rng.integers(0, 3000, (5, 4, 6)).astype(np.uint16)
+    cam = np.zeros((3, 4, 6), np.float32)
+    cam[2] = 100.25
+    out, r = orc.camcor(raw, cam)
+    assert np.array_equal(out, np.clip(np.trunc(raw.astype(np.float64) - 100.25), 0, 65535).astype(np.uint16))
+    # b = 0 also removes the correction whatever a is
+    cam[0] = 50.0
+    out0, _ = orc.camcor(raw, cam)
+    assert np.array_equal(out0, out)
+    # one pixel by hand, z = 0 uses plane nz-1
+    cam[1] = 0.01
+    out1, r1 = orc.camcor(raw, cam)
+    v, p = float(raw[0, 1, 2]), float(raw[-1, 1, 2])
+    want = v - 100.25 - 0.9 * 50.0 * (1 - np.exp(-0.01 * (p - 100.25)))
+    assert abs(r1[0, 1, 2] - want) < 1e-9
+    v, p = float(raw[3, 1, 2]), float(raw[2, 1, 2])
+    assert abs(r1[3, 1, 2] - (v - 100.25 - 45.0 * (1 - np.exp(-0.01 * (p - 100.25))))) < 1e-9
+    # saturation
+    big = np.full((2, 1, 2), 65535, np.uint16)
+    cz = np.zeros((3, 1, 2), np.float32)
+    cz[2] = -10.0
+    assert np.all(orc.camcor(big, cz)[0] == 65535)
+    cz[2] = 70000.0
+    assert np.all(orc.camcor(big, cz)[0] == 0)
+
+
+def test_wiener_closed_forms(orc):
+    """H = 1 everywhere inside the cut-off: the filter is 1 / (1 + w^2) times
+    the (1 - kr / rcutoff) apodisation; a cut-off above every kr with w = 0
+    and H = 1 is the identity up to the apodisation of the DC term (1)."""
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal((8, 12, 16))
+    tab = np.ones((40, 8), np.complex64)   # RA table, nr = 40 > every kr used
+    ks = (np.float32(1.0), np.float32(1.0), np.float32(1.0))
+    # rcutoff huge -> apodisation ~ 1; w = 0 -> identity
+    out = orc.wiener(v, tab, (40, 1, 8), ks, (1e-6, 1e-6), 1e3, 0.0)
+    assert np.abs(out - v).max() < 1e-6
+    # w = 1 halves everything
+    out = orc.wiener(v, tab, (40, 1, 8), ks, (1e-6, 1e-6), 1e3, 1.0)
+    assert np.abs(out - 0.5 * v).max() < 1e-6
+    # background is subtracted first
+    out = orc.wiener(v + 7.0, tab, (40, 1, 8), ks, (1e-6, 1e-6), 1e3, 0.0, background=7.0)
+    assert np.abs(out - v).max() < 1e-6
+    # a cut-off below the first lateral bin keeps only kx = ky = 0 (all kz):
+    # each plane becomes its mean
+    out = orc.wiener(v, tab, (40, 1, 8), ks, (1.0, 1.0), 0.5, 0.0)
+    assert np.abs(out - v.mean(axis=(1, 2), keepdims=True)).max() < 1e-9
