@@ -740,13 +740,21 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sample_sum(const float* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// inv_s[b] = n / sum (the inverse of the sample mean; 1 for an all-zero sample)
-__global__ void k_inv_scale(const double* __restrict__ part, int nb, int blocks, long per, float* __restrict__ inv_s) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+// inv_s[b] = n / sum (the inverse of the sample mean; 1 for an all-zero
+// sample); one block per sample, fixed summation order
+__global__ __launch_bounds__(SUM_THREADS) void k_inv_scale(const double* __restrict__ part, int blocks, long per,
+                                                           float* __restrict__ inv_s) {
+  const int b = blockIdx.x;
   double s = 0.0;
-  for (int k = 0; k < blocks; ++k) s += part[b * blocks + k];
-  inv_s[b] = s > 0.0 ? (float)((double)per / s) : 1.f;
+  for (int k = threadIdx.x; k < blocks; k += SUM_THREADS) s += part[(long)b * blocks + k];
+  __shared__ double red[SUM_THREADS];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = SUM_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) inv_s[b] = red[0] > 0.0 ? (float)((double)per / red[0]) : 1.f;
 }
 
 // Loss and head gradient from the head partials: loss = sum res^2 / n_total,
@@ -973,7 +981,7 @@ void net_inv_scale(hipStream_t st, const float* raw, Vol v, double* part, float*
   const long per = (long)v.nz * v.ny * v.nx;
   const int blocks = net_sum_blocks(v.nb);
   hipLaunchKernelGGL(k_sample_sum, dim3(v.nb * blocks), dim3(SUM_THREADS), 0, st, raw, per, blocks, part);
-  hipLaunchKernelGGL(k_inv_scale, dim3((v.nb + 63) / 64), dim3(64), 0, st, part, v.nb, blocks, per, inv_s);
+  hipLaunchKernelGGL(k_inv_scale, dim3(v.nb), dim3(SUM_THREADS), 0, st, part, blocks, per, inv_s);
 }
 
 void net_pack(hipStream_t st, const float* W, int cin, int cout, int flip, bf16_t* out) {

@@ -196,7 +196,8 @@ def evaluate(net, R, tab, ks, iters=TARGET_ITERS, ks_report=(0, 2, 5, 10), warm_
     best = int(np.argmin(warm))
     out["warm_best"] = {"k": best, "rel_l2": round(float(warm[best]), 6)}
     ttr = {}
-    for r in (1e-3, 1e-2, 2e-2, 5e-2):
+    levels = [1e-3, 1e-2, 5e-2, 0.15, 0.2, round(float(warm[0]), 6)]
+    for r in levels:
         kp, kw = first_k(plain, r), first_k(warm, r)
         tp = None if kp is None else kp * t_it
         tw = None if kw is None else t_net + kw * t_it
@@ -219,6 +220,10 @@ def main(argv=None):
     t.add_argument("--chunk", type=int, default=250)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--out", default=WEIGHTS)
+    i = sub.add_parser("infer", help="time the warm start (network on the device) on a synthetic stack")
+    i.add_argument("--weights", default=WEIGHTS)
+    i.add_argument("--shape", type=int, nargs=3, default=list(EVAL_STACK))
+    i.add_argument("--repeat", type=int, default=3)
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--shape", type=int, nargs=3, default=list(EVAL_STACK))
@@ -233,6 +238,18 @@ def main(argv=None):
     p, meta = load_params(a.weights)
     net = WarmNet(params=p)
     shape = tuple(a.shape)
+    if a.cmd == "infer":
+        # a stack of the evaluation kind is not needed for timing: background
+        # plus one block of signal, staged on the device
+        _, psf = light_sheet_stack((8, 8, 8), 0)
+        tab = make_otf(psf)
+        eng = RLEngine(shape, tab, kscale_for(shape, tab))
+        blk = np.random.default_rng(0).uniform(0, 50, (min(64, shape[0]), 64, 64)).astype(np.float32)
+        eng.set_raw_block(blk, (0, 0, 0), background=100.0)
+        for r in range(a.repeat):
+            log(f"warm start {shape}: {eng.warm_start(net):.3f} ms")
+        eng.close()
+        return 0
     t0 = time.perf_counter()
     R, psf = light_sheet_stack(shape, a.seed)
     tab = make_otf(psf)
