@@ -479,8 +479,9 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
 #endif
 template <int L>
 __device__ __forceinline__ long xcd_logical_block(long b, long G) {
-  // only tiles narrower than a 128-byte line share lines with a neighbour
-  if constexpr (CDN_XCD_REMAP == 0 || L * 8 >= 128) return b;
+  // 1: only tiles narrower than a 128-byte line (they share lines with a
+  // neighbour); 2: every tile (neighbouring tiles also share pages)
+  if constexpr (CDN_XCD_REMAP == 0 || (CDN_XCD_REMAP == 1 && L * 8 >= 128)) return b;
   const long q = G / 8, r = G % 8, x = b % 8;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
 }

@@ -80,9 +80,87 @@ def _fixorigin(avg: np.ndarray, nz: int, kx2: int) -> bool:
     return True
 
 
-def make_otf(psf: np.ndarray, interpkr: int = 10, background: float | None = None) -> np.ndarray:
+def _cleanup(avg: np.ndarray, nx: int, nz: int, dkr: float, dkz: float, lambdanm: int, krmax_user: int, NA: float,
+             NIMM: float):
+    # src/radialft_interface.cpp:383-411 (in place; float arithmetic)
+    f = np.float32
+    lamda = f(lambdanm * 0.001)
+    sinalpha = f(f(NA) / f(NIMM))
+    cosalpha = f(np.cos(np.arcsin(np.float64(sinalpha))))
+    krmax = f(f(2 * NA) / lamda)
+    if krmax_user * f(dkr) < krmax and krmax_user != 0:
+        krmax = f(krmax_user * f(dkr))
+    for ix in range(nx // 2 + 1):
+        kr = f(ix * f(dkr))
+        if kr <= krmax:
+            beta = f(np.arcsin(np.float64(f(f(NA) - kr * lamda) / f(NIMM))))
+            kzedge = f((f(NIMM) / lamda) * f(np.float64(np.cos(np.float64(beta))) - cosalpha))
+            kzstart = int(np.rint(np.float64(f(kzedge / f(dkz)) + 1)))
+            kzend = nz - kzstart
+            if kzstart <= kzend:
+                avg[ix, max(kzstart, 0):kzend + 1] = 0
+        else:
+            avg[ix, :] = 0
+
+
+def _centred_bands(psf: np.ndarray, background):
+    """R2C of PSF - background with the bead centre moved to the origin
+    (determine_center_and_background + shift_center)."""
+    psf = np.ascontiguousarray(psf, dtype=np.float32)
+    nz, ny, nx = psf.shape
+    (xc, yc, zc), est_bg = center_and_background(psf)
+    bg = np.float32(est_bg if background is None else background)
+    img = psf - bg
+    bands = np.fft.rfftn(img.astype(np.float32)).astype(np.complex64)  # (nz, ny, nx//2+1)
+    return bands, (xc, yc, zc)
+
+
+def make_otf_3d(psf: np.ndarray, interpkr: int = 10, background: float | None = None) -> np.ndarray:
+    """3-D OTF of a (nz, ny, nx) PSF stack (makeOTF with b3Dout = true,
+    src/radialft_interface.cpp:157-162): the centred half spectrum with the
+    (0, 0, 0) value extrapolated from the +-45 degree lines (:445-486) and
+    divided by its max |.| (:506-518).  Returns complex64 (nz, ny, nx//2+1)."""
+    bands, c = _centred_bands(psf, background)
+    nz, ny, nh = bands.shape
+    bands = _shift(bands, *c, psf.shape[2])
+    if interpkr > 1:
+        proj = np.zeros(nh, np.complex64)
+        for z in range(nz):
+            for x in range(1, min(nh, ny)):
+                proj[x] += bands[z, x, x]
+                proj[x] += bands[z, ny - x, x]
+        xs = np.arange(1, interpkr + 1, dtype=np.float64)
+        vals = proj[1:interpkr + 1].astype(np.complex128)
+        mean_val = vals.sum() / interpkr
+        mean_kx = (interpkr + 1) / 2.0
+        slope = np.sum((xs - mean_kx) * (vals - mean_val)) / np.sum((xs - mean_kx) ** 2)
+        bands[0, 0, 0] = np.complex64(mean_val - slope * mean_kx)
+    m = np.float32(np.abs(bands).max())
+    return (bands.view(np.float32) / m).view(np.complex64)
+
+
+def _shift(bands, xc, yc, zc, nx):
+    # shift_center (src/radialft_interface.cpp:286-319)
+    nz, ny, nh = bands.shape
+    kz = np.arange(nz)
+    kz = np.where(kz > nz // 2, kz - nz, kz)
+    ky = np.arange(ny)
+    ky = np.where(ky > ny // 2, ky - ny, ky)
+    kx = np.arange(nh)
+    dphiz = np.float32(2 * np.pi * float(zc) / nz)
+    dphiy = np.float32(2 * np.pi * float(yc) / ny)
+    dphix = np.float32(2 * np.pi * float(xc) / nx)
+    phi = (dphiz * kz.astype(np.float32))[:, None, None] + (dphiy * ky.astype(np.float32))[None, :, None] + \
+        (dphix * kx.astype(np.float32))[None, None, :]
+    phi = phi.astype(np.float32)
+    return (bands * (np.cos(phi) + 1j * np.sin(phi)).astype(np.complex64)).astype(np.complex64)
+
+
+def make_otf(psf: np.ndarray, interpkr: int = 10, background: float | None = None, cleanup: dict | None = None) -> np.ndarray:
     """Rotationally averaged OTF of a (nz, ny, nx) PSF stack (makeOTF with
-    b3Dout = false and no cleanup).  Returns complex64 (nx//2+1, nz)."""
+    b3Dout = false).  cleanup: None, or dict(lambdanm, NA, NIMM, dr, dz,
+    krmax) to zero the table outside the OTF support as bDoCleanup does.
+    Returns complex64 (nx//2+1, nz)."""
     psf = np.ascontiguousarray(psf, dtype=np.float32)
     nz, ny, nx = psf.shape
     (xc, yc, zc), est_bg = center_and_background(psf)
@@ -123,6 +201,11 @@ def make_otf(psf: np.ndarray, interpkr: int = 10, background: float | None = Non
             v = (avg[r, k] + np.conj(avg[r, nz - k])) / np.float32(2)
             avg[r, k] = v
             avg[r, nz - k] = np.conj(v)
+    if cleanup is not None:
+        c = dict(dict(lambdanm=520, NA=1.25, NIMM=1.3, dr=0.102, dz=0.102, krmax=0), **cleanup)
+        dkr = np.float32(1.0 / (ny * np.float32(c["dr"])))
+        dkz = np.float32(1.0 / (nz * np.float32(c["dz"])))
+        _cleanup(avg, nx, nz, dkr, dkz, c["lambdanm"], c["krmax"], c["NA"], c["NIMM"])
     if interpkr > 0:
         k2 = interpkr
         while not _fixorigin(avg, nz, k2):
@@ -173,3 +256,33 @@ def write_otf_tiff(path: str, otf: np.ndarray):
         write_tiff_f32(path, f.reshape(t.shape[0], 2 * t.shape[1]))
     else:
         write_tiff_f32(path, f.reshape(t.shape[0], t.shape[1], 2 * t.shape[2]))
+
+
+def read_tiff_f32(path: str) -> np.ndarray:
+    """Pages of an uncompressed 32-bit float TIFF (either byte order, any
+    strip layout) as a (pages, height, width) float32 array."""
+    buf = open(path, "rb").read()
+    bo = {b"II": "<", b"MM": ">"}[buf[:2]]
+    if struct.unpack(bo + "H", buf[2:4])[0] != 42:
+        raise ValueError(f"{path}: not a TIFF file")
+    size = {1: 1, 2: 1, 3: 2, 4: 4, 16: 8}
+    fmt = {1: "B", 2: "B", 3: "H", 4: "I", 16: "Q"}
+    pages = []
+    off = struct.unpack(bo + "I", buf[4:8])[0]
+    while off:
+        n = struct.unpack(bo + "H", buf[off:off + 2])[0]
+        tags = {}
+        for i in range(n):
+            e = off + 2 + 12 * i
+            tag, typ, cnt = struct.unpack(bo + "HHI", buf[e:e + 8])
+            sz = size[typ] * cnt
+            data = buf[e + 8:e + 8 + sz] if sz <= 4 else \
+                buf[struct.unpack(bo + "I", buf[e + 8:e + 12])[0]:][:sz]
+            tags[tag] = struct.unpack(bo + fmt[typ] * cnt, data)
+        w, h = tags[256][0], tags[257][0]
+        if tags.get(258, (32,))[0] != 32 or tags.get(339, (3,))[0] != 3 or tags.get(259, (1,))[0] != 1:
+            raise ValueError(f"{path}: only uncompressed 32-bit float pages are read here")
+        raw = b"".join(buf[o:o + c] for o, c in zip(tags[273], tags[279]))
+        pages.append(np.frombuffer(raw, dtype=bo + "f4", count=w * h).reshape(h, w))
+        off = struct.unpack(bo + "I", buf[off + 2 + 12 * n:off + 6 + 12 * n])[0]
+    return np.stack(pages).astype(np.float32)
