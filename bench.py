@@ -76,6 +76,21 @@ def batch_cfg2(cdn, tab, iters=10, nstreams=8):
         out[mode] = {"ms": round(ms, 2), "Gvoxel_iter_per_s": round(vox * iters / (ms * 1e-3) / 1e9, 2)}
     out["speedup"] = round(out["sequential"]["ms"] / out["graph"]["ms"], 3)
     b.close()
+    # the same batch after the autotuner picked launch shapes for its lengths
+    t0 = time.perf_counter()
+    rep = cdn.autotune(shapes, reps=3, use_cache=True)
+    out["autotune_s"] = round(time.perf_counter() - t0, 1)
+    out["tuned_lengths"] = {str(e["n"]): e["chosen"] for e in rep["lengths"]
+                            if any(v not in (0, -1) for v in e["chosen"].values())}
+    b = cdn.Batch(shapes, tab, ks, nstreams=nstreams)
+    for i in range(len(shapes)):
+        b.set_raw_block(i, blk, (8, 8, 8), 20.0)
+    b.run(iters, "graph")
+    ms = min(b.run(iters, "graph") for _ in range(2))
+    out["graph_tuned"] = {"ms": round(ms, 2), "Gvoxel_iter_per_s": round(vox * iters / (ms * 1e-3) / 1e9, 2)}
+    out["speedup_tuned"] = round(out["sequential"]["ms"] / ms, 3)
+    b.close()
+    cdn.autotune_clear()
     return out
 
 

@@ -86,6 +86,9 @@ SIGNATURES = {
     "cudaDecon_batch_get": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "cudaDecon_batch_destroy": (None, [C.c_void_p]),
     "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
+    "cudaDecon_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]),
+    "cudaDecon_autotune_clear": (C.c_int, []),
+    "cudaDecon_fft_variant": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cudaDecon_wiener": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_float, C.c_float,
                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
     "cudaDecon_rl_distance": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
@@ -268,6 +271,28 @@ def filter_volume(v, otf_table, kscale, conj=False):
                                   float(kscale[0]), float(kscale[1]), float(kscale[2]), int(conj)),
            "cudaDecon_filter")
     return v
+
+
+def autotune(shapes_zyx, reps=5, use_cache=True):
+    """Tune the FFT launch shapes for the grids (list of (nz, ny, nx)); the
+    choice applies to every later call in this process.  Returns the report."""
+    import json
+    dims = np.ascontiguousarray(np.array(shapes_zyx, dtype=np.int32).reshape(-1, 3))
+    buf = C.create_string_buffer(1 << 20)
+    _check(lib().cudaDecon_autotune(dims.ctypes.data, dims.shape[0], int(reps), int(bool(use_cache)), buf, len(buf)),
+           "cudaDecon_autotune")
+    return json.loads(buf.value.decode())
+
+
+def autotune_clear():
+    _check(lib().cudaDecon_autotune_clear(), "cudaDecon_autotune_clear")
+
+
+def fft_variant(n):
+    """None for the built-in shape of length n, else {cols, xthreads, col_nt_load}."""
+    v = (C.c_int * 3)()
+    tuned = lib().cudaDecon_fft_variant(int(n), C.byref(v, 0), C.byref(v, 4), C.byref(v, 8))
+    return dict(cols=v[0], xthreads=v[1], col_nt_load=v[2]) if tuned else None
 
 
 def wiener_filter(v, otf_table, kscale, dkxy, rcutoff, wiener, background=0.0):

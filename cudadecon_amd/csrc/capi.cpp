@@ -11,10 +11,12 @@
 #include "net.hpp"
 #include "rl_kernels.hpp"
 #include "tiff.hpp"
+#include "tune.hpp"
 
 #include <cfloat>
 #include <cmath>
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -590,6 +592,37 @@ int cudaDecon_wiener(float* vol, int nx, int ny, int nz, const float* otf, int o
     (void)hipStreamDestroy(st);
     return 1;
   });
+}
+
+int cudaDecon_autotune(const int* dims_zyx, int count, int reps, int use_cache, char* report, int report_len) {
+  return guard([&]() -> int {
+    if (!dims_zyx || count <= 0) fail("cudaDecon_autotune: no grids");
+    std::vector<std::array<int, 3>> g((size_t)count);
+    for (int i = 0; i < count; ++i) g[(size_t)i] = {dims_zyx[3 * i], dims_zyx[3 * i + 1], dims_zyx[3 * i + 2]};
+    const std::string r = autotune(g, reps, use_cache != 0);
+    if (report && report_len > 0) {
+      const size_t n = std::min(r.size(), (size_t)report_len - 1);
+      std::memcpy(report, r.data(), n);
+      report[n] = 0;
+    }
+    return 1;
+  });
+}
+
+int cudaDecon_autotune_clear(void) {
+  return guard([]() -> int {
+    fft_clear_variants();
+    return 1;
+  });
+}
+
+int cudaDecon_fft_variant(int n, int* cols, int* xthreads, int* col_nt_load) {
+  FftVariant v;
+  const bool tuned = fft_get_variant(n, &v);
+  if (cols) *cols = v.cols;
+  if (xthreads) *xthreads = v.xthreads;
+  if (col_nt_load) *col_nt_load = v.col_nt_load;
+  return tuned ? 1 : 0;
 }
 
 // ---- RL engine handle -----------------------------------------------------

@@ -167,7 +167,8 @@ void RLEngine::setup(int nx, int ny, int nz, const OtfTable& t, const float ksca
   g.ensure(vb);
   spec.ensure(sizeof(float2) * (size_t)fft.nspec);
   nb_part = x_update_blocks(nx, (long)ny * nz);
-  part.ensure(sizeof(double) * 2 * (size_t)nb_part);
+  // room for any x-pass launch shape the autotuner may install later
+  part.ensure(sizeof(double) * 2 * (size_t)x_update_blocks_max((long)ny * nz));
   cur = xa.as<float>();
   prev = xb.as<float>();
 }
@@ -247,7 +248,8 @@ void RLEngine::iterate(int n, int k0, bool seeded, hipStream_t st, double* lambd
         mark(PASS_XLR);
       }
     }
-    launch_x_update(fft.nx, st, S, yv, prev, g.as<float>(), part.as<double>(), k >= 1, rows, fft.pitch, twx);
+    nb_part = launch_x_update(fft.nx, st, S, yv, prev, g.as<float>(), part.as<double>(), k >= 1, rows, fft.pitch,
+                              twx);
     mark(PASS_XUPD);
     std::swap(cur, prev);
   }

@@ -7,6 +7,8 @@
 #include "jit.hpp"
 
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,41 +72,79 @@ static inline unsigned grid1(long n) {
   return (unsigned)n;
 }
 
+// ---------------------------------------------------------------------------
+// Tuned launch shapes per length (set by the autotuner).  A length with a
+// variant always runs that variant's run-time compiled kernels.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_vmu;
+std::map<int, FftVariant> g_variants;
+}  // namespace
+
+void fft_set_variant(int n, const FftVariant& v) {
+  std::lock_guard<std::mutex> lk(g_vmu);
+  if (v.is_default())
+    g_variants.erase(n);
+  else
+    g_variants[n] = v;
+}
+bool fft_get_variant(int n, FftVariant* v) {
+  std::lock_guard<std::mutex> lk(g_vmu);
+  auto it = g_variants.find(n);
+  if (it == g_variants.end()) return false;
+  if (v) *v = it->second;
+  return true;
+}
+void fft_clear_variants() {
+  std::lock_guard<std::mutex> lk(g_vmu);
+  g_variants.clear();
+}
+
 #define CDN_LAUNCH(kernel, N, grid, block, ...) \
   hipLaunchKernelGGL(kernel<N>, dim3(grid), dim3(block), 0, st, __VA_ARGS__)
 
 void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam, int mode,
                   float2* S, long rows, long pitch, const float2* tw) {
-  const XShape xs = x_shape(nx);
+  FftVariant tv;
+  const bool tuned = fft_get_variant(nx, &tv);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
-  switch (nx) {
+  void* args[] = {&X, &Xp, &Y, &lam, &mode, &S, &rows, &pitch, &tw};
+  if (tuned) {
+    jit_launch(nx, JK_X_R2C, grid, xs.THREADS, st, args, tv);
+  } else {
+    switch (nx) {
 #define CDN_CASE(N)                                                                     \
   case N:                                                                               \
     CDN_LAUNCH(k_x_r2c, N, grid, xs.THREADS, X, Xp, Y, lam, mode, S, rows, pitch, tw); \
     break;
-    CDN_FFT_LENGTHS(CDN_CASE)
+      CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default: {
-      void* args[] = {&X, &Xp, &Y, &lam, &mode, &S, &rows, &pitch, &tw};
-      jit_launch(nx, JK_X_R2C, grid, xs.THREADS, st, args);
+      default:
+        jit_launch(nx, JK_X_R2C, grid, xs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());
 }
 
 void launch_x_c2r(int nx, hipStream_t st, const float2* S, float* out, long rows, long pitch, const float2* tw) {
-  const XShape xs = x_shape(nx);
+  FftVariant tv;
+  const bool tuned = fft_get_variant(nx, &tv);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
-  switch (nx) {
+  void* args[] = {&S, &out, &rows, &pitch, &tw};
+  if (tuned) {
+    jit_launch(nx, JK_X_C2R, grid, xs.THREADS, st, args, tv);
+  } else {
+    switch (nx) {
 #define CDN_CASE(N)                                                      \
   case N:                                                                \
     CDN_LAUNCH(k_x_c2r, N, grid, xs.THREADS, S, out, rows, pitch, tw); \
     break;
-    CDN_FFT_LENGTHS(CDN_CASE)
+      CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default: {
-      void* args[] = {&S, &out, &rows, &pitch, &tw};
-      jit_launch(nx, JK_X_C2R, grid, xs.THREADS, st, args);
+      default:
+        jit_launch(nx, JK_X_C2R, grid, xs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());
@@ -112,42 +152,56 @@ void launch_x_c2r(int nx, hipStream_t st, const float2* S, float* out, long rows
 
 void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long rows, long pitch, const float2* tw,
                      float eps) {
-  const XShape xs = x_shape(nx);
+  FftVariant tv;
+  const bool tuned = fft_get_variant(nx, &tv);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
-  switch (nx) {
+  void* args[] = {&S, &raw, &rows, &pitch, &tw, &eps};
+  if (tuned) {
+    jit_launch(nx, JK_X_LRCORE, grid, xs.THREADS, st, args, tv);
+  } else {
+    switch (nx) {
 #define CDN_CASE(N)                                                              \
   case N:                                                                        \
     CDN_LAUNCH(k_x_lrcore, N, grid, xs.THREADS, S, raw, rows, pitch, tw, eps); \
     break;
-    CDN_FFT_LENGTHS(CDN_CASE)
+      CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default: {
-      void* args[] = {&S, &raw, &rows, &pitch, &tw, &eps};
-      jit_launch(nx, JK_X_LRCORE, grid, xs.THREADS, st, args);
+      default:
+        jit_launch(nx, JK_X_LRCORE, grid, xs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());
 }
 
 int x_update_blocks(int nx, long rows) {
-  const XShape xs = x_shape(nx);
+  FftVariant tv;
+  const bool tuned = fft_get_variant(nx, &tv);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
   return (int)(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
 }
 
+int x_update_blocks_max(long rows) { return (int)((rows + 1) / 2); }
+
 int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, float* Xout, float* G, double* part,
                     int dots, long rows, long pitch, const float2* tw) {
-  const XShape xs = x_shape(nx);
-  const int nb = x_update_blocks(nx, rows);
-  switch (nx) {
+  FftVariant tv;
+  const bool tuned = fft_get_variant(nx, &tv);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
+  const int nb = (int)(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
+  void* args[] = {&S, &Y, &Xout, &G, &part, &dots, &rows, &pitch, &tw};
+  if (tuned) {
+    jit_launch(nx, JK_X_UPDATE, grid1(nb), xs.THREADS, st, args, tv);
+  } else {
+    switch (nx) {
 #define CDN_CASE(N)                                                                                        \
   case N:                                                                                                  \
     CDN_LAUNCH(k_x_update, N, grid1(nb), xs.THREADS, S, Y, Xout, G, part, dots, rows, pitch, tw);        \
     break;
-    CDN_FFT_LENGTHS(CDN_CASE)
+      CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default: {
-      void* args[] = {&S, &Y, &Xout, &G, &part, &dots, &rows, &pitch, &tw};
-      jit_launch(nx, JK_X_UPDATE, grid1(nb), xs.THREADS, st, args);
+      default:
+        jit_launch(nx, JK_X_UPDATE, grid1(nb), xs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());
@@ -156,10 +210,16 @@ int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, flo
 
 void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols, long line_stride,
                     long outer_stride, long n_outer, const float2* tw) {
-  const CShape cs = c_shape(len);
+  FftVariant tv;
+  const bool tuned = fft_get_variant(len, &tv);
+  const CShape cs = c_shape(len, tuned ? tv.cols : -1);
   long ntiles = (ncols + cs.L - 1) / cs.L;
   const unsigned grid = grid1(ntiles * n_outer);
-  switch (len) {
+  void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &tw};
+  if (tuned) {
+    jit_launch(len, inverse ? JK_COL_I : JK_COL_F, grid, cs.THREADS, st, args, tv);
+  } else {
+    switch (len) {
 #define CDN_CASE(N)                                                                                            \
   case N:                                                                                                      \
     if (inverse)                                                                                               \
@@ -169,11 +229,10 @@ void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols,
       hipLaunchKernelGGL((k_col<N, false>), dim3(grid), dim3(cs.THREADS), 0, st, S, ncols, line_stride,        \
                          outer_stride, ntiles, tw);                                                            \
     break;
-    CDN_FFT_LENGTHS(CDN_CASE)
+      CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default: {
-      void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &tw};
-      jit_launch(len, inverse ? JK_COL_I : JK_COL_F, grid, cs.THREADS, st, args);
+      default:
+        jit_launch(len, inverse ? JK_COL_I : JK_COL_F, grid, cs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());
@@ -181,21 +240,26 @@ void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols,
 
 void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride, long ny,
                      int conj, const OtfParams& otf, const float2* tw) {
-  const CShape cs = c_shape(nz);
+  FftVariant tv;
+  const bool tuned = fft_get_variant(nz, &tv);
+  const CShape cs = c_shape(nz, tuned ? tv.cols : -1);
   long ntiles = (ncols + cs.L - 1) / cs.L;
   const unsigned grid = grid1(ntiles * ny);
   OtfParams op = otf;
-  switch (nz) {
+  void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &ny, &conj, &op, &tw};
+  if (tuned) {
+    jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args, tv);
+  } else {
+    switch (nz) {
 #define CDN_CASE(N)                                                                                      \
   case N:                                                                                                \
     CDN_LAUNCH(k_z_filter, N, grid, cs.THREADS, S, ncols, line_stride, outer_stride, ntiles, ny, conj,  \
                op, tw);                                                                                  \
     break;
-    CDN_FFT_LENGTHS(CDN_CASE)
+      CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
-    default: {
-      void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &ny, &conj, &op, &tw};
-      jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args);
+      default:
+        jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());

@@ -10,6 +10,7 @@
 #include <cstdint>
 
 #include "fft_types.hpp"
+#include "jit.hpp"
 
 namespace cdn {
 
@@ -39,6 +40,14 @@ void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long r
 int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, float* Xout, float* G,
                     double* part, int dots, long rows, long pitch, const float2* tw);
 int x_update_blocks(int nx, long rows);
+// upper bound of x_update_blocks over every launch shape (one row pair per block)
+int x_update_blocks_max(long rows);
+
+// Tuned launch shapes: a length with a variant runs that variant's run-time
+// compiled kernels (results are bitwise those of the default shapes).
+void fft_set_variant(int n, const FftVariant& v);
+bool fft_get_variant(int n, FftVariant* v);
+void fft_clear_variants();
 
 // Column FFT (forward or inverse) over `len` points at element stride
 // line_stride, for columns [0, ncols) of each of n_outer slabs.

@@ -97,9 +97,10 @@ constexpr int tpt_x(int N) {
 }
 
 // columns per tile for y/z passes: fill the LDS budget, at most 64 columns
-// and 1024 threads, power of two
-constexpr int lc_for(int N, int TPT) {
-  if (lookup_override(kColsOverride, N) > 0) return lookup_override(kColsOverride, N);
+// and 1024 threads, power of two (cols > 0 forces a width)
+constexpr int lc_for(int N, int TPT, int cols = -1) {
+  if (cols < 0) cols = lookup_override(kColsOverride, N);
+  if (cols > 0) return cols;
   const int budget = (N >= 768) ? 131072 : 65536;
   int L = budget / (N * 8);
   if (L > 64) L = 64;
@@ -115,12 +116,13 @@ struct XShape {
   int TPT, E, NL, PAD, LS, THREADS, EH;
   bool SHUFFLE;
 };
-constexpr XShape x_shape(int N) {
+// xthreads > 0 forces the workgroup size (the build-time override otherwise)
+constexpr XShape x_shape(int N, int xthreads = -1) {
   XShape s{};
   s.TPT = tpt_x(N);
   s.E = N / s.TPT;
   s.SHUFFLE = plan_exact(N, s.TPT) && s.TPT <= 64 && (64 % s.TPT) == 0 && (s.E % 2) == 0;
-  const int ov = lookup_override(kXThreadsOverride, N);
+  const int ov = xthreads >= 0 ? xthreads : lookup_override(kXThreadsOverride, N);
   const int xt = ov > 0 ? ov : 128;
   const int nl0 = (xt / s.TPT) < 1 ? 1 : (xt / s.TPT);
   s.NL = (nl0 * N * 8 > 65536) ? ((65536 / (N * 8)) < 1 ? 1 : 65536 / (N * 8)) : nl0;
@@ -133,10 +135,10 @@ constexpr XShape x_shape(int N) {
 struct CShape {
   int TPT, L, THREADS, E;
 };
-constexpr CShape c_shape(int N) {
+constexpr CShape c_shape(int N, int cols = -1) {
   CShape s{};
   s.TPT = tpt_col(N);
-  s.L = lc_for(N, s.TPT);
+  s.L = lc_for(N, s.TPT, cols);
   s.THREADS = s.TPT * s.L;
   s.E = N / s.TPT;
   return s;

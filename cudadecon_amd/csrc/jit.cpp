@@ -43,8 +43,8 @@ struct Loaded {
 };
 
 std::mutex g_mu;
-std::map<int, Compiled> g_code;
-std::map<std::pair<int, int>, Loaded> g_loaded;  // (device, length)
+std::map<std::pair<int, std::string>, Compiled> g_code;                 // (length, variant defines)
+std::map<std::pair<std::pair<int, int>, std::string>, Loaded> g_loaded;  // ((device, length), defines)
 
 std::string expr(int k, int n) {
   char b[64];
@@ -103,9 +103,17 @@ void store_cached(const std::string& path, const Compiled& c) {
     if (r_ != HIPRTC_SUCCESS) fail(std::string(#call) + ": " + hiprtcGetErrorString(r_)); \
   } while (0)
 
-Compiled compile(int n) {
+std::vector<std::string> split_opts(const std::string& s) {
+  std::vector<std::string> out;
+  std::istringstream is(s);
+  std::string w;
+  while (is >> w) out.push_back(w);
+  return out;
+}
+
+Compiled compile(int n, const std::string& defines) {
   const std::string src = "#include \"fft_kernels_dev.hpp\"\n";
-  std::string key = src + std::to_string(n);
+  std::string key = src + std::to_string(n) + defines;
   for (const char* o : kOpts) key += o;
   for (int i = 0; i < kJitHeaderCount; ++i) key += kJitHeaderText[i];
   char hex[32];
@@ -123,7 +131,10 @@ Compiled compile(int n) {
     exprs[k] = expr(k, n);
     CDN_RTC(hiprtcAddNameExpression(prog, exprs[k].c_str()));
   }
-  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(kOpts) / sizeof(kOpts[0])), kOpts);
+  const std::vector<std::string> extra = split_opts(defines);
+  std::vector<const char*> opts(std::begin(kOpts), std::end(kOpts));
+  for (const std::string& e : extra) opts.push_back(e.c_str());
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
   if (rc != HIPRTC_SUCCESS) {
     size_t ls = 0;
     hiprtcGetProgramLogSize(prog, &ls);
@@ -148,32 +159,50 @@ Compiled compile(int n) {
 
 }  // namespace
 
+std::string jit_cache_dir() { return cache_dir(); }
+
 bool jit_length_ok(int n) {
   if (n < 1 || n > kMaxJitLength) return false;
   return make_plan(n).ok;
 }
 
-void jit_prepare(int n) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  if (g_code.count(n)) return;
-  if (!jit_length_ok(n)) fail("FFT length " + std::to_string(n) + " is not 2,3,5,7-smooth or exceeds 16384");
-  g_code.emplace(n, compile(n));
+std::string FftVariant::defines(int n) const {
+  std::string d;
+  if (cols > 0) d += " -DCDN_COLS_OVERRIDE=" + std::to_string(n) + "," + std::to_string(cols);
+  if (xthreads > 0) d += " -DCDN_XTHREADS_OVERRIDE=" + std::to_string(n) + "," + std::to_string(xthreads);
+  if (col_nt_load >= 0) d += " -DCDN_NT_COL_LOAD=" + std::to_string(col_nt_load);
+  return d;
 }
 
-void jit_launch(int n, JitKernel k, unsigned grid, unsigned block, hipStream_t st, void** args) {
-  jit_prepare(n);
+void jit_prepare(int n, const FftVariant& v) {
+  const std::string d = v.defines(n);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_code.count({n, d})) return;
+  }
+  if (!jit_length_ok(n)) fail("FFT length " + std::to_string(n) + " is not 2,3,5,7-smooth or exceeds 16384");
+  // compile outside the lock, so that several variants can compile at once
+  Compiled c = compile(n, d);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_code.emplace(std::make_pair(n, d), std::move(c));
+}
+
+void jit_launch(int n, JitKernel k, unsigned grid, unsigned block, hipStream_t st, void** args,
+                const FftVariant& v) {
+  jit_prepare(n, v);
+  const std::string d = v.defines(n);
   int dev = 0;
   CDN_HIP(hipGetDevice(&dev));
   hipFunction_t fn;
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_loaded.find({dev, n});
+    auto it = g_loaded.find({{dev, n}, d});
     if (it == g_loaded.end()) {
-      const Compiled& c = g_code.at(n);
+      const Compiled& c = g_code.at({n, d});
       Loaded L;
       CDN_HIP(hipModuleLoadData(&L.mod, c.code.data()));
       for (int j = 0; j < JK_COUNT; ++j) CDN_HIP(hipModuleGetFunction(&L.fn[j], L.mod, c.names[j].c_str()));
-      it = g_loaded.emplace(std::make_pair(dev, n), L).first;
+      it = g_loaded.emplace(std::make_pair(std::make_pair(dev, n), d), L).first;
     }
     fn = it->second.fn[k];
   }

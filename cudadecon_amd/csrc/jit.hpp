@@ -5,15 +5,36 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 namespace cdn {
 
 enum JitKernel { JK_X_R2C = 0, JK_X_C2R, JK_X_LRCORE, JK_X_UPDATE, JK_COL_F, JK_COL_I, JK_Z_FILTER, JK_COUNT };
 
+// Launch-shape choices for the kernels of one length (0 / -1: the built-in
+// rule): columns per y/z tile, threads per x-pass workgroup, non-temporal
+// loads in the column FFT.  Picked by the autotuner (tune.cpp).
+struct FftVariant {
+  int cols = 0;
+  int xthreads = 0;
+  int col_nt_load = -1;
+  bool is_default() const { return cols == 0 && xthreads == 0 && col_nt_load < 0; }
+  // -D options that build this variant for length n
+  std::string defines(int n) const;
+  bool operator==(const FftVariant& o) const {
+    return cols == o.cols && xthreads == o.xthreads && col_nt_load == o.col_nt_load;
+  }
+};
+
+// cache directory ($XDG_CACHE_HOME or $HOME/.cache, /cudadecon_amd); "" if none
+std::string jit_cache_dir();
+
 // n is 2,3,5,7-smooth and within the lengths the kernels are sized for
 bool jit_length_ok(int n);
 // compile the kernels of length n (host only, cached); throws on failure
-void jit_prepare(int n);
+void jit_prepare(int n, const FftVariant& v = FftVariant());
 // launch kernel k of length n on the current device (loads its module once)
-void jit_launch(int n, JitKernel k, unsigned grid, unsigned block, hipStream_t st, void** args);
+void jit_launch(int n, JitKernel k, unsigned grid, unsigned block, hipStream_t st, void** args,
+                const FftVariant& v = FftVariant());
 
 }  // namespace cdn

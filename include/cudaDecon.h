@@ -122,6 +122,21 @@ int cudaDecon_wiener(float* vol, int nx, int ny, int nz, const float* otf, int o
                      float kxscale, float kyscale, float kzscale, float dkx, float dky, float rcutoff,
                      float wiener, float background);
 
+/* Autotuner of the FFT launch shapes.  For every length used by the grids
+ * (dims_zyx: 3 ints nz, ny, nx per grid) it times candidate shapes of the x
+ * passes (workgroup size) and of the y/z column passes (columns per tile,
+ * cached or non-temporal loads) on the largest grid using the length,
+ * compiles them at run time, and installs the fastest for all later calls
+ * of this process (results are bitwise unchanged).  use_cache: reuse and
+ * extend the decisions kept under the cache directory.  report (nullable):
+ * a JSON summary of the timings.  Do not run while other work of this
+ * library is in flight. */
+int cudaDecon_autotune(const int* dims_zyx, int count, int reps, int use_cache, char* report, int report_len);
+/* back to the built-in shapes for every length */
+int cudaDecon_autotune_clear(void);
+/* 1 if length n has a tuned shape (and its parameters, 0 / -1 = built-in) */
+int cudaDecon_fft_variant(int n, int* cols, int* xthreads, int* col_nt_load);
+
 /* RL-Biggs-Andrews engine on HBM-resident state (the iteration of
  * RL_interface without pre/post-processing).  The OTF is given as its
  * image (see cudaDecon_filter) and the k-scales of the grid. */
