@@ -143,6 +143,32 @@ def warm_start_eval(cdn, iters=50):
     return res
 
 
+def cfg4_pipeline(cdn):
+    """512x2048x2048 uint16 stack (bead blocks made as the light-sheet
+    workload on a flat 200-count background) through camera offset
+    subtraction, warm start, 10 RL-BA iterations, deskew and a small affine,
+    all resident in HBM; stage times."""
+    from cudadecon_amd import pipeline, warmstart
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import make_stack
+    shape = (512, 2048, 2048)
+    blk, psf, _ = make_stack("cfg1", shape=(64, 128, 128), seed=4)
+    raw = np.full(shape, 200, np.uint16)
+    for off in ((100, 300, 500), (300, 1500, 900), (420, 100, 1800)):
+        raw[off[0]:off[0] + 64, off[1]:off[1] + 128, off[2]:off[2] + 128] = blk
+    tab = make_otf(psf)
+    p, _ = warmstart.load_params()
+    net = cdn.WarmNet(params=p)
+    eng = cdn.RLEngine(shape, tab, warmstart.kscale_for(shape, tab, dz=0.4))
+    pipeline.run(eng, raw, net)   # allocates every buffer once
+    t = pipeline.run(eng, raw, net)
+    eng.close()
+    net.close()
+    t["stack_zyx"] = list(shape)
+    t["iteration_Gvoxel_s"] = round(np.prod(shape) * 10 / (t["iterations_ms"] * 1e-3) / 1e9, 2)
+    return t
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +183,7 @@ def main():
     ap.add_argument("--e2e-iters", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-stack batch measurement")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
+    ap.add_argument("--no-cfg4", action="store_true", help="skip the 512x2048x2048 pipeline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1 "
@@ -220,6 +247,7 @@ def main():
 
     batch = None if args.no_batch else batch_cfg2(cdn, tab)
     warm = None if args.no_warm else warm_start_eval(cdn)
+    big = None if args.no_cfg4 else cfg4_pipeline(cdn)
 
     Rb = 4.0 * nvox
     Pb = 8.0 * nspec
@@ -283,6 +311,7 @@ def main():
         "rl_interface_e2e": e2e,
         "batch_cfg2": batch,
         "warm_start": warm,
+        "cfg4_pipeline": big,
         "setup_s": round(gen_s, 1),
     }
     if rank == 0 and not args.no_cpu_baseline:

@@ -86,6 +86,13 @@ SIGNATURES = {
     "cudaDecon_batch_get": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "cudaDecon_batch_destroy": (None, [C.c_void_p]),
     "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
+    "cudaDecon_rl_set_raw_u16": (C.c_int, [C.c_void_p, _u16, C.c_float]),
+    "cudaDecon_rl_deskew": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
+                                      C.c_void_p]),
+    "cudaDecon_rl_affine": (C.c_int, [C.c_void_p, _f]),
+    "cudaDecon_rl_post_dims": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cudaDecon_rl_post_get": (C.c_int, [C.c_void_p, _f]),
+    "cudaDecon_rl_post_stats": (C.c_int, [C.c_void_p, _d]),
     "cudaDecon_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "cudaDecon_autotune_clear": (C.c_int, []),
     "cudaDecon_fft_variant": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -396,6 +403,41 @@ class RLEngine:
         ms = np.zeros(1, np.float64)
         _check(lib().cudaDecon_rl_copy_probe(self.h, int(n), ms.ctypes.data), "cudaDecon_rl_copy_probe")
         return float(ms[0])
+
+    def set_raw_u16(self, raw_u16, offset):
+        """Stage a uint16 stack as max(v - offset, 0) (camera offset subtraction on the device)."""
+        r = np.ascontiguousarray(raw_u16, dtype=np.uint16)
+        _check(lib().cudaDecon_rl_set_raw_u16(self.h, r, float(offset)), "cudaDecon_rl_set_raw_u16")
+
+    def deskew(self, dz, dr, angle, width=0, shift=0, pad_val=0.0):
+        """Deskew the current X on the device; returns the output width."""
+        nx = C.c_int(0)
+        _check(lib().cudaDecon_rl_deskew(self.h, float(dz), float(dr), float(angle), int(width), int(shift),
+                                         float(pad_val), C.byref(nx)), "cudaDecon_rl_deskew")
+        return nx.value
+
+    def affine(self, mat):
+        """Affine resampling of the last deskew result (else of X) on the device."""
+        m = _f32(np.asarray(mat, np.float32).reshape(-1))
+        if m.size != 16:
+            raise ValueError("affine matrix must have 16 entries (row-major 4x4)")
+        _check(lib().cudaDecon_rl_affine(self.h, m), "cudaDecon_rl_affine")
+
+    def post_shape(self):
+        v = (C.c_int * 3)()
+        _check(lib().cudaDecon_rl_post_dims(self.h, C.byref(v, 0), C.byref(v, 4), C.byref(v, 8)),
+               "cudaDecon_rl_post_dims")
+        return int(v[2]), int(v[1]), int(v[0])
+
+    def post_get(self):
+        out = np.empty(self.post_shape(), np.float32)
+        _check(lib().cudaDecon_rl_post_get(self.h, out), "cudaDecon_rl_post_get")
+        return out
+
+    def post_stats(self):
+        o = np.zeros(3, np.float64)
+        _check(lib().cudaDecon_rl_post_stats(self.h, o), "cudaDecon_rl_post_stats")
+        return tuple(o)
 
     def snapshot(self):
         """Keep a device copy of the current X as the reference of distance()."""

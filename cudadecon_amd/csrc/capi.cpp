@@ -140,6 +140,9 @@ struct cudaDecon_rl {
   RLEngine eng;
   int k_next = 0;
   DevBuf red, ref;  // reduction partials of the diagnostics (not the engine's), snapshot of X
+  DevBuf u16;       // staging of uint16 stacks
+  DevBuf post[2];   // device-resident deskew / affine results
+  int post_cur = -1, pnx = 0, pny = 0, pnz = 0;
   ~cudaDecon_rl() {
     if (st) (void)hipStreamDestroy(st);
   }
@@ -774,6 +777,97 @@ int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6) {
     h->red.ensure(sizeof(double) * 3 * 1024);
     stats(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, h->red.as<double>(), out6);
     stats(h->st, h->eng.result(), h->eng.fft.nvox, h->red.as<double>(), out6 + 3);
+    return 1;
+  });
+}
+
+int cudaDecon_rl_set_raw_u16(cudaDecon_rl* h, const unsigned short* raw, float offset) {
+  return guard([&]() -> int {
+    if (!h || !raw) fail("cudaDecon_rl_set_raw_u16: null argument");
+    const Fft3& F = h->eng.fft;
+    h->u16.ensure(sizeof(uint16_t) * (size_t)F.nvox);
+    CDN_HIP(hipMemcpyAsync(h->u16.p, raw, sizeof(uint16_t) * (size_t)F.nvox, hipMemcpyHostToDevice, h->st));
+    u16_to_f32_crop(h->st, h->u16.as<uint16_t>(), F.nx, F.ny, F.nz, h->eng.raw.as<float>(), F.nx, F.ny, F.nz);
+    bgsub(h->st, h->eng.raw.as<float>(), F.nvox, offset);  // src/RLgpuImpl.cu:124-132 with bg = offset
+    h->eng.reset(h->st);
+    CDN_HIP(hipStreamSynchronize(h->st));
+    h->k_next = 0;
+    h->post_cur = -1;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_deskew(cudaDecon_rl* h, float dz, float dr, float deskewAngle, int outputWidth, int extraShift,
+                        float padVal, int* nx_out) {
+  return guard([&]() -> int {
+    if (!h) fail("cudaDecon_rl_deskew: null handle");
+    const Fft3& F = h->eng.fft;
+    double factor = 0.;
+    int xdim = F.nx;
+    if (std::fabs(deskewAngle) > 0.0) {
+      double a = deskewAngle;
+      if (a < 0) a += 180.;
+      factor = std::cos(a * M_PI / 180.) * dz / dr;
+      xdim = outputWidth > 0 ? outputWidth
+                             : (int)find_optimal_dimension(
+                                   (unsigned)(F.nx + std::floor(F.nz * dz * std::fabs(std::cos(a * M_PI / 180.)) / dr)));
+    }
+    DevBuf& out = h->post[0];
+    out.ensure(sizeof(float) * (size_t)xdim * F.ny * F.nz);
+    deskew(h->st, h->eng.result(), F.nx, F.ny, F.nz, factor, out.as<float>(), xdim, extraShift, padVal);
+    CDN_HIP(hipStreamSynchronize(h->st));
+    h->post_cur = 0;
+    h->pnx = xdim;
+    h->pny = F.ny;
+    h->pnz = F.nz;
+    if (nx_out) *nx_out = xdim;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_affine(cudaDecon_rl* h, const float* affMat) {
+  return guard([&]() -> int {
+    if (!h || !affMat) fail("cudaDecon_rl_affine: null argument");
+    const Fft3& F = h->eng.fft;
+    const float* in = h->post_cur >= 0 ? h->post[h->post_cur].as<float>() : h->eng.result();
+    if (h->post_cur < 0) {
+      h->pnx = F.nx;
+      h->pny = F.ny;
+      h->pnz = F.nz;
+    }
+    const int dst = h->post_cur == 0 ? 1 : 0;
+    h->post[dst].ensure(sizeof(float) * (size_t)h->pnx * h->pny * h->pnz);
+    affine(h->st, in, h->pnx, h->pny, h->pnz, h->post[dst].as<float>(), affMat, 0, 0.f, 0.f, 0.f);
+    CDN_HIP(hipStreamSynchronize(h->st));
+    h->post_cur = dst;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_post_dims(cudaDecon_rl* h, int* nx, int* ny, int* nz) {
+  return guard([&]() -> int {
+    if (!h || h->post_cur < 0) fail("cudaDecon_rl_post_dims: no deskew or affine result");
+    if (nx) *nx = h->pnx;
+    if (ny) *ny = h->pny;
+    if (nz) *nz = h->pnz;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_post_get(cudaDecon_rl* h, float* out) {
+  return guard([&]() -> int {
+    if (!h || !out || h->post_cur < 0) fail("cudaDecon_rl_post_get: no deskew or affine result");
+    CDN_HIP(hipMemcpy(out, h->post[h->post_cur].p, sizeof(float) * (size_t)h->pnx * h->pny * h->pnz,
+                      hipMemcpyDeviceToHost));
+    return 1;
+  });
+}
+
+int cudaDecon_rl_post_stats(cudaDecon_rl* h, double* out3) {
+  return guard([&]() -> int {
+    if (!h || !out3 || h->post_cur < 0) fail("cudaDecon_rl_post_stats: no deskew or affine result");
+    h->red.ensure(sizeof(double) * 3 * 1024);
+    stats(h->st, h->post[h->post_cur].as<float>(), (long)h->pnx * h->pny * h->pnz, h->red.as<double>(), out3);
     return 1;
   });
 }

@@ -186,6 +186,24 @@ int cudaDecon_batch_run(cudaDecon_batch* h, int iters, int mode, double* ms);
 int cudaDecon_batch_get(cudaDecon_batch* h, int i, float* X);
 void cudaDecon_batch_destroy(cudaDecon_batch* h);
 
+/* Device-resident pipeline around the engine (no host round trip):
+ * set_raw_u16 stages a uint16 stack with camera-offset subtraction
+ * (max(v - offset, 0), the background kernel src/RLgpuImpl.cu:124-132);
+ * deskew shears the current X (deskew_kernel, src/geometryTransform.cu:4-48)
+ * with the factor of RL_interface_init (width: outputWidth, or
+ * findOptimalDimension(nx + floor(nz dz |cos| / dr)) when 0); affine
+ * resamples the last result (the deskewed volume, else X) with a row-major
+ * 4x4 matrix (affine_GPU, src/geometryTransform.cu:183-290).  post_* read
+ * the last deskew / affine result. */
+int cudaDecon_rl_set_raw_u16(cudaDecon_rl* h, const unsigned short* raw, float offset);
+int cudaDecon_rl_deskew(cudaDecon_rl* h, float dz, float dr, float deskewAngle, int outputWidth, int extraShift,
+                        float padVal, int* nx_out);
+int cudaDecon_rl_affine(cudaDecon_rl* h, const float* affMat);
+int cudaDecon_rl_post_dims(cudaDecon_rl* h, int* nx, int* ny, int* nz);
+int cudaDecon_rl_post_get(cudaDecon_rl* h, float* out);
+/* FP64 sum, min, max of the last deskew / affine result */
+int cudaDecon_rl_post_stats(cudaDecon_rl* h, double* out3);
+
 /* Reference copy of the engine's current X (snapshot) and the relative L2
  * distance ||V - ref|| / ||ref|| (FP64 sums) of V = the current X
  * (which = 0) or the Y_0 seed buffer (which = 1). */

@@ -196,3 +196,33 @@ def test_rl_interface_cfg1_full_size_against_fp64_digest(cdn, tmp_path):
     err = float(np.abs(got.reshape(-1)[d["idx"]] - d["val"]).max() / d["xmax"])
     print(f"cfg1 max |gpu - fp64| / max = {err:.2e}", flush=True)
     assert err <= TOL
+
+
+def test_device_pipeline_matches_host_entry_points(cdn, orc):
+    """set_raw_u16 / deskew / affine on the engine's buffers give bitwise the
+    results of the float path and of Deskew_interface / Affine_interface."""
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.pipeline import small_affine
+    from cudadecon_amd.synth import make_stack
+    raw, psf, c = make_stack("cfg0")
+    tab = make_otf(psf)
+    g = orc.init_geometry(128, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, (2 * tab.shape[1], tab.shape[0], 1))
+    eng = cdn.RLEngine(raw.shape, tab, g["kscale"])
+    eng.set_raw(np.maximum(raw.astype(np.float32) - 100.0, 0.0))
+    eng.iterate(4)
+    X = eng.get()
+    eng.set_raw_u16(raw, 100.0)
+    eng.iterate(4)
+    assert np.array_equal(eng.get(), X)
+    w = eng.deskew(0.4, 0.104, 31.5, width=300)
+    assert w == 300
+    d = eng.post_get()
+    ref = cdn.deskew(X, 31.5, 0.4, 0.104, width=300)
+    assert d.shape == ref.shape and np.array_equal(d, ref)
+    M, _ = small_affine(d.shape, seed=4)
+    eng.affine(M)
+    a = eng.post_get()
+    assert np.array_equal(a, cdn.affine(d, M))
+    s = eng.post_stats()
+    assert abs(s[0] - float(a.astype(np.float64).sum())) <= 1e-9 * abs(s[0]) and s[1] == a.min() and s[2] == a.max()
+    eng.close()

@@ -56,3 +56,33 @@ def test_cfg4_size_shift_equivariance_and_flux(cdn, orc):
     b, lb = out["B"]
     assert np.abs(a - b).max() <= 1e-4 * np.abs(a).max()
     assert np.abs(la - lb).max() < 1e-4
+
+
+def test_cfg4_full_pipeline(cdn):
+    """The 512 x 2048 x 2048 pipeline resident in HBM: camera offset
+    subtraction of a uint16 stack, learned warm start, 10 RL-BA iterations,
+    deskew (31.5 deg, dz 0.4, dx 0.104 -> 3675 wide) and an affine with a
+    small rotation and scale (seed 4).  Checks stage results (finite,
+    non-negative, bounded by the stack) and prints the stage times."""
+    from cudadecon_amd import pipeline, warmstart
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import make_stack
+    nz, ny, nx = SHAPE
+    print("cfg4 pipeline", SHAPE, flush=True)
+    blk, psf, _ = make_stack("cfg1", shape=(64, 128, 128), seed=4)
+    raw = np.full(SHAPE, 200, np.uint16)
+    for off in ((100, 300, 500), (300, 1500, 900), (420, 100, 1800)):
+        raw[off[0]:off[0] + 64, off[1]:off[1] + 128, off[2]:off[2] + 128] = blk
+    tab = make_otf(psf)
+    p, _ = warmstart.load_params()
+    net = cdn.WarmNet(params=p)
+    eng = cdn.RLEngine(SHAPE, tab, warmstart.kscale_for(SHAPE, tab, dz=0.4))
+    t = pipeline.run(eng, raw, net)
+    print(t, flush=True)
+    assert t["deskewed_nx"] == 3675 and t["output_zyx"] == [512, 2048, 3675]
+    st = eng.stats()
+    assert st["x"][1] >= 0 and np.isfinite(st["x"][2]) and st["x"][2] <= 100 * st["raw"][2]
+    s = eng.post_stats()
+    assert np.isfinite(s[0]) and s[1] >= 0 and s[2] <= st["x"][2] * (1 + 1e-6)
+    eng.close()
+    net.close()
