@@ -157,6 +157,126 @@ __global__ __launch_bounds__(THREADS) void k_conv(const bf16_t* __restrict__ in,
   }
 }
 
+// ---------------------------------------------------------------------------
+// 3x3x3 convolution with 2.5-D blocking.  A workgroup of 4 waves owns 4
+// output rows (wave w: row y0 + w) of TX voxels and walks a chunk of
+// z-planes.  The input planes z-1, z, z+1 (rows y0-1..y0+4, columns
+// x0-1..x0+TX, all CIN channels, zero outside the volume) sit in an LDS ring
+// of 3 planes, so every tap reads its B fragment from LDS without branches;
+// plane z+2 is loaded into registers while plane z is computed and stored
+// into the slot of plane z-1 after it.  Weights (A fragments) are read from
+// the packed array (L1/L2 resident).  Same arithmetic as k_conv.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT, int TX, bool RELU, bool BIAS>
+__global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, const bf16_t* __restrict__ wf,
+                                                const float* __restrict__ bias, bf16_t* __restrict__ out, Vol v,
+                                                int zchunk) {
+  constexpr int KS = CIN == 16 ? 16 : 32, NS = CIN / KS, NBO = COUT / 16, FR = KS / 4;
+  constexpr int TY = 4, MB = TX / 16, RW = TX + 2, ROWS = TY + 2;
+  constexpr int ROWCH = RW * CIN / 8;          // 16-byte chunks per tile row
+  constexpr int PLANE = ROWS * RW * CIN;       // bf16 per plane
+  constexpr int CHUNKS = ROWS * ROWCH;         // 16-byte chunks per plane
+  constexpr int PER_T = (CHUNKS + 255) / 256;  // chunks per thread
+  typedef typename Mfma<KS>::frag frag;
+  __shared__ __attribute__((aligned(16))) bf16_t ring[3 * PLANE];
+  const int xt_n = (v.nx + TX - 1) / TX, yt_n = (v.ny + TY - 1) / TY, zc_n = (v.nz + zchunk - 1) / zchunk;
+  int blk = blockIdx.x;
+  const int xt = blk % xt_n;
+  blk /= xt_n;
+  const int yt = blk % yt_n;
+  blk /= yt_n;
+  const int zc = blk % zc_n;
+  const int b = blk / zc_n;
+  const int x0 = xt * TX, y0 = yt * TY, z0 = zc * zchunk, z1 = min(z0 + zchunk, v.nz);
+  const bf16_t* smp = in + (long)b * v.nz * v.ny * v.nx * CIN;
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15, w = threadIdx.x >> 6;
+
+  u32x4 reg[PER_T];
+  auto load_plane = [&](int zz) {
+#pragma unroll
+    for (int k = 0; k < PER_T; ++k) {
+      const int c = threadIdx.x + 256 * k;
+      const int rr = c / ROWCH, rem = c - rr * ROWCH;
+      const int xx = rem / (CIN / 8), cg = rem - xx * (CIN / 8);
+      const int y = y0 - 1 + rr, x = x0 - 1 + xx;
+      const bool ok = c < CHUNKS && zz >= 0 && zz < v.nz && y >= 0 && y < v.ny && x >= 0 && x < v.nx;
+      const long off = ok ? ((((long)zz * v.ny + y) * v.nx + x) * CIN + cg * 8) : 0L;
+      const u32x4 d = *reinterpret_cast<const u32x4*>(smp + off);
+      reg[k] = ok ? d : u32x4(0u);
+    }
+  };
+  auto store_plane = [&](int zz) {
+    bf16_t* dst = ring + ((zz + 3) % 3) * PLANE;
+#pragma unroll
+    for (int k = 0; k < PER_T; ++k) {
+      const int c = threadIdx.x + 256 * k;
+      if (c < CHUNKS) *reinterpret_cast<u32x4*>(dst + c * 8) = reg[k];
+    }
+  };
+  float bl[NBO][4];
+#pragma unroll
+  for (int j = 0; j < NBO; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bl[j][i] = BIAS ? bias[16 * j + 4 * q + i] : 0.f;
+
+  load_plane(z0 - 1);
+  store_plane(z0 - 1);
+  load_plane(z0);
+  store_plane(z0);
+  load_plane(z0 + 1);
+  store_plane(z0 + 1);
+  const int y = y0 + w;
+  for (int z = z0; z < z1; ++z) {
+    if (z + 2 <= z1) load_plane(z + 2);  // in flight while plane z is computed
+    __syncthreads();
+    f32x4 acc[NBO][MB];
+#pragma unroll
+    for (int j = 0; j < NBO; ++j)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) acc[j][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#ifndef CDN_CONV25_DZ_UNROLL
+#define CDN_CONV25_DZ_UNROLL 3
+#endif
+#pragma unroll CDN_CONV25_DZ_UNROLL
+    for (int dz = -1; dz <= 1; ++dz) {
+      const bf16_t* base = ring + ((z + dz + 3) % 3) * PLANE;
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+#pragma unroll
+          for (int s = 0; s < NS; ++s) {
+            frag bfr[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+              bfr[mb] = *reinterpret_cast<const frag*>(base + ((w + 1 + dy) * RW + 16 * mb + r + 1 + dx) * CIN +
+                                                       s * KS + q * FR);
+#pragma unroll
+            for (int j = 0; j < NBO; ++j) {
+              const frag a = *reinterpret_cast<const frag*>(wf + (((t * NS + s) * NBO + j) * 64 + lane) * FR);
+#pragma unroll
+              for (int mb = 0; mb < MB; ++mb) acc[j][mb] = Mfma<KS>::run(a, bfr[mb], acc[j][mb]);
+            }
+          }
+        }
+      }
+    }
+    if (y < v.ny) {
+      bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + y) * v.nx) * COUT;
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const int x = x0 + 16 * mb + r;
+        if (x < v.nx)
+#pragma unroll
+          for (int j = 0; j < NBO; ++j) store4<RELU>(dst + (long)x * COUT + 16 * j + 4 * q, acc[j][mb], bl[j]);
+      }
+    }
+    __syncthreads();                     // every wave is done with plane z-1
+    if (z + 2 <= z1) store_plane(z + 2);  // into the slot of plane z-1
+  }
+}
+
 // First layer: 1 -> 16 channels from the FP32 stack scaled by 1/s per sample,
 // K running over the 27 taps (padded to 32).  A workgroup of 4 waves covers
 // 2 (z) x 8 (y) x 64 (x) voxels: wave w takes x0 + 16w .. x0 + 16w + 15 and
@@ -822,9 +942,31 @@ int grid_for(long n, int threads, int cap = 4096) {
   return (int)std::max(1L, std::min<long>(g, cap));
 }
 
+#ifndef CDN_CONV25
+#define CDN_CONV25 1
+#endif
+template <int CIN, int COUT, bool RELU, bool BIAS>
+void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
+  constexpr int TX = CIN == 64 ? 32 : 64;
+  // z-chunks of 16 planes (2 extra planes of loads per chunk), fewer when the
+  // grid is already large
+  const long tiles = (long)v.nb * ((v.ny + 3) / 4) * ((v.nx + TX - 1) / TX);
+  int zchunk = v.nz < 16 ? v.nz : 16;
+  if (tiles >= 4096) zchunk = v.nz < 32 ? v.nz : 32;
+  const long grid = tiles * ((v.nz + zchunk - 1) / zchunk);
+  hipLaunchKernelGGL((k_conv25<CIN, COUT, TX, RELU, BIAS>), dim3((unsigned)grid), dim3(256), 0, st, in, wf, bias,
+                     out, v, zchunk);
+}
+
 template <int CIN, int COUT, bool RELU, bool BIAS>
 void conv_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
-  constexpr int MB = COUT == 64 ? 2 : 4;  // 4 keeps COUT = 64 within 128 VGPRs only with spills
+  // 64-channel layers keep k_conv (the blocked kernel would spill registers)
+  if (CDN_CONV25 && CIN <= 32 && COUT <= 32) return conv25_launch<CIN, COUT, RELU, BIAS>(st, in, wf, bias, out, v);
+#ifndef CDN_CONV_MB
+#define CDN_CONV_MB 4
+#endif
+  // 16-voxel blocks per wave; 2 for COUT = 64 (more would spill at 1024 threads)
+  constexpr int MB = COUT == 64 ? 2 : CDN_CONV_MB;
   constexpr int THREADS = 27 * CIN * COUT * 2 > 64 * 1024 ? 1024 : 256;
   const long items = (long)v.nb * v.nz * v.ny * ((v.nx + 16 * MB - 1) / (16 * MB));
   const int waves = THREADS / 64;
