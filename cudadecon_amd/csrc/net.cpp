@@ -80,10 +80,10 @@ void WarmNet::pack(hipStream_t st) {
   const float* p = P.as<float>();
   for (int l = 0; l < NET_CONVS; ++l) {
     const int ci = NET_CIN[l], co = NET_COUT[l];
-    wf[l].ensure(sizeof(bf16_t) * (size_t)(27L * ci * co + (ci == 1 ? 5L * co : 0L)));
+    wf[l].ensure(sizeof(bf16_t) * (size_t)net_packed_weights(ci, co));
     net_pack(st, p + w_off[l], ci, co, 0, wf[l].as<bf16_t>());
     if (ci > 1) {
-      wb[l].ensure(sizeof(bf16_t) * (size_t)(27L * ci * co));
+      wb[l].ensure(sizeof(bf16_t) * (size_t)net_packed_weights(co, ci));
       net_pack(st, p + w_off[l], ci, co, 1, wb[l].as<bf16_t>());
     }
   }

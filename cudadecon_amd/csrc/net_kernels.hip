@@ -61,6 +61,12 @@ __device__ __forceinline__ void tap_offsets(int t, int& dz, int& dy, int& dx) {
   dx = t % 3 - 1;
 }
 
+// packed BF16 operand elements of a layer with ci input / co output channels:
+// 1 input channel: 32 taps (27 + zeros); 16: 14 tap pairs; else 27 taps
+__host__ __device__ constexpr long packed_weights(int ci, int co) {
+  return ci == 1 ? 32L * co : (ci == 16 ? 28L * 16 * co : 27L * ci * co);
+}
+
 // Store 4 consecutive channels (acc + bias, optional ReLU) of one voxel.
 template <bool RELU>
 __device__ __forceinline__ void store4(bf16_t* p, f32x4 a, const float* b) {
@@ -234,6 +240,31 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
     for (int j = 0; j < NBO; ++j)
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) acc[j][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (CIN == 16) {
+      // 14 tap pairs on the full-rate K = 32 instruction: lanes q < 2 read
+      // tap 2p, lanes q >= 2 tap 2p + 1 (tap 27 has zero weights)
+      int slot[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) slot[d] = ((z + d - 1 + 3) % 3) * PLANE;
+      const bool hi = q >= 2;
+      const int cofs = 8 * (q & 1);
+#pragma unroll
+      for (int pr = 0; pr < 14; ++pr) {
+        const int t0 = 2 * pr, t1 = 2 * pr + 1 < 27 ? 2 * pr + 1 : 26;
+        const int o0 = slot[t0 / 9] + ((w + (t0 / 3) % 3) * RW + t0 % 3) * 16;
+        const int o1 = slot[t1 / 9] + ((w + (t1 / 3) % 3) * RW + t1 % 3) * 16;
+        const bf16_t* base = ring + (hi ? o1 : o0) + cofs;
+        u32x4 bfr[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) bfr[mb] = *reinterpret_cast<const u32x4*>(base + (16 * mb + r) * 16);
+#pragma unroll
+        for (int j = 0; j < NBO; ++j) {
+          const u32x4 a = *reinterpret_cast<const u32x4*>(wf + ((pr * NBO + j) * 64 + lane) * 8);
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) acc[j][mb] = Mfma<32>::run(a, bfr[mb], acc[j][mb]);
+        }
+      }
+    } else {
 #ifndef CDN_CONV25_DZ_UNROLL
 #define CDN_CONV25_DZ_UNROLL 3
 #endif
@@ -261,6 +292,7 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
           }
         }
       }
+    }
     }
     if (y < v.ny) {
       bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + y) * v.nx) * COUT;
@@ -899,13 +931,21 @@ __global__ void k_head_reduce(const float* __restrict__ part, int blocks, double
 // channel K runs over the taps ([lane][8], tap = 8q + i, zero for tap >= 27).
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int flip, bf16_t* __restrict__ out) {
   const int ci_n = flip ? cout : cin, co_n = flip ? cin : cout;  // the packed layer's channels
-  const long n = 27L * ci_n * co_n + (ci_n == 1 ? 5L * co_n : 0L);
+  const long n = packed_weights(ci_n, co_n);
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     float val;
     if (ci_n == 1) {
       const int i = (int)(e % 8), lane = (int)((e / 8) % 64), j = (int)(e / 512);
       const int tap = 8 * (lane >> 4) + i, co = 16 * j + (lane & 15);
       val = tap < 27 ? W[tap * co_n + co] : 0.f;
+    } else if (ci_n == 16) {
+      // tap pairs: K = 32 = taps 2p (lanes q < 2) and 2p + 1 (q >= 2), 16
+      // channels each; [pair][co-block][lane][8], tap 27 is zero
+      const int i = (int)(e % 8), lane = (int)((e / 8) % 64), NBO = co_n / 16;
+      const int j = (int)((e / 512) % NBO), pr = (int)(e / (512L * NBO));
+      const int q = lane >> 4, t = 2 * pr + (q >> 1), ci = 8 * (q & 1) + i, co = 16 * j + (lane & 15);
+      val = t >= 27 ? 0.f
+                    : (flip ? W[((long)(26 - t) * cin + co) * cout + ci] : W[((long)t * cin + ci) * cout + co]);
     } else {
       const int KS = ci_n == 16 ? 16 : 32, F = KS / 4, NS = ci_n / KS, NBO = co_n / 16;
       long k = e;
@@ -958,13 +998,11 @@ void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const flo
                      out, v, zchunk);
 }
 
-template <int CIN, int COUT, bool RELU, bool BIAS>
-void conv_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
-  // 64-channel layers keep k_conv (the blocked kernel would spill registers)
-  if (CDN_CONV25 && CIN <= 32 && COUT <= 32) return conv25_launch<CIN, COUT, RELU, BIAS>(st, in, wf, bias, out, v);
 #ifndef CDN_CONV_MB
 #define CDN_CONV_MB 4
 #endif
+template <int CIN, int COUT, bool RELU, bool BIAS>
+void conv_plain_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
   // 16-voxel blocks per wave; 2 for COUT = 64 (more would spill at 1024 threads)
   constexpr int MB = COUT == 64 ? 2 : CDN_CONV_MB;
   constexpr int THREADS = 27 * CIN * COUT * 2 > 64 * 1024 ? 1024 : 256;
@@ -983,6 +1021,16 @@ void conv_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float
   const int grid = (int)std::max(1L, std::min<long>((items + waves - 1) / waves, resident));
   hipLaunchKernelGGL((k_conv<CIN, COUT, MB, THREADS, RELU, BIAS>), dim3(grid), dim3(THREADS), 0, st, in, wf, bias,
                      out, v);
+}
+
+// 64-channel layers use k_conv (the blocked kernel would spill registers);
+// 16 input channels need the blocked kernel (tap-pair operand layout)
+template <int CIN, int COUT, bool RELU, bool BIAS>
+void conv_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
+  if constexpr (CIN == 16 || (CDN_CONV25 && CIN <= 32 && COUT <= 32))
+    conv25_launch<CIN, COUT, RELU, BIAS>(st, in, wf, bias, out, v);
+  else
+    conv_plain_launch<CIN, COUT, RELU, BIAS>(st, in, wf, bias, out, v);
 }
 
 template <int CIN, int COUT>
@@ -1126,9 +1174,10 @@ void net_inv_scale(hipStream_t st, const float* raw, Vol v, double* part, float*
   hipLaunchKernelGGL(k_inv_scale, dim3(v.nb), dim3(SUM_THREADS), 0, st, part, blocks, per, inv_s);
 }
 
+long net_packed_weights(int cin, int cout) { return packed_weights(cin, cout); }
+
 void net_pack(hipStream_t st, const float* W, int cin, int cout, int flip, bf16_t* out) {
-  const int ci_n = flip ? cout : cin;
-  const long n = 27L * cin * cout + (ci_n == 1 ? 5L * cout : 0L);
+  const long n = packed_weights(flip ? cout : cin, flip ? cin : cout);
   hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256, 1024)), dim3(256), 0, st, W, cin, cout, flip, out);
 }
 

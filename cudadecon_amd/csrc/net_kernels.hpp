@@ -38,6 +38,8 @@ void net_wgrad_in1(hipStream_t st, const float* raw, const float* inv_s, const b
 void net_reduce_rows(hipStream_t st, const float* part, int rows, long cols, float* out);
 int net_sum_blocks(int nb);
 void net_inv_scale(hipStream_t st, const float* raw, Vol v, double* part, float* inv_s);
+// packed operand size (bf16 elements) of a layer as the convolution reads it
+long net_packed_weights(int cin, int cout);
 void net_pack(hipStream_t st, const float* W, int cin, int cout, int flip, bf16_t* out);
 void net_adam(hipStream_t st, float* p, const float* g, float* m, float* v, long n, float lr_t, float b1, float b2,
               float eps, float c2);
