@@ -105,7 +105,7 @@ void WarmNet::geometry(int nb, int nz, int ny, int nx, bool training) {
   p2.ensure(h * n3 * 32);
   e3.ensure(h * n3 * 64);
   d3.ensure(h * n3 * 32);
-  a2.ensure(h * n2 * 32);
+  if (training) a2.ensure(h * n2 * 32);
   d2.ensure(h * n2 * 16);
   if (training) {
     dout.ensure(sizeof(float) * n1);
@@ -130,6 +130,7 @@ void WarmNet::forward(hipStream_t st, const float* raw, bool keep_e1) {
   net_inv_scale(st, raw, L1, sum_part.as<double>(), inv_s.as<float>());
   if (keep_e1) {
     e1.ensure(sizeof(bf16_t) * 16 * (size_t)vox(L1));
+    a2.ensure(sizeof(bf16_t) * 32 * (size_t)vox(L2));
     net_first(st, 0, raw, inv_s.as<float>(), wf[0].as<bf16_t>(), p + b_off[0], e1.as<bf16_t>(), nullptr, nullptr,
               nullptr, L1);
     net_pool(st, 16, e1.as<bf16_t>(), p1.as<bf16_t>(), L1);
@@ -142,8 +143,12 @@ void WarmNet::forward(hipStream_t st, const float* raw, bool keep_e1) {
   net_pool(st, 32, e2.as<bf16_t>(), p2.as<bf16_t>(), L2);
   net_conv(st, 32, 64, true, p2.as<bf16_t>(), wf[2].as<bf16_t>(), p + b_off[2], e3.as<bf16_t>(), L3);
   net_conv(st, 64, 32, true, e3.as<bf16_t>(), wf[3].as<bf16_t>(), p + b_off[3], d3.as<bf16_t>(), L3);
-  net_upadd(st, 32, d3.as<bf16_t>(), e2.as<bf16_t>(), a2.as<bf16_t>(), L2);
-  net_conv(st, 32, 16, true, a2.as<bf16_t>(), wf[4].as<bf16_t>(), p + b_off[4], d2.as<bf16_t>(), L2);
+  if (keep_e1) {  // training keeps a2 (the input of the decoder's weight gradient)
+    net_upadd(st, 32, d3.as<bf16_t>(), e2.as<bf16_t>(), a2.as<bf16_t>(), L2);
+    net_conv(st, 32, 16, true, a2.as<bf16_t>(), wf[4].as<bf16_t>(), p + b_off[4], d2.as<bf16_t>(), L2);
+  } else {
+    net_conv_upadd_32_16(st, d3.as<bf16_t>(), e2.as<bf16_t>(), wf[4].as<bf16_t>(), p + b_off[4], d2.as<bf16_t>(), L2);
+  }
 }
 
 void WarmNet::head(hipStream_t st, const float* raw, int mode, float* out) {
@@ -215,7 +220,8 @@ long WarmNet::fetch(int which, float* host, hipStream_t st) {
                    {&gA2, &L2, 32}, {&g4, &L3, 32}, {&gE3, &L3, 64}, {&g3, &L3, 64}, {&gP2, &L3, 32},
                    {&g2, &L2, 32}, {&gP1, &L2, 16}, {&g1, &L1, 16}};
   if (which < 0 || which >= (int)(sizeof(tab) / sizeof(tab[0]))) fail("cudaDecon_net_fetch: unknown tensor");
-  if (which == 0 && !e1_kept) fail("cudaDecon_net_fetch: e1 is not stored by the inference path");
+  if ((which == 0 || which == 6) && !e1_kept)
+    fail("cudaDecon_net_fetch: e1 and a2 are not stored by the inference path");
   const T& t = tab[which];
   const long n = vox(*t.v) * t.c;
   if (!t.b->p || t.b->bytes < (size_t)n * (which == 8 ? 4 : 2)) fail("cudaDecon_net_fetch: tensor not computed");

@@ -173,10 +173,13 @@ __global__ __launch_bounds__(THREADS) void k_conv(const bf16_t* __restrict__ in,
 // into the slot of plane z-1 after it.  Weights (A fragments) are read from
 // the packed array (L1/L2 resident).  Same arithmetic as k_conv.
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT, int TX, bool RELU, bool BIAS>
+// UPADD: the input is up(coarse) + in (nearest upsampling of a half-size
+// tensor of the same channels), formed while the planes are loaded, as
+// k_upadd stores it (bf16 of the FP32 sum).
+template <int CIN, int COUT, int TX, bool RELU, bool BIAS, bool UPADD>
 __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, const bf16_t* __restrict__ wf,
                                                 const float* __restrict__ bias, bf16_t* __restrict__ out, Vol v,
-                                                int zchunk) {
+                                                int zchunk, const bf16_t* __restrict__ coarse) {
   constexpr int KS = CIN == 16 ? 16 : 32, NS = CIN / KS, NBO = COUT / 16, FR = KS / 4;
   constexpr int TY = 4, MB = TX / 16, RW = TX + 2, ROWS = TY + 2;
   constexpr int ROWCH = RW * CIN / 8;          // 16-byte chunks per tile row
@@ -207,7 +210,14 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
       const int y = y0 - 1 + rr, x = x0 - 1 + xx;
       const bool ok = c < CHUNKS && zz >= 0 && zz < v.nz && y >= 0 && y < v.ny && x >= 0 && x < v.nx;
       const long off = ok ? ((((long)zz * v.ny + y) * v.nx + x) * CIN + cg * 8) : 0L;
-      const u32x4 d = *reinterpret_cast<const u32x4*>(smp + off);
+      u32x4 d = *reinterpret_cast<const u32x4*>(smp + off);
+      if constexpr (UPADD) {
+        const int hz = v.nz / 2, hy = v.ny / 2, hx = v.nx / 2;
+        const long coff = ok ? (((((long)b * hz + zz / 2) * hy + y / 2) * hx + x / 2) * CIN + cg * 8) : 0L;
+        const u32x4 c = *reinterpret_cast<const u32x4*>(coarse + coff);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) d[e] = pack2(lo(c[e]) + lo(d[e]), hi(c[e]) + hi(d[e]));
+      }
       reg[k] = ok ? d : u32x4(0u);
     }
   };
@@ -985,8 +995,9 @@ int grid_for(long n, int threads, int cap = 4096) {
 #ifndef CDN_CONV25
 #define CDN_CONV25 1
 #endif
-template <int CIN, int COUT, bool RELU, bool BIAS>
-void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
+template <int CIN, int COUT, bool RELU, bool BIAS, bool UPADD = false>
+void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v,
+                   const bf16_t* coarse = nullptr) {
   constexpr int TX = CIN == 64 ? 32 : 64;
   // z-chunks of 16 planes (2 extra planes of loads per chunk), fewer when the
   // grid is already large
@@ -994,8 +1005,8 @@ void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const flo
   int zchunk = v.nz < 16 ? v.nz : 16;
   if (tiles >= 4096) zchunk = v.nz < 32 ? v.nz : 32;
   const long grid = tiles * ((v.nz + zchunk - 1) / zchunk);
-  hipLaunchKernelGGL((k_conv25<CIN, COUT, TX, RELU, BIAS>), dim3((unsigned)grid), dim3(256), 0, st, in, wf, bias,
-                     out, v, zchunk);
+  hipLaunchKernelGGL((k_conv25<CIN, COUT, TX, RELU, BIAS, UPADD>), dim3((unsigned)grid), dim3(256), 0, st, in, wf,
+                     bias, out, v, zchunk, coarse);
 }
 
 #ifndef CDN_CONV_MB
@@ -1062,6 +1073,11 @@ void net_conv(hipStream_t st, int cin, int cout, bool relu, const bf16_t* in, co
     }
   }
   fail("net_conv: unsupported channel pair");
+}
+
+void net_conv_upadd_32_16(hipStream_t st, const bf16_t* coarse, const bf16_t* skip, const bf16_t* wf,
+                          const float* bias, bf16_t* out, Vol v) {
+  conv25_launch<32, 16, true, true, true>(st, skip, wf, bias, out, v, coarse);
 }
 
 void net_first(hipStream_t st, int mode, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,

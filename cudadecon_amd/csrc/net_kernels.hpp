@@ -20,6 +20,10 @@ void net_conv(hipStream_t st, int cin, int cout, bool relu, const bf16_t* in, co
 // 2 / 3 also apply up(d2) and the head hp and write y / the RL seed (out32)
 void net_first(hipStream_t st, int mode, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,
                bf16_t* out16, const bf16_t* d2, const float* hp, float* out32, Vol v);
+// 32 -> 16 convolution (bias + ReLU) of up(coarse) + skip, the sum formed
+// on load (inference: the decoder's a2 is never stored)
+void net_conv_upadd_32_16(hipStream_t st, const bf16_t* coarse, const bf16_t* skip, const bf16_t* wf,
+                          const float* bias, bf16_t* out, Vol v);
 void net_pool(hipStream_t st, int c, const bf16_t* in, bf16_t* out, Vol fine);
 void net_upadd(hipStream_t st, int c, const bf16_t* coarse, const bf16_t* skip, bf16_t* out, Vol fine);
 int net_head_blocks(Vol v);
