@@ -8,7 +8,7 @@ buffers.  The default workload is the 256 x 1024 x 1024 FP32 stack (plain
 RL-BA, no deskew) with a synthetic bead image made like the 64 x 128 x 128
 reference case.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload t256|cfg0|cfg1]
+    python bench.py [--steps K] [--warmup W] [--workload beads_256x1024x1024|beads_64x128x128|lightsheet_128x512x512]
 """
 from __future__ import annotations
 
@@ -44,8 +44,8 @@ def pass_bytes(R: float, P: float) -> dict:
 
 def pmc_traffic(kernel: str):
     """HBM bytes per launch of `kernel` measured by rocprofv3 PMC runs and
-    committed in profiles/rl_iteration_t256_pmc.md (JSON block), else None."""
-    path = os.path.join(ROOT, "profiles", "rl_iteration_t256_pmc.md")
+    committed in profiles/rl_iteration_256x1024x1024_pmc.md (JSON block), else None."""
+    path = os.path.join(ROOT, "profiles", "rl_iteration_256x1024x1024_pmc.md")
     try:
         text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
@@ -54,7 +54,7 @@ def pmc_traffic(kernel: str):
         return None
 
 
-def batch_cfg2(cdn, tab, iters=10, nstreams=8):
+def batch_48_stacks(cdn, tab, iters=10, nstreams=8):
     """48 stacks of uneven shape (z in {64..256}, y, x in {256..1024}, drawn
     with seed 2): iterations of all stacks one at a time vs one captured
     hipGraph over nstreams streams.  Stacks are staged on the device (a bead
@@ -143,7 +143,7 @@ def warm_start_eval(cdn, iters=50):
     return res
 
 
-def cfg4_pipeline(cdn):
+def pipeline_512x2048x2048(cdn):
     """512x2048x2048 uint16 stack (bead blocks made as the light-sheet
     workload on a flat 200-count background) through camera offset
     subtraction, warm start, 10 RL-BA iterations, deskew and a small affine,
@@ -152,7 +152,7 @@ def cfg4_pipeline(cdn):
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.synth import make_stack
     shape = (512, 2048, 2048)
-    blk, psf, _ = make_stack("cfg1", shape=(64, 128, 128), seed=4)
+    blk, psf, _ = make_stack("lightsheet_128x512x512", shape=(64, 128, 128), seed=4)
     raw = np.full(shape, 200, np.uint16)
     for off in ((100, 300, 500), (300, 1500, 900), (420, 100, 1800)):
         raw[off[0]:off[0] + 64, off[1]:off[1] + 128, off[2]:off[2] + 128] = blk
@@ -174,7 +174,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="t256", choices=["t256", "cfg0", "cfg1"])
+    ap.add_argument("--workload", default="beads_256x1024x1024", choices=["beads_256x1024x1024", "beads_64x128x128", "lightsheet_128x512x512"])
     ap.add_argument("--pass-iters", type=int, default=3, help="iterations with per-pass event timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
@@ -183,7 +183,7 @@ def main():
     ap.add_argument("--e2e-iters", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-stack batch measurement")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
-    ap.add_argument("--no-cfg4", action="store_true", help="skip the 512x2048x2048 pipeline")
+    ap.add_argument("--no-pipeline", action="store_true", help="skip the 512x2048x2048 pipeline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1 "
@@ -202,7 +202,7 @@ def main():
     tab = make_otf(psf)
     nz, ny, nx = raw.shape
     if c["angle"] != 0.0:
-        raise SystemExit("bench.py times the plain (no-deskew) iteration; pick t256 or cfg0")
+        raise SystemExit("bench.py times the plain (no-deskew) iteration; pick beads_256x1024x1024 or beads_64x128x128")
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "otf.tif")
         write_otf_tiff(p, tab)
@@ -245,9 +245,9 @@ def main():
         e2e = {"entry": "RL_interface", "iterations": args.e2e_iters, "seconds": round(e2e_s, 4),
                "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
-    batch = None if args.no_batch else batch_cfg2(cdn, tab)
+    batch = None if args.no_batch else batch_48_stacks(cdn, tab)
     warm = None if args.no_warm else warm_start_eval(cdn)
-    big = None if args.no_cfg4 else cfg4_pipeline(cdn)
+    big = None if args.no_pipeline else pipeline_512x2048x2048(cdn)
 
     Rb = 4.0 * nvox
     Pb = 8.0 * nspec
@@ -309,9 +309,9 @@ def main():
         },
         "passes": passes,
         "rl_interface_e2e": e2e,
-        "batch_cfg2": batch,
+        "batch_48_stacks": batch,
         "warm_start": warm,
-        "cfg4_pipeline": big,
+        "pipeline_512x2048x2048": big,
         "setup_s": round(gen_s, 1),
     }
     if rank == 0 and not args.no_cpu_baseline:

@@ -4,11 +4,11 @@ Gaussian PSF blur, Poisson noise, camera offset, uint16).
 Every stack is a function of its seed (numpy PCG64), so the same input can
 be regenerated on any host.  Sizes and parameters of the named workloads:
 
-* ``cfg0``  64 x 128 x 128, 300 beads, PSF sigma_xy 1.5 / sigma_z 3.0 px,
+* ``beads_64x128x128``  64 x 128 x 128, 300 beads, PSF sigma_xy 1.5 / sigma_z 3.0 px,
   10 iterations, no deskew;
-* ``cfg1``  128 x 512 x 512, 5000 beads, PSF 1.3 / 4.0 px, deskew 31.5 deg,
+* ``lightsheet_128x512x512``  128 x 512 x 512, 5000 beads, PSF 1.3 / 4.0 px, deskew 31.5 deg,
   dz 0.3 um, dx 0.104 um, then rotation, 15 iterations;
-* ``t256``  256 x 1024 x 1024 plain RL-BA (the throughput workload).
+* ``beads_256x1024x1024``  256 x 1024 x 1024 plain RL-BA (the throughput workload).
 """
 from __future__ import annotations
 
@@ -112,11 +112,11 @@ def image(blurred: np.ndarray, seed_seq: np.random.SeedSequence, background=100.
 
 
 CONFIGS = {
-    "cfg0": dict(shape=(64, 128, 128), beads=300, sxy=1.5, sz=3.0, seed=0, iters=10, dr=0.104, dz=0.2,
+    "beads_64x128x128": dict(shape=(64, 128, 128), beads=300, sxy=1.5, sz=3.0, seed=0, iters=10, dr=0.104, dz=0.2,
                  angle=0.0, rot=0.0),
-    "cfg1": dict(shape=(128, 512, 512), beads=5000, sxy=1.3, sz=4.0, seed=1, iters=15, dr=0.104, dz=0.3,
+    "lightsheet_128x512x512": dict(shape=(128, 512, 512), beads=5000, sxy=1.3, sz=4.0, seed=1, iters=15, dr=0.104, dz=0.3,
                  angle=31.5, rot=31.5),
-    "t256": dict(shape=(256, 1024, 1024), beads=300 * 256, sxy=1.5, sz=3.0, seed=7, iters=10, dr=0.104, dz=0.2,
+    "beads_256x1024x1024": dict(shape=(256, 1024, 1024), beads=300 * 256, sxy=1.5, sz=3.0, seed=7, iters=10, dr=0.104, dz=0.2,
                  angle=0.0, rot=0.0),
 }
 

@@ -42,7 +42,7 @@ EVAL_STACK = (256, 1024, 1024)
 TRAIN_SEED0 = 100000
 VALID_SEED = 200000
 EVAL_SEED = 3
-WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_cfg1.npz")
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_lightsheet.npz")
 
 
 def log(*a):
@@ -64,7 +64,7 @@ def kscale_for(shape, tab, dr=0.104, dz=0.3):
 def light_sheet_stack(shape, seed):
     """(background-subtracted float32 stack, PSF) made as the 128x512x512
     workload at another size and seed."""
-    raw, psf, _ = make_stack("cfg1", shape=shape, seed=seed)
+    raw, psf, _ = make_stack("lightsheet_128x512x512", shape=shape, seed=seed)
     return np.maximum(raw.astype(np.float32) - np.float32(BACKGROUND), np.float32(0.0)), psf
 
 

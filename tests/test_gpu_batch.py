@@ -18,7 +18,7 @@ def test_batch_modes_match_single_engines(cdn, orc):
         g = orc.init_geometry(s[2], s[1], s[0], 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False,
                               (2 * tab.shape[1], tab.shape[0], 1))
         ks.append(g["kscale"])
-        raw, _, _ = make_stack("cfg0", shape=s)
+        raw, _, _ = make_stack("beads_64x128x128", shape=s)
         R = np.maximum(raw.astype(np.float32) - 100.0, 0.0)
         raws.append(R)
         eng = cdn.RLEngine(s, tab, g["kscale"])

@@ -161,7 +161,7 @@ def test_identity_net_warm_start_equals_plain_rl(cdn):
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.synth import make_stack
     import oracle as orc
-    raw_u16, psf, c = make_stack("cfg0")
+    raw_u16, psf, c = make_stack("beads_64x128x128")
     raw = np.maximum(raw_u16.astype(np.float32) - 100.0, 0.0)
     tab = make_otf(psf)
     g = orc.init_geometry(128, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, (2 * tab.shape[1], tab.shape[0], 1))

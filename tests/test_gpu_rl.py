@@ -33,11 +33,11 @@ def _relmax(a, b):
 
 @pytest.mark.parametrize("shape,iters", [((64, 128, 128), 10), ((64, 96, 810), 6), ((96, 64, 64), 12),
                                          ((70, 72, 150), 8)],
-                         ids=["cfg0", "x810", "z96", "runtime-compiled"])
+                         ids=["beads_64x128x128", "x810", "z96", "runtime-compiled"])
 def test_engine_matches_oracle(cdn, orc, shape, iters):
     print("engine", shape, iters, flush=True)
     from cudadecon_amd.synth import make_stack
-    raw, _, _ = make_stack("cfg0", shape=shape)
+    raw, _, _ = make_stack("beads_64x128x128", shape=shape)
     R = _preprocessed(raw)
     tab, g = _otf_and_geom(orc, shape)
     H = orc.otf_interp(tab, g["otf_dims"], g["fft_dims"], g["kscale"])
@@ -54,7 +54,7 @@ def test_engine_matches_oracle(cdn, orc, shape, iters):
 def test_engine_continuation_equals_single_run(cdn, orc):
     shape = (64, 128, 128)
     from cudadecon_amd.synth import make_stack
-    raw, _, _ = make_stack("cfg0", shape=shape)
+    raw, _, _ = make_stack("beads_64x128x128", shape=shape)
     R = _preprocessed(raw).astype(np.float32)
     tab, g = _otf_and_geom(orc, shape)
     eng = cdn.RLEngine(shape, tab, g["kscale"])
@@ -71,7 +71,7 @@ def test_engine_continuation_equals_single_run(cdn, orc):
 def test_seeded_start_matches_oracle(cdn, orc):
     shape = (64, 128, 128)
     from cudadecon_amd.synth import make_stack
-    raw, _, _ = make_stack("cfg0", shape=shape)
+    raw, _, _ = make_stack("beads_64x128x128", shape=shape)
     R = _preprocessed(raw)
     seed = np.full(shape, R.mean())
     tab, g = _otf_and_geom(orc, shape)
@@ -109,9 +109,9 @@ def _run_both(cdn, orc, tmp_path, raw, tab, dr, dz, angle, rot, iters, **kw):
     return got, ref
 
 
-def test_rl_interface_cfg0(cdn, orc, tmp_path):
+def test_rl_interface_beads_64x128x128(cdn, orc, tmp_path):
     from cudadecon_amd.synth import make_stack
-    raw, _, c = make_stack("cfg0")
+    raw, _, c = make_stack("beads_64x128x128")
     tab, _ = _otf_and_geom(orc, raw.shape)
     got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.2, 0.0, 0.0, c["iters"])
     assert got.shape == ref.shape
@@ -119,10 +119,10 @@ def test_rl_interface_cfg0(cdn, orc, tmp_path):
 
 
 def test_rl_interface_deskew_rotate_small(cdn, orc, tmp_path):
-    # cfg1 geometry on a reduced stack: 64 x 128 x 100 raw, deskew 31.5 deg,
+    # the light-sheet geometry on a reduced stack: 64 x 128 x 100 raw, deskew 31.5 deg,
     # dz 0.3 um -> deskewed width findOptimalDimension(100 + 157) = 256
     from cudadecon_amd.synth import make_stack
-    raw, _, _ = make_stack("cfg1", shape=(64, 128, 100))
+    raw, _, _ = make_stack("lightsheet_128x512x512", shape=(64, 128, 100))
     tab, _ = _otf_and_geom(orc, raw.shape, 1.3, 4.0, 0.104, 0.3)
     nz, ny, nx = raw.shape
     g = orc.init_geometry(nx, ny, nz, 0.104, 0.3, 0.104, 0.3, 31.5, 31.5, 0, False,
@@ -136,7 +136,7 @@ def test_rl_interface_deskew_rotate_small(cdn, orc, tmp_path):
 def test_rl_interface_deskew_to_runtime_compiled_width(cdn, orc, tmp_path):
     # 64 x 128 x 100 raw at -32.8 deg: deskewed width findOptimalDimension(255) = 252
     from cudadecon_amd.synth import make_stack
-    raw, _, _ = make_stack("cfg1", shape=(64, 128, 100))
+    raw, _, _ = make_stack("lightsheet_128x512x512", shape=(64, 128, 100))
     tab, _ = _otf_and_geom(orc, raw.shape, 1.3, 4.0, 0.104, 0.3)
     got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.3, -32.8, 0.0, 5)
     assert got.shape == ref.shape == (64, 128, 252)
@@ -145,7 +145,7 @@ def test_rl_interface_deskew_to_runtime_compiled_width(cdn, orc, tmp_path):
 
 def test_rl_interface_preprocessing_only_with_rotation(cdn, orc, tmp_path):
     from cudadecon_amd.synth import make_stack
-    raw, _, _ = make_stack("cfg0", shape=(64, 64, 64))
+    raw, _, _ = make_stack("beads_64x128x128", shape=(64, 64, 64))
     tab, _ = _otf_and_geom(orc, raw.shape)
     got, ref = _run_both(cdn, orc, tmp_path, raw, tab, 0.104, 0.2, 0.0, 20.0, 0, napodize=5)
     assert got.shape == ref.shape
@@ -177,14 +177,14 @@ def test_affine_interfaces(cdn, orc):
     assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
-def test_rl_interface_cfg1_full_size_against_fp64_digest(cdn, tmp_path):
-    """cfg1 at full size: 128 x 512 x 512 raw, deskew 31.5 deg -> 128 x 512 x 810,
+def test_rl_interface_lightsheet_full_size_against_fp64_digest(cdn, tmp_path):
+    """The light-sheet workload at full size: 128 x 512 x 512 raw, deskew 31.5 deg -> 128 x 512 x 810,
     15 RL-BA iterations, rotation -> 192 x 512 x 791, against the FP64 oracle's
-    result sampled at 40,000 seeded voxels (tests/golden/make_cfg1_digest.py)."""
+    result sampled at 40,000 seeded voxels (tests/golden/make_lightsheet_digest.py)."""
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.synth import make_stack
-    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1_digest.npz"))
-    raw, psf, c = make_stack("cfg1")
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "lightsheet_128x512x512_digest.npz"))
+    raw, psf, c = make_stack("lightsheet_128x512x512")
     tab = make_otf(psf)
     p = _write_otf(tmp_path, tab)
     cdn.rl_cleanup()
@@ -194,7 +194,7 @@ def test_rl_interface_cfg1_full_size_against_fp64_digest(cdn, tmp_path):
     cdn.rl_cleanup()
     assert got.shape == tuple(d["shape"]) == (192, 512, 791)
     err = float(np.abs(got.reshape(-1)[d["idx"]] - d["val"]).max() / d["xmax"])
-    print(f"cfg1 max |gpu - fp64| / max = {err:.2e}", flush=True)
+    print(f"light-sheet max |gpu - fp64| / max = {err:.2e}", flush=True)
     assert err <= TOL
 
 
@@ -204,7 +204,7 @@ def test_device_pipeline_matches_host_entry_points(cdn, orc):
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.pipeline import small_affine
     from cudadecon_amd.synth import make_stack
-    raw, psf, c = make_stack("cfg0")
+    raw, psf, c = make_stack("beads_64x128x128")
     tab = make_otf(psf)
     g = orc.init_geometry(128, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, (2 * tab.shape[1], tab.shape[0], 1))
     eng = cdn.RLEngine(raw.shape, tab, g["kscale"])

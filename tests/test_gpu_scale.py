@@ -1,4 +1,4 @@
-"""cfg4-size checks (512 x 2048 x 2048, 2^31 voxels, ~52 GB resident in HBM).
+"""512 x 2048 x 2048 checks (, 2^31 voxels, ~52 GB resident in HBM).
 
 The FP64 oracle cannot run at this size, so size-independent properties of
 RL-BA are checked instead:
@@ -20,14 +20,14 @@ pytestmark = pytest.mark.gpu
 SHAPE = (512, 2048, 2048)
 
 
-def test_cfg4_size_shift_equivariance_and_flux(cdn, orc):
+def test_512x2048x2048_shift_equivariance_and_flux(cdn, orc):
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.synth import gaussian_psf, make_stack
     nz, ny, nx = SHAPE
     tab = make_otf(gaussian_psf(1.5, 3.0))
     g = orc.init_geometry(nx, ny, nz, 0.104, 0.4, 0.104, 0.4, 0.0, 0.0, 0, False,
                           (2 * tab.shape[1], tab.shape[0], 1))
-    raw, _, _ = make_stack("cfg0")  # 64 x 128 x 128 bead block
+    raw, _, _ = make_stack("beads_64x128x128")  # 64 x 128 x 128 bead block
     block = np.maximum(raw.astype(np.float32) - 100.0, 0.0)
     bg = 1.0
     offA = (24, 24, 40)
@@ -58,7 +58,7 @@ def test_cfg4_size_shift_equivariance_and_flux(cdn, orc):
     assert np.abs(la - lb).max() < 1e-4
 
 
-def test_cfg4_full_pipeline(cdn):
+def test_512x2048x2048_full_pipeline(cdn):
     """The 512 x 2048 x 2048 pipeline resident in HBM: camera offset
     subtraction of a uint16 stack, learned warm start, 10 RL-BA iterations,
     deskew (31.5 deg, dz 0.4, dx 0.104 -> 3675 wide) and an affine with a
@@ -68,8 +68,8 @@ def test_cfg4_full_pipeline(cdn):
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.synth import make_stack
     nz, ny, nx = SHAPE
-    print("cfg4 pipeline", SHAPE, flush=True)
-    blk, psf, _ = make_stack("cfg1", shape=(64, 128, 128), seed=4)
+    print("pipeline", SHAPE, flush=True)
+    blk, psf, _ = make_stack("lightsheet_128x512x512", shape=(64, 128, 128), seed=4)
     raw = np.full(SHAPE, 200, np.uint16)
     for off in ((100, 300, 500), (300, 1500, 900), (420, 100, 1800)):
         raw[off[0]:off[0] + 64, off[1]:off[1] + 128, off[2]:off[2] + 128] = blk

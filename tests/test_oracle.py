@@ -17,7 +17,7 @@ here by a library routine, brute force, a closed form or an invariant:
 | apodize / zblend / bgsub   | closed forms on linear ramps                                      |
 | mean_above                 | brute force                                                      |
 | find_optimal_dimension     | exhaustive against a brute factoriser                            |
-| init_geometry              | the cfg1 sizes (128x512x512 raw -> 810 wide, 192 x 791 rotated)  |
+| init_geometry              | the light-sheet sizes (128x512x512 raw -> 810 wide, 192 x 791 rotated)  |
 """
 import math
 import os
@@ -195,7 +195,7 @@ def test_rl_unit_otf_is_fixed_point(orc):
 def test_rl_conserves_flux_with_normalised_otf(orc):
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.synth import gaussian_psf, make_stack
-    raw, _, _ = make_stack("cfg0", shape=(32, 64, 64))
+    raw, _, _ = make_stack("beads_64x128x128", shape=(32, 64, 64))
     R = np.maximum(raw.astype(np.float64) - 100, 0)
     tab = make_otf(gaussian_psf(1.5, 3.0))
     g = orc.init_geometry(64, 64, 32, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, (66, 17, 1))
@@ -339,7 +339,7 @@ def test_find_optimal_dimension_exhaustive(orc):
     assert orc.find_optimal_dimension(826) == 810 and orc.find_optimal_dimension(3727) == 3675
 
 
-def test_init_geometry_cfg1(orc):
+def test_init_geometry_lightsheet_128x512x512(orc):
     # 128 x 512 x 512 raw, 31.5 deg, dz 0.3, dx 0.104: deskewed 512 + 314 = 826 -> 810;
     # dz' = 0.15675; rotation gives 192 x 791 (z, x)
     g = orc.init_geometry(512, 512, 128, 0.104, 0.3, 0.104, 0.15675, 31.5, 31.5, 0, False, (66, 17, 1))
