@@ -275,10 +275,9 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
         }
       }
     } else {
-#ifndef CDN_CONV25_DZ_UNROLL
-#define CDN_CONV25_DZ_UNROLL 3
-#endif
-#pragma unroll CDN_CONV25_DZ_UNROLL
+    // 64-channel layers and the fused up-add: the dz loop is not unrolled (an
+    // unrolled one keeps too many fragments live: spills or occupancy 1)
+#pragma unroll((CIN == 64 || COUT == 64 || UPADD) ? 1 : 3)
     for (int dz = -1; dz <= 1; ++dz) {
       const bf16_t* base = ring + ((z + dz + 3) % 3) * PLANE;
 #pragma unroll
@@ -998,7 +997,7 @@ int grid_for(long n, int threads, int cap = 4096) {
 template <int CIN, int COUT, bool RELU, bool BIAS, bool UPADD = false>
 void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v,
                    const bf16_t* coarse = nullptr) {
-  constexpr int TX = CIN == 64 ? 32 : 64;
+  constexpr int TX = (CIN == 64 || COUT == 64) ? 32 : 64;
   // z-chunks of 16 planes (2 extra planes of loads per chunk), fewer when the
   // grid is already large
   const long tiles = (long)v.nb * ((v.ny + 3) / 4) * ((v.nx + TX - 1) / TX);
@@ -1038,7 +1037,7 @@ void conv_plain_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const
 // 16 input channels need the blocked kernel (tap-pair operand layout)
 template <int CIN, int COUT, bool RELU, bool BIAS>
 void conv_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v) {
-  if constexpr (CIN == 16 || (CDN_CONV25 && CIN <= 32 && COUT <= 32))
+  if constexpr (CIN == 16 || CDN_CONV25)
     conv25_launch<CIN, COUT, RELU, BIAS>(st, in, wf, bias, out, v);
   else
     conv_plain_launch<CIN, COUT, RELU, BIAS>(st, in, wf, bias, out, v);
