@@ -29,13 +29,15 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
-// round to nearest even (inputs are finite)
-__device__ __forceinline__ bf16_t f2bf(float f) {
-  unsigned u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (bf16_t)(u >> 16);
+// round to nearest even (v_cvt_pk_bf16_f32; inputs are finite)
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16_t f2bf(float f) { return __builtin_bit_cast(bf16_t, (__bf16)f); }
+__device__ __forceinline__ unsigned pack2(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2_t){a, b}, bf16x2_t));
 }
-__device__ __forceinline__ unsigned pack2(float a, float b) { return (unsigned)f2bf(a) | ((unsigned)f2bf(b) << 16); }
+// max(x, 0) for finite x as one v_med3_f32 (fmaxf adds a NaN-quieting max)
+__device__ __forceinline__ float relu(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 3.4028235e38f); }
 __device__ __forceinline__ float lo(unsigned w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float hi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
 
@@ -72,10 +74,10 @@ template <bool RELU>
 __device__ __forceinline__ void store4(bf16_t* p, f32x4 a, const float* b) {
   float v0 = a[0] + b[0], v1 = a[1] + b[1], v2 = a[2] + b[2], v3 = a[3] + b[3];
   if (RELU) {
-    v0 = fmaxf(v0, 0.f);
-    v1 = fmaxf(v1, 0.f);
-    v2 = fmaxf(v2, 0.f);
-    v3 = fmaxf(v3, 0.f);
+    v0 = relu(v0);
+    v1 = relu(v1);
+    v2 = relu(v2);
+    v3 = relu(v3);
   }
   u32x2 w;
   w[0] = pack2(v0, v1);
@@ -319,75 +321,101 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
 }
 
 // First layer: 1 -> 16 channels from the FP32 stack scaled by 1/s per sample,
-// K running over the 27 taps (padded to 32).  A workgroup of 4 waves covers
-// 2 (z) x 8 (y) x 64 (x) voxels: wave w takes x0 + 16w .. x0 + 16w + 15 and
-// its 16 M-blocks are the 16 (z, y) rows.  The LDS tile holds bf16(raw / s)
-// of planes z0-1..z0+2, rows y0-1..y0+8, columns x0-1..x0+64 (0 outside the
-// volume), so every lane reads its 8 taps at fixed offsets.
+// K running over the 27 taps (padded to 32 with zero weights).  A workgroup
+// of 4 waves owns rows y0..y0+7 and columns x0..x0+63 (wave w: x0 + 16w ..
+// x0 + 16w + 15, one M-block per row) and walks the planes zb..ze-1 of its
+// z-chunk.  LDS holds bf16(raw / s) of 4 planes over rows y0-1..y0+8 and
+// columns x0-1..x0+64 (0 outside the volume): plane p in slot
+// (p - zb + 1) mod 4, slots 0 and 1 mirrored in slots 4 and 5, so planes
+// z-1, z, z+1 are always the consecutive slots from (z - zb) mod 4 and each
+// lane reads its 8 taps at fixed offsets from one base per plane.  While
+// plane z is computed, plane z+2 is stored and plane z+3 is in flight to
+// registers; one barrier per plane orders the ring.
 // MODE 0: write e1;  1: write pool(e1) only;  2: fused head, write y;
 // 3: fused head, write the RL seed max(y, 0) * s.  In the head modes e1 is
 // recomputed here and never stored; d2 and the head parameters hp[0..16]
 // complete y = u + bh + sum_c Wh[c] (e1[c] + d2[v/2][c]).
-constexpr int FT_X = 66, FT_Y = 10, FT_ROW = FT_Y * FT_X, FT_N = 4 * FT_ROW;
+constexpr int FT_X = 66, FT_Y = 10, FT_PLANE = FT_Y * FT_X, FT_SLOTS = 6;
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_first(const float* __restrict__ raw, const float* __restrict__ inv_s,
                                                const bf16_t* __restrict__ wf, const float* __restrict__ bias,
                                                bf16_t* __restrict__ out16, const bf16_t* __restrict__ d2,
-                                               const float* __restrict__ hp, float* __restrict__ out32, Vol v) {
-  __shared__ bf16_t tile[FT_N];
-  const int xb = (v.nx + 63) / 64, yb = (v.ny + 7) / 8, hz = v.nz / 2;
+                                               const float* __restrict__ hp, float* __restrict__ out32, Vol v,
+                                               int zchunk) {
+  __shared__ bf16_t tile[FT_SLOTS * FT_PLANE];
+  const int xb = (v.nx + 63) / 64, yb = (v.ny + 7) / 8, zn = (v.nz + zchunk - 1) / zchunk;
   int blk = blockIdx.x;
   const int xt = blk % xb;
   blk /= xb;
   const int yt = blk % yb;
   blk /= yb;
-  const int zp = blk % hz;
-  const int b = blk / hz;
-  const int z0 = 2 * zp, y0 = 8 * yt, x0 = 64 * xt;
-  const float sc = inv_s[b];
-  const float* smp = raw + (long)b * v.nz * v.ny * v.nx;
-  // 40 tile rows of 66 columns; 16 threads per row, rows tid/16 + 16k.  All
-  // loads are issued before the first LDS store (one memory latency).
-  float val[3][5];
+  const int zt = blk % zn;
+  const int b = blk / zn;
+  const int y0 = 8 * yt, x0 = 64 * xt, zb = zt * zchunk;
+  const int ze = min(zb + zchunk, v.nz);
+  const float sc = inv_s[b], smean = 1.f / sc;
+  const long plane = (long)v.ny * v.nx;
+  const float* smp = raw + (long)b * v.nz * plane;
+  // loader: tile element e = tid + 256k (k < 3, e < 660) of a plane, row
+  // e / 66, column e % 66; in-plane offset and validity fixed per thread
+  int loff[3];
+  bool lok[3];
 #pragma unroll
-  for (int k3 = 0; k3 < 3; ++k3) {
-    const int rr = (threadIdx.x >> 4) + 16 * k3;
-    const int pp = rr / FT_Y, ry = rr - pp * FT_Y;
-    const int z = z0 - 1 + pp, y = y0 - 1 + ry;
-    const bool rok = rr < 4 * FT_Y && z >= 0 && z < v.nz && y >= 0 && y < v.ny;
-    const float* row = smp + ((long)(rok ? z : 0) * v.ny + (rok ? y : 0)) * v.nx;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int j = (threadIdx.x & 15) + 16 * k, x = x0 - 1 + j;
-      const bool ok = rok && j < FT_X && x >= 0 && x < v.nx;
-      const float f = row[ok ? x : 0];
-      val[k3][k] = ok ? f * sc : 0.f;
-    }
+  for (int k = 0; k < 3; ++k) {
+    const int e = threadIdx.x + 256 * k, ry = e / FT_X, cx = e - ry * FT_X;
+    const int y = y0 - 1 + ry, x = x0 - 1 + cx;
+    lok[k] = e < FT_PLANE && y >= 0 && y < v.ny && x >= 0 && x < v.nx;
+    loff[k] = lok[k] ? y * v.nx + x : 0;
   }
+  auto load_plane = [&](int z, float (&val)[3]) {
+    const bool zok = z >= 0 && z < v.nz;
+    const float* pl = smp + (zok ? z : 0) * plane;
 #pragma unroll
-  for (int k3 = 0; k3 < 3; ++k3) {
-    const int rr = (threadIdx.x >> 4) + 16 * k3;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int j = (threadIdx.x & 15) + 16 * k;
-      if (rr < 4 * FT_Y && j < FT_X) tile[rr * FT_X + j] = f2bf(val[k3][k]);
+    for (int k = 0; k < 3; ++k) {
+      const float f = pl[loff[k]];
+      val[k] = zok && lok[k] ? f * sc : 0.f;
     }
+  };
+  auto store_plane = [&](int z, const float (&val)[3]) {
+    const int slot = (z - zb + 1) & 3;
+    bf16_t* sl = tile + slot * FT_PLANE;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      if (e < FT_PLANE) {
+        const bf16_t h = f2bf(val[k]);
+        sl[e] = h;
+        if (slot < 2) sl[4 * FT_PLANE + e] = h;
+      }
+    }
+  };
+  {
+    float p0[3], p1[3], p2[3];
+    load_plane(zb - 1, p0);
+    load_plane(zb, p1);
+    load_plane(zb + 1, p2);
+    store_plane(zb - 1, p0);
+    store_plane(zb, p1);
+    store_plane(zb + 1, p2);
   }
+  float nxt[3] = {0.f, 0.f, 0.f};
+  if (zb + 2 <= ze) load_plane(zb + 2, nxt);
   __syncthreads();
+
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15, w = threadIdx.x >> 6;
   const u32x4 a = *reinterpret_cast<const u32x4*>(wf + lane * 8);
-  int off[8];
-  unsigned tm[8];  // 0xffff for a real tap, 0 for the padding taps 27..31
+  // tap offsets within the 3 consecutive slots of planes z-1..z+1; the
+  // padding taps 27..31 read tap 26's element (their weights are 0)
+  int toff[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int t = 8 * q + i;
     int dz, dy, dx;
     tap_offsets(t < 27 ? t : 26, dz, dy, dx);
-    tm[i] = t < 27 ? 0xffffu : 0u;
-    off[i] = (1 + dz) * FT_ROW + (1 + dy) * FT_X + (1 + dx) + 16 * w + r;
+    toff[i] = (1 + dz) * FT_PLANE + (1 + dy) * FT_X + (1 + dx) + 16 * w + r;
   }
-  float bl[4];
+  f32x4 bl;
 #pragma unroll
   for (int i = 0; i < 4; ++i) bl[i] = bias[4 * q + i];
   const int x = x0 + 16 * w + r;
@@ -398,86 +426,135 @@ __global__ __launch_bounds__(256) void k_first(const float* __restrict__ raw, co
     for (int i = 0; i < 4; ++i) wh[i] = hp[4 * q + i];
     bh = hp[16];
   }
-  float ps[4][4];  // MODE 1: pooled sums per y pair
+  const int hz = v.nz / 2;
+  float ps[4][4];  // MODE 1: pooled sums per y pair over a z pair
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
     for (int i = 0; i < 4; ++i) ps[k][i] = 0.f;
-  // head modes: the 4 coarse d2 fragments and the 16 FP32 inputs of this
-  // lane's voxels, loaded before the MFMA loop
-  u32x2 dfr[4];
-  float uin[16];
-  if (MODE >= 2) {
+  float dd[4][4];  // head modes: coarse d2 of the current z pair, per y pair
+
+  // head modes: after the 8 rows, lane (q, r) owns rows hrow(q, 0..1) of
+  // column x (see the transposed reduction below)
+  const int hrow0 = 4 * (q & 1) + 2 * (q >> 1);
+  for (int z = zb; z < ze; ++z) {
+    const bf16_t* tz = tile + ((z - zb) & 3) * FT_PLANE;
+    float uin[2];
+    if (MODE >= 2) {
+      if (((z - zb) & 1) == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int yc = (y0 >> 1) + k;
-      const bool ok = xin && 2 * yc < v.ny;
-      const long cw = (((long)b * hz + zp) * (v.ny / 2) + (ok ? yc : 0)) * (v.nx / 2) + (ok ? (x >> 1) : 0);
-      const u32x2 d = *reinterpret_cast<const u32x2*>(d2 + cw * 16 + 4 * q);
-      dfr[k] = ok ? d : u32x2(0u);
-    }
+        for (int k = 0; k < 4; ++k) {
+          const int yc = (y0 >> 1) + k;
+          const bool ok = xin && 2 * yc < v.ny;
+          const long cw = (((long)b * hz + (z >> 1)) * (v.ny / 2) + (ok ? yc : 0)) * (v.nx / 2) + (ok ? (x >> 1) : 0);
+          u32x2 d = *reinterpret_cast<const u32x2*>(d2 + cw * 16 + 4 * q);
+          d = ok ? d : u32x2(0u);
+          dd[k][0] = lo(d[0]);
+          dd[k][1] = hi(d[0]);
+          dd[k][2] = lo(d[1]);
+          dd[k][3] = hi(d[1]);
+        }
+      }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int y = y0 + (m & 7);
-      const bool ok = xin && y < v.ny && q == 0;
-      const long vox = (((long)b * v.nz + z0 + (m >> 3)) * v.ny + (ok ? y : 0)) * v.nx + (ok ? x : 0);
-      const float f = raw[vox];
-      uin[m] = ok ? f * sc : 0.f;
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const int mz = m >> 3, my = m & 7;
-    const int base = mz * FT_ROW + my * FT_X;
-    u32x4 bf;
-#pragma unroll
-    for (int i = 0; i < 8; i += 2) {
-      const unsigned l16 = (unsigned)tile[base + off[i]] & tm[i];
-      const unsigned h16 = (unsigned)tile[base + off[i + 1]] & tm[i + 1];
-      bf[i / 2] = l16 | (h16 << 16);
-    }
-    const f32x4 acc = Mfma<32>::run(a, bf, f32x4{0.f, 0.f, 0.f, 0.f});
-    const int z = z0 + mz, y = y0 + my;
-    const bool yin = y < v.ny;
-    const long vox = (((long)b * v.nz + z) * v.ny + y) * v.nx + x;
-    if (MODE == 0) {
-      if (xin && yin) store4<true>(out16 + vox * 16 + 4 * q, acc, bl);
-      continue;
-    }
-    float e[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) e[i] = bf2f(f2bf(fmaxf(acc[i] + bl[i], 0.f)));  // e1 as stored by MODE 0
-    if (MODE == 1) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) ps[my >> 1][i] += e[i];
-      continue;
-    }
-    const u32x2 d = dfr[my >> 1];
-    const float dd[4] = {lo(d[0]), hi(d[0]), lo(d[1]), hi(d[1])};
-    float part = 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) part += wh[i] * (e[i] + dd[i]);
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
-    if (xin && yin && q == 0) {
-      const float yv = uin[m] + bh + part;
-      out32[vox] = MODE == 2 ? yv : fmaxf(yv, 0.f) / sc;
-    }
-  }
-  if (MODE == 1) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) ps[k][i] += __shfl_xor(ps[k][i], 1);
-      const int yc = (y0 >> 1) + k;
-      if (xin && (r & 1) == 0 && 2 * yc < v.ny) {
-        const long cw = (((long)b * hz + zp) * (v.ny / 2) + yc) * (v.nx / 2) + (x >> 1);
-        u32x2 o;
-        o[0] = pack2(0.125f * ps[k][0], 0.125f * ps[k][1]);
-        o[1] = pack2(0.125f * ps[k][2], 0.125f * ps[k][3]);
-        *reinterpret_cast<u32x2*>(out16 + cw * 16 + 4 * q) = o;
+      for (int j = 0; j < 2; ++j) {
+        const int y = y0 + hrow0 + j;
+        const bool ok = xin && y < v.ny;
+        const float f = smp[z * plane + (ok ? y : 0) * v.nx + (ok ? x : 0)];
+        uin[j] = ok ? f * sc : 0.f;
       }
     }
+    auto gather = [&](int my) {
+      const bf16_t* tb = tz + my * FT_X;
+      u32x4 bf;
+#pragma unroll
+      for (int i = 0; i < 8; i += 2) bf[i / 2] = (unsigned)tb[toff[i]] | ((unsigned)tb[toff[i + 1]] << 16);
+      return bf;
+    };
+    u32x2 o16[8];    // MODE 0: e1 of the 8 rows
+    float part[8];   // head modes: this lane's 4-channel share of sum_c Wh[c] (e1 + d2)
+    u32x4 bcur = gather(0);
+#pragma unroll
+    for (int my = 0; my < 8; ++my) {
+      const f32x4 acc = Mfma<32>::run(a, bcur, bl);  // bias as the accumulator input
+      if (my < 7) bcur = gather(my + 1);              // next row's operand while the MFMA runs
+      const unsigned e01 = pack2(relu(acc[0]), relu(acc[1]));
+      const unsigned e23 = pack2(relu(acc[2]), relu(acc[3]));
+      if (MODE == 0) {
+        o16[my] = u32x2{e01, e23};
+        continue;
+      }
+      const float e[4] = {lo(e01), hi(e01), lo(e23), hi(e23)};  // e1 as stored by MODE 0
+      if (MODE == 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ps[my >> 1][i] += e[i];
+        continue;
+      }
+      float pt = 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pt += wh[i] * (e[i] + dd[my >> 1][i]);
+      part[my] = pt;
+    }
+    if (MODE == 0) {
+#pragma unroll
+      for (int my = 0; my < 8; ++my) {
+        const int y = y0 + my;
+        if (xin && y < v.ny) {
+          const long vox = (((long)b * v.nz + z) * v.ny + y) * v.nx + x;
+          *reinterpret_cast<u32x2*>(out16 + vox * 16 + 4 * q) = o16[my];
+        }
+      }
+    }
+    if (MODE >= 2) {
+      // sum over the 4 lane groups, transposed: xor 16 pairs groups (0,1) and
+      // (2,3), even groups keeping rows 0..3 and odd ones rows 4..7; xor 32
+      // then leaves group q rows hrow0, hrow0 + 1.  Each total is
+      // (g0 + g1) + (g2 + g3) in this order.
+      float t4[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool odd = q & 1;
+        const float keep = odd ? part[4 + j] : part[j], give = odd ? part[j] : part[4 + j];
+        t4[j] = keep + __shfl_xor(give, 16);
+      }
+      float tot[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const bool hi2 = q >> 1;
+        const float keep = hi2 ? t4[2 + j] : t4[j], give = hi2 ? t4[j] : t4[2 + j];
+        tot[j] = keep + __shfl_xor(give, 32);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int y = y0 + hrow0 + j;
+        if (xin && y < v.ny) {
+          const long vox = (((long)b * v.nz + z) * v.ny + y) * v.nx + x;
+          const float yv = uin[j] + bh + tot[j];
+          out32[vox] = MODE == 2 ? yv : relu(yv) * smean;
+        }
+      }
+    }
+    if (MODE == 1 && ((z - zb) & 1) == 1) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ps[k][i] += __shfl_xor(ps[k][i], 1);
+        const int yc = (y0 >> 1) + k;
+        if (xin && (r & 1) == 0 && 2 * yc < v.ny) {
+          const long cw = (((long)b * hz + (z >> 1)) * (v.ny / 2) + yc) * (v.nx / 2) + (x >> 1);
+          u32x2 o;
+          o[0] = pack2(0.125f * ps[k][0], 0.125f * ps[k][1]);
+          o[1] = pack2(0.125f * ps[k][2], 0.125f * ps[k][3]);
+          *reinterpret_cast<u32x2*>(out16 + cw * 16 + 4 * q) = o;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ps[k][i] = 0.f;
+      }
+    }
+    // plane z+2 into the slot of z-2 (last read before the previous barrier),
+    // plane z+3 into registers
+    if (z + 2 <= ze) store_plane(z + 2, nxt);
+    if (z + 3 <= ze) load_plane(z + 3, nxt);
+    __syncthreads();
   }
 }
 
@@ -1081,20 +1158,30 @@ void net_conv_upadd_32_16(hipStream_t st, const bf16_t* coarse, const bf16_t* sk
 
 void net_first(hipStream_t st, int mode, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,
                bf16_t* out16, const bf16_t* d2, const float* hp, float* out32, Vol v) {
-  const long grid = (long)v.nb * (v.nz / 2) * ((v.ny + 7) / 8) * ((v.nx + 63) / 64);
+  if (v.nz % 2 != 0) fail("net_first: nz must be even");
+  // z-chunks of 16 planes (even, so z pairs never straddle chunks), 32 when
+  // the grid is already large
+  const long tiles = (long)v.nb * ((v.ny + 7) / 8) * ((v.nx + 63) / 64);
+  int zchunk = v.nz < 16 ? v.nz : 16;
+  if (tiles >= 4096) zchunk = v.nz < 32 ? v.nz : 32;
+  const long grid = tiles * ((v.nz + zchunk - 1) / zchunk);
   if (grid > 0x7fffffffL) fail("net_first: volume too large");
   switch (mode) {
     case 0:
-      hipLaunchKernelGGL(k_first<0>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      hipLaunchKernelGGL(k_first<0>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v,
+                         zchunk);
       return;
     case 1:
-      hipLaunchKernelGGL(k_first<1>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      hipLaunchKernelGGL(k_first<1>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v,
+                         zchunk);
       return;
     case 2:
-      hipLaunchKernelGGL(k_first<2>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      hipLaunchKernelGGL(k_first<2>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v,
+                         zchunk);
       return;
     case 3:
-      hipLaunchKernelGGL(k_first<3>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v);
+      hipLaunchKernelGGL(k_first<3>, dim3(grid), dim3(256), 0, st, raw, inv_s, wf, bias, out16, d2, hp, out32, v,
+                         zchunk);
       return;
   }
   fail("net_first: mode");
