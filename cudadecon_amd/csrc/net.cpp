@@ -139,8 +139,7 @@ void WarmNet::forward(hipStream_t st, const float* raw, bool keep_e1) {
               nullptr, L1);
   }
   e1_kept = keep_e1;
-  net_conv(st, 16, 32, true, p1.as<bf16_t>(), wf[1].as<bf16_t>(), p + b_off[1], e2.as<bf16_t>(), L2);
-  net_pool(st, 32, e2.as<bf16_t>(), p2.as<bf16_t>(), L2);
+  net_conv_pool_16_32(st, p1.as<bf16_t>(), wf[1].as<bf16_t>(), p + b_off[1], e2.as<bf16_t>(), p2.as<bf16_t>(), L2);
   net_conv(st, 32, 64, true, p2.as<bf16_t>(), wf[2].as<bf16_t>(), p + b_off[2], e3.as<bf16_t>(), L3);
   net_conv(st, 64, 32, true, e3.as<bf16_t>(), wf[3].as<bf16_t>(), p + b_off[3], d3.as<bf16_t>(), L3);
   if (keep_e1) {  // training keeps a2 (the input of the decoder's weight gradient)

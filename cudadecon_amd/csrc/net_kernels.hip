@@ -71,7 +71,7 @@ __host__ __device__ constexpr long packed_weights(int ci, int co) {
 
 // Store 4 consecutive channels (acc + bias, optional ReLU) of one voxel.
 template <bool RELU>
-__device__ __forceinline__ void store4(bf16_t* p, f32x4 a, const float* b) {
+__device__ __forceinline__ u32x2 epi4(f32x4 a, const float* b) {
   float v0 = a[0] + b[0], v1 = a[1] + b[1], v2 = a[2] + b[2], v3 = a[3] + b[3];
   if (RELU) {
     v0 = relu(v0);
@@ -82,7 +82,11 @@ __device__ __forceinline__ void store4(bf16_t* p, f32x4 a, const float* b) {
   u32x2 w;
   w[0] = pack2(v0, v1);
   w[1] = pack2(v2, v3);
-  *reinterpret_cast<u32x2*>(p) = w;
+  return w;
+}
+template <bool RELU>
+__device__ __forceinline__ void store4(bf16_t* p, f32x4 a, const float* b) {
+  *reinterpret_cast<u32x2*>(p) = epi4<RELU>(a, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -178,10 +182,16 @@ __global__ __launch_bounds__(THREADS) void k_conv(const bf16_t* __restrict__ in,
 // UPADD: the input is up(coarse) + in (nearest upsampling of a half-size
 // tensor of the same channels), formed while the planes are loaded, as
 // k_upadd stores it (bf16 of the FP32 sum).
-template <int CIN, int COUT, int TX, bool RELU, bool BIAS, bool UPADD>
+// POOL: also writes pooled = 2x2x2 mean of the output, as k_pool computes it
+// from the stored values (same children order, FP32 sums).  The bf16 outputs
+// of a z pair go to an LDS buffer; after the pair's second plane each thread
+// forms one 8-channel group of one coarse voxel of the tile (z chunks start
+// on even planes, so pairs never straddle chunks).
+template <int CIN, int COUT, int TX, bool RELU, bool BIAS, bool UPADD, bool POOL = false>
 __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, const bf16_t* __restrict__ wf,
                                                 const float* __restrict__ bias, bf16_t* __restrict__ out, Vol v,
-                                                int zchunk, const bf16_t* __restrict__ coarse) {
+                                                int zchunk, const bf16_t* __restrict__ coarse,
+                                                bf16_t* __restrict__ pooled) {
   constexpr int KS = CIN == 16 ? 16 : 32, NS = CIN / KS, NBO = COUT / 16, FR = KS / 4;
   constexpr int TY = 4, MB = TX / 16, RW = TX + 2, ROWS = TY + 2;
   constexpr int ROWCH = RW * CIN / 8;          // 16-byte chunks per tile row
@@ -190,6 +200,8 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
   constexpr int PER_T = (CHUNKS + 255) / 256;  // chunks per thread
   typedef typename Mfma<KS>::frag frag;
   __shared__ __attribute__((aligned(16))) bf16_t ring[3 * PLANE];
+  __shared__ __attribute__((aligned(16))) bf16_t pool_buf[POOL ? 2 * TY * TX * COUT : 8];
+  static_assert(!POOL || TX * COUT / 8 == 256, "POOL: one 8-channel coarse group per thread");
   const int xt_n = (v.nx + TX - 1) / TX, yt_n = (v.ny + TY - 1) / TY, zc_n = (v.nz + zchunk - 1) / zchunk;
   int blk = blockIdx.x;
   const int xt = blk % xt_n;
@@ -305,7 +317,22 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
       }
     }
     }
-    if (y < v.ny) {
+    if constexpr (POOL) {
+      // every row and column of the tile goes to the pool buffer (the ones
+      // outside the volume hold finite values that no output uses)
+      bf16_t* pb = pool_buf + (((z - z0) & 1) * TY + w) * TX * COUT;
+      bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + (y < v.ny ? y : 0)) * v.nx) * COUT;
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const int x = x0 + 16 * mb + r;
+#pragma unroll
+        for (int j = 0; j < NBO; ++j) {
+          const u32x2 e = epi4<RELU>(acc[j][mb], bl[j]);
+          *reinterpret_cast<u32x2*>(pb + (16 * mb + r) * COUT + 16 * j + 4 * q) = e;
+          if (y < v.ny && x < v.nx) *reinterpret_cast<u32x2*>(dst + (long)x * COUT + 16 * j + 4 * q) = e;
+        }
+      }
+    } else if (y < v.ny) {
       bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + y) * v.nx) * COUT;
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
@@ -316,6 +343,31 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
       }
     }
     __syncthreads();                     // every wave is done with plane z-1
+    if constexpr (POOL) {
+      if ((z - z0) & 1) {
+        constexpr int G = COUT / 8;
+        const int cg = threadIdx.x % G, cxl = (threadIdx.x / G) % (TX / 2), cyl = threadIdx.x / G / (TX / 2);
+        float sm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int row = (c >> 2) * TY + 2 * cyl + ((c >> 1) & 1), col = 2 * cxl + (c & 1);
+          const u32x4 d = *reinterpret_cast<const u32x4*>(pool_buf + (row * TX + col) * COUT + 8 * cg);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            sm[2 * k] += lo(d[k]);
+            sm[2 * k + 1] += hi(d[k]);
+          }
+        }
+        const int hy = v.ny / 2, hx = v.nx / 2, cy = (y0 >> 1) + cyl, cx = (x0 >> 1) + cxl;
+        if (cy < hy && cx < hx) {
+          u32x4 o;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) o[k] = pack2(sm[2 * k] * 0.125f, sm[2 * k + 1] * 0.125f);
+          const long ow = (((long)b * (v.nz / 2) + (z >> 1)) * hy + cy) * (long)hx + cx;
+          *reinterpret_cast<u32x4*>(pooled + ow * COUT + 8 * cg) = o;
+        }
+      }
+    }
     if (z + 2 <= z1) store_plane(z + 2);  // into the slot of plane z-1
   }
 }
@@ -1071,9 +1123,9 @@ int grid_for(long n, int threads, int cap = 4096) {
 #ifndef CDN_CONV25
 #define CDN_CONV25 1
 #endif
-template <int CIN, int COUT, bool RELU, bool BIAS, bool UPADD = false>
+template <int CIN, int COUT, bool RELU, bool BIAS, bool UPADD = false, bool POOL = false>
 void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v,
-                   const bf16_t* coarse = nullptr) {
+                   const bf16_t* coarse = nullptr, bf16_t* pooled = nullptr) {
   constexpr int TX = (CIN == 64 || COUT == 64) ? 32 : 64;
   // z-chunks of 16 planes (2 extra planes of loads per chunk), fewer when the
   // grid is already large
@@ -1081,8 +1133,8 @@ void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const flo
   int zchunk = v.nz < 16 ? v.nz : 16;
   if (tiles >= 4096) zchunk = v.nz < 32 ? v.nz : 32;
   const long grid = tiles * ((v.nz + zchunk - 1) / zchunk);
-  hipLaunchKernelGGL((k_conv25<CIN, COUT, TX, RELU, BIAS, UPADD>), dim3((unsigned)grid), dim3(256), 0, st, in, wf,
-                     bias, out, v, zchunk, coarse);
+  hipLaunchKernelGGL((k_conv25<CIN, COUT, TX, RELU, BIAS, UPADD, POOL>), dim3((unsigned)grid), dim3(256), 0, st, in,
+                     wf, bias, out, v, zchunk, coarse, pooled);
 }
 
 #ifndef CDN_CONV_MB
@@ -1154,6 +1206,12 @@ void net_conv(hipStream_t st, int cin, int cout, bool relu, const bf16_t* in, co
 void net_conv_upadd_32_16(hipStream_t st, const bf16_t* coarse, const bf16_t* skip, const bf16_t* wf,
                           const float* bias, bf16_t* out, Vol v) {
   conv25_launch<32, 16, true, true, true>(st, skip, wf, bias, out, v, coarse);
+}
+
+void net_conv_pool_16_32(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out,
+                         bf16_t* pooled, Vol v) {
+  if (v.nz % 2 || v.ny % 2 || v.nx % 2) fail("net_conv_pool_16_32: dimensions must be even");
+  conv25_launch<16, 32, true, true, false, true>(st, in, wf, bias, out, v, nullptr, pooled);
 }
 
 void net_first(hipStream_t st, int mode, const float* raw, const float* inv_s, const bf16_t* wf, const float* bias,

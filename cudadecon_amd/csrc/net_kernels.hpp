@@ -22,6 +22,10 @@ void net_first(hipStream_t st, int mode, const float* raw, const float* inv_s, c
                bf16_t* out16, const bf16_t* d2, const float* hp, float* out32, Vol v);
 // 32 -> 16 convolution (bias + ReLU) of up(coarse) + skip, the sum formed
 // on load (inference: the decoder's a2 is never stored)
+// enc2 (16 -> 32, ReLU, bias) writing out and also pooled = pool(out) (2x2x2
+// mean, as net_pool computes it); dimensions even
+void net_conv_pool_16_32(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out,
+                         bf16_t* pooled, Vol v);
 void net_conv_upadd_32_16(hipStream_t st, const bf16_t* coarse, const bf16_t* skip, const bf16_t* wf,
                           const float* bias, bf16_t* out, Vol v);
 void net_pool(hipStream_t st, int c, const bf16_t* in, bf16_t* out, Vol fine);
