@@ -65,6 +65,39 @@ def test_forward_layers_match_oracle(cdn, shape):
     assert s.min() >= 0.0
 
 
+def _bf16(x):
+    """Round float32 to the nearest BF16 (ties to even), kept as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32)
+    u = (u + np.uint32(0x7FFF) + ((u >> 16) & np.uint32(1))) & np.uint32(0xFFFF0000)
+    return u.view(np.float32)
+
+
+@pytest.mark.parametrize("shape", SHAPES + [(1, 64, 64, 256)], ids=lambda s: "x".join(map(str, s)))
+def test_pooled_tensors_are_exact_means_of_the_stored_ones(cdn, shape):
+    """p1 = pool(e1) (k_pool) and p2 = pool(e2), which the enc2 convolution
+    writes from its LDS pair buffer: both are bitwise BF16(0.125 * s), s the
+    FP32 sum of the 8 stored BF16 children in (dz, dy, dx) order from 0.
+    1x64x64x256 spans two z-chunks of the convolution and a 128-wide half
+    level; 1x12x20x36 has partial tiles in x and y at half resolution."""
+    print("pool exactness", shape, flush=True)
+    p = _params(5)
+    raw, _ = _stack(shape, 6)
+    net = cdn.WarmNet(params=p)
+    net.forward(raw)
+    nb, nz, ny, nx = shape
+    for fine, coarse, c, f in (("e1", "p1", 16, 1), ("e2", "p2", 32, 2)):
+        e = net.fetch(fine).reshape(nb, nz // f, ny // f, nx // f, c)
+        got = net.fetch(coarse).reshape(nb, nz // (2 * f), ny // (2 * f), nx // (2 * f), c)
+        s = np.zeros(got.shape, np.float32)
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    s = (s + e[:, dz::2, dy::2, dx::2, :]).astype(np.float32)
+        ref = _bf16(s * np.float32(0.125))
+        assert np.abs(e).max() > 0, fine
+        assert np.array_equal(got, ref), (coarse, int((got != ref).sum()))
+
+
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 def test_inference_path_matches_training_path(cdn, shape):
     """The warm-start path (pooled first layer, e1 recomputed in the head)
