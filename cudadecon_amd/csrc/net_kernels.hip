@@ -194,6 +194,10 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
                                                 bf16_t* __restrict__ pooled) {
   constexpr int KS = CIN == 16 ? 16 : 32, NS = CIN / KS, NBO = COUT / 16, FR = KS / 4;
   constexpr int TY = 4, MB = TX / 16, RW = TX + 2, ROWS = TY + 2;
+  // one output-channel block and one K step: the waves split the tile by
+  // column blocks instead of rows
+  constexpr bool COLBLK = CIN == 32 && COUT == 16 && !POOL;
+  static_assert(!COLBLK || MB == TY, "one column block per wave");
   constexpr int ROWCH = RW * CIN / 8;          // 16-byte chunks per tile row
   constexpr int PLANE = ROWS * RW * CIN;       // bf16 per plane
   constexpr int CHUNKS = ROWS * ROWCH;         // 16-byte chunks per plane
@@ -259,87 +263,125 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
   for (int z = z0; z < z1; ++z) {
     if (z + 2 <= z1) load_plane(z + 2);  // in flight while plane z is computed
     __syncthreads();
-    f32x4 acc[NBO][MB];
+    if constexpr (COLBLK) {
+      // 32 -> 16: wave w owns columns x0 + 16w .. + 15 of all TY rows, so the
+      // B fragment of an input row (read once from LDS) feeds the up to 3
+      // output rows it is a tap of; taps accumulate in (dz, dx, dy) order
+      f32x4 acc4[TY];
 #pragma unroll
-    for (int j = 0; j < NBO; ++j)
+      for (int j = 0; j < TY; ++j) acc4[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) acc[j][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    if constexpr (CIN == 16) {
-      // 14 tap pairs on the full-rate K = 32 instruction: lanes q < 2 read
-      // tap 2p, lanes q >= 2 tap 2p + 1 (tap 27 has zero weights)
-      int slot[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) slot[d] = ((z + d - 1 + 3) % 3) * PLANE;
-      const bool hi = q >= 2;
-      const int cofs = 8 * (q & 1);
-#pragma unroll
-      for (int pr = 0; pr < 14; ++pr) {
-        const int t0 = 2 * pr, t1 = 2 * pr + 1 < 27 ? 2 * pr + 1 : 26;
-        const int o0 = slot[t0 / 9] + ((w + (t0 / 3) % 3) * RW + t0 % 3) * 16;
-        const int o1 = slot[t1 / 9] + ((w + (t1 / 3) % 3) * RW + t1 % 3) * 16;
-        const bf16_t* base = ring + (hi ? o1 : o0) + cofs;
-        u32x4 bfr[MB];
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb) bfr[mb] = *reinterpret_cast<const u32x4*>(base + (16 * mb + r) * 16);
-#pragma unroll
-        for (int j = 0; j < NBO; ++j) {
-          const u32x4 a = *reinterpret_cast<const u32x4*>(wf + ((pr * NBO + j) * 64 + lane) * 8);
-#pragma unroll
-          for (int mb = 0; mb < MB; ++mb) acc[j][mb] = Mfma<32>::run(a, bfr[mb], acc[j][mb]);
-        }
-      }
-    } else {
-    // 64-channel layers and the fused up-add: the dz loop is not unrolled (an
-    // unrolled one keeps too many fragments live: spills or occupancy 1)
-#pragma unroll((CIN == 64 || COUT == 64 || UPADD) ? 1 : 3)
-    for (int dz = -1; dz <= 1; ++dz) {
-      const bf16_t* base = ring + ((z + dz + 3) % 3) * PLANE;
-#pragma unroll
-      for (int dy = -1; dy <= 1; ++dy) {
+      for (int dz = -1; dz <= 1; ++dz) {
+        const bf16_t* base = ring + ((z + dz + 3) % 3) * PLANE;
 #pragma unroll
         for (int dx = -1; dx <= 1; ++dx) {
-          const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+          frag a[3];
 #pragma unroll
-          for (int s = 0; s < NS; ++s) {
-            frag bfr[MB];
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+            a[dy + 1] = *reinterpret_cast<const frag*>(wf + (t * 64 + lane) * FR);
+          }
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb)
-              bfr[mb] = *reinterpret_cast<const frag*>(base + ((w + 1 + dy) * RW + 16 * mb + r + 1 + dx) * CIN +
-                                                       s * KS + q * FR);
+          for (int rr = 0; rr < ROWS; ++rr) {
+            const frag bfr = *reinterpret_cast<const frag*>(base + (rr * RW + 16 * w + r + 1 + dx) * CIN + q * FR);
 #pragma unroll
-            for (int j = 0; j < NBO; ++j) {
-              const frag a = *reinterpret_cast<const frag*>(wf + (((t * NS + s) * NBO + j) * 64 + lane) * FR);
-#pragma unroll
-              for (int mb = 0; mb < MB; ++mb) acc[j][mb] = Mfma<KS>::run(a, bfr[mb], acc[j][mb]);
+            for (int dy = -1; dy <= 1; ++dy) {
+              const int j = rr - 1 - dy;
+              if (j >= 0 && j < TY) acc4[j] = Mfma<KS>::run(a[dy + 1], bfr, acc4[j]);
             }
           }
         }
       }
-    }
-    }
-    if constexpr (POOL) {
-      // every row and column of the tile goes to the pool buffer (the ones
-      // outside the volume hold finite values that no output uses)
-      bf16_t* pb = pool_buf + (((z - z0) & 1) * TY + w) * TX * COUT;
-      bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + (y < v.ny ? y : 0)) * v.nx) * COUT;
+      const int x = x0 + 16 * w + r;
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const int x = x0 + 16 * mb + r;
-#pragma unroll
-        for (int j = 0; j < NBO; ++j) {
-          const u32x2 e = epi4<RELU>(acc[j][mb], bl[j]);
-          *reinterpret_cast<u32x2*>(pb + (16 * mb + r) * COUT + 16 * j + 4 * q) = e;
-          if (y < v.ny && x < v.nx) *reinterpret_cast<u32x2*>(dst + (long)x * COUT + 16 * j + 4 * q) = e;
+      for (int j = 0; j < TY; ++j) {
+        const int yj = y0 + j;
+        if (yj < v.ny && x < v.nx)
+          store4<RELU>(out + ((((long)b * v.nz + z) * v.ny + yj) * v.nx + x) * COUT + 4 * q, acc4[j], bl[0]);
+      }
+    } else {
+      f32x4 acc[NBO][MB];
+  #pragma unroll
+      for (int j = 0; j < NBO; ++j)
+  #pragma unroll
+        for (int mb = 0; mb < MB; ++mb) acc[j][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+      if constexpr (CIN == 16) {
+        // 14 tap pairs on the full-rate K = 32 instruction: lanes q < 2 read
+        // tap 2p, lanes q >= 2 tap 2p + 1 (tap 27 has zero weights)
+        int slot[3];
+  #pragma unroll
+        for (int d = 0; d < 3; ++d) slot[d] = ((z + d - 1 + 3) % 3) * PLANE;
+        const bool hi = q >= 2;
+        const int cofs = 8 * (q & 1);
+  #pragma unroll
+        for (int pr = 0; pr < 14; ++pr) {
+          const int t0 = 2 * pr, t1 = 2 * pr + 1 < 27 ? 2 * pr + 1 : 26;
+          const int o0 = slot[t0 / 9] + ((w + (t0 / 3) % 3) * RW + t0 % 3) * 16;
+          const int o1 = slot[t1 / 9] + ((w + (t1 / 3) % 3) * RW + t1 % 3) * 16;
+          const bf16_t* base = ring + (hi ? o1 : o0) + cofs;
+          u32x4 bfr[MB];
+  #pragma unroll
+          for (int mb = 0; mb < MB; ++mb) bfr[mb] = *reinterpret_cast<const u32x4*>(base + (16 * mb + r) * 16);
+  #pragma unroll
+          for (int j = 0; j < NBO; ++j) {
+            const u32x4 a = *reinterpret_cast<const u32x4*>(wf + ((pr * NBO + j) * 64 + lane) * 8);
+  #pragma unroll
+            for (int mb = 0; mb < MB; ++mb) acc[j][mb] = Mfma<32>::run(a, bfr[mb], acc[j][mb]);
+          }
+        }
+      } else {
+      // 64-channel layers and the fused up-add: the dz loop is not unrolled (an
+      // unrolled one keeps too many fragments live: spills or occupancy 1)
+  #pragma unroll((CIN == 64 || COUT == 64 || UPADD) ? 1 : 3)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const bf16_t* base = ring + ((z + dz + 3) % 3) * PLANE;
+  #pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+  #pragma unroll
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+  #pragma unroll
+            for (int s = 0; s < NS; ++s) {
+              frag bfr[MB];
+  #pragma unroll
+              for (int mb = 0; mb < MB; ++mb)
+                bfr[mb] = *reinterpret_cast<const frag*>(base + ((w + 1 + dy) * RW + 16 * mb + r + 1 + dx) * CIN +
+                                                         s * KS + q * FR);
+  #pragma unroll
+              for (int j = 0; j < NBO; ++j) {
+                const frag a = *reinterpret_cast<const frag*>(wf + (((t * NS + s) * NBO + j) * 64 + lane) * FR);
+  #pragma unroll
+                for (int mb = 0; mb < MB; ++mb) acc[j][mb] = Mfma<KS>::run(a, bfr[mb], acc[j][mb]);
+              }
+            }
+          }
         }
       }
-    } else if (y < v.ny) {
-      bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + y) * v.nx) * COUT;
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const int x = x0 + 16 * mb + r;
-        if (x < v.nx)
-#pragma unroll
-          for (int j = 0; j < NBO; ++j) store4<RELU>(dst + (long)x * COUT + 16 * j + 4 * q, acc[j][mb], bl[j]);
+      }
+      if constexpr (POOL) {
+        // every row and column of the tile goes to the pool buffer (the ones
+        // outside the volume hold finite values that no output uses)
+        bf16_t* pb = pool_buf + (((z - z0) & 1) * TY + w) * TX * COUT;
+        bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + (y < v.ny ? y : 0)) * v.nx) * COUT;
+  #pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+          const int x = x0 + 16 * mb + r;
+  #pragma unroll
+          for (int j = 0; j < NBO; ++j) {
+            const u32x2 e = epi4<RELU>(acc[j][mb], bl[j]);
+            *reinterpret_cast<u32x2*>(pb + (16 * mb + r) * COUT + 16 * j + 4 * q) = e;
+            if (y < v.ny && x < v.nx) *reinterpret_cast<u32x2*>(dst + (long)x * COUT + 16 * j + 4 * q) = e;
+          }
+        }
+      } else if (y < v.ny) {
+        bf16_t* dst = out + ((((long)b * v.nz + z) * v.ny + y) * v.nx) * COUT;
+  #pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+          const int x = x0 + 16 * mb + r;
+          if (x < v.nx)
+  #pragma unroll
+            for (int j = 0; j < NBO; ++j) store4<RELU>(dst + (long)x * COUT + 16 * j + 4 * q, acc[j][mb], bl[j]);
+        }
       }
     }
     __syncthreads();                     // every wave is done with plane z-1
