@@ -526,7 +526,7 @@ __global__ __launch_bounds__(256) void k_first(const float* __restrict__ raw, co
   for (int k = 0; k < 4; ++k)
 #pragma unroll
     for (int i = 0; i < 4; ++i) ps[k][i] = 0.f;
-  float dd[4][4];  // head modes: coarse d2 of the current z pair, per y pair
+  float hd[4];  // head modes: sum_c Wh[c] d2[c] of the current z pair's coarse voxels, per y pair
 
   // head modes: after the 8 rows, lane (q, r) owns rows hrow(q, 0..1) of
   // column x (see the transposed reduction below)
@@ -543,10 +543,10 @@ __global__ __launch_bounds__(256) void k_first(const float* __restrict__ raw, co
           const long cw = (((long)b * hz + (z >> 1)) * (v.ny / 2) + (ok ? yc : 0)) * (v.nx / 2) + (ok ? (x >> 1) : 0);
           u32x2 d = *reinterpret_cast<const u32x2*>(d2 + cw * 16 + 4 * q);
           d = ok ? d : u32x2(0u);
-          dd[k][0] = lo(d[0]);
-          dd[k][1] = hi(d[0]);
-          dd[k][2] = lo(d[1]);
-          dd[k][3] = hi(d[1]);
+          hd[k] = wh[0] * lo(d[0]);
+          hd[k] += wh[1] * hi(d[0]);
+          hd[k] += wh[2] * lo(d[1]);
+          hd[k] += wh[3] * hi(d[1]);
         }
       }
 #pragma unroll
@@ -583,10 +583,11 @@ __global__ __launch_bounds__(256) void k_first(const float* __restrict__ raw, co
         for (int i = 0; i < 4; ++i) ps[my >> 1][i] += e[i];
         continue;
       }
-      float pt = 0.f;
+      // this lane's 4 channels of sum_c Wh[c] (e1[c] + d2[c])
+      float pt = wh[0] * e[0];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) pt += wh[i] * (e[i] + dd[my >> 1][i]);
-      part[my] = pt;
+      for (int i = 1; i < 4; ++i) pt += wh[i] * e[i];
+      part[my] = pt + hd[my >> 1];
     }
     if (MODE == 0) {
 #pragma unroll
