@@ -1169,7 +1169,7 @@ int grid_for(long n, int threads, int cap = 4096) {
 template <int CIN, int COUT, bool RELU, bool BIAS, bool UPADD = false, bool POOL = false>
 void conv25_launch(hipStream_t st, const bf16_t* in, const bf16_t* wf, const float* bias, bf16_t* out, Vol v,
                    const bf16_t* coarse = nullptr, bf16_t* pooled = nullptr) {
-  constexpr int TX = (CIN == 64 || COUT == 64) ? 32 : 64;
+  constexpr int TX = CIN == 64 ? 32 : 64;
   // z-chunks of 16 planes (2 extra planes of loads per chunk), fewer when the
   // grid is already large
   const long tiles = (long)v.nb * ((v.ny + 3) / 4) * ((v.nx + TX - 1) / TX);
