@@ -72,7 +72,12 @@ def _bf16(x):
     return u.view(np.float32)
 
 
-@pytest.mark.parametrize("shape", SHAPES + [(1, 64, 64, 256)], ids=lambda s: "x".join(map(str, s)))
+# several z-chunks of the blocked kernels, the last one partial (16 + 16 + 8
+# planes at full resolution, 16 + 4 at half), three 64-voxel tiles per row
+CHUNKED = (1, 40, 64, 192)
+
+
+@pytest.mark.parametrize("shape", SHAPES + [(1, 64, 64, 256), CHUNKED], ids=lambda s: "x".join(map(str, s)))
 def test_pooled_tensors_are_exact_means_of_the_stored_ones(cdn, shape):
     """p1 = pool(e1) (k_pool) and p2 = pool(e2), which the enc2 convolution
     writes from its LDS pair buffer: both are bitwise BF16(0.125 * s), s the
@@ -98,7 +103,7 @@ def test_pooled_tensors_are_exact_means_of_the_stored_ones(cdn, shape):
         assert np.array_equal(got, ref), (coarse, int((got != ref).sum()))
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("shape", SHAPES + [CHUNKED], ids=lambda s: "x".join(map(str, s)))
 def test_inference_path_matches_training_path(cdn, shape):
     """The warm-start path (pooled first layer, e1 recomputed in the head)
     against the path that stores e1: same BF16 values, other FP32 sum order."""
