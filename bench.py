@@ -94,6 +94,41 @@ def batch_48_stacks(cdn, tab, iters=10, nstreams=8):
     return out
 
 
+def stacks_256(cdn, tab, iters=10, nstreams=8):
+    """256 stacks of 64x128x128 (1 M voxels each, 268 M in all, as much as
+    the 256x1024x1024 stack): one engine per stack run one at a time, the
+    same engines round-robin over nstreams streams in one captured hipGraph,
+    and one engine holding all 256 stacks (every pass one launch over all of
+    them).  Stacks are a bead block on a constant background."""
+    shape, n = (64, 128, 128), 256
+    ks = (np.float32(0.25), np.float32(0.25), np.float32(0.515625))
+    blk = np.random.default_rng(0).uniform(0, 100, (32, 64, 64)).astype(np.float32)
+    vox = n * int(np.prod(shape))
+    out = {"stacks": n, "stack_zyx": list(shape), "voxels": vox, "iterations": iters, "streams": nstreams}
+    b = cdn.Batch([shape] * n, tab, [ks] * n, nstreams=nstreams)
+    for i in range(n):
+        b.set_raw_block(i, blk, (8, 8, 8), 20.0)
+    for mode in ("sequential", "graph"):
+        b.run(iters, mode)
+        ms = min(b.run(iters, mode) for _ in range(2))
+        out[mode] = {"ms": round(ms, 2), "Gvoxel_iter_per_s": round(vox * iters / (ms * 1e-3) / 1e9, 2)}
+    x_batch = b.get(n - 1)
+    b.close()
+    full = np.full((n,) + shape, 20.0, np.float32)
+    full[:, 8:40, 8:72, 8:72] = blk
+    m = cdn.RLEngine(shape, tab, ks, stacks=n)
+    m.set_raw(full)
+    del full
+    m.iterate(iters)
+    ms = min(m.iterate(iters)[1] for _ in range(2))
+    out["one_engine"] = {"ms": round(ms, 2), "Gvoxel_iter_per_s": round(vox * iters / (ms * 1e-3) / 1e9, 2)}
+    out["bitwise_equal_to_batch"] = bool(np.array_equal(m.get_stack(n - 1), x_batch))
+    m.close()
+    out["speedup_vs_sequential"] = round(out["sequential"]["ms"] / ms, 3)
+    out["speedup_vs_graph"] = round(out["graph"]["ms"] / ms, 3)
+    return out
+
+
 def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int):
     """FP32 oracle RL-BA (OpenMP) on the first `slab` z-planes of the stack."""
     import oracle as orc
@@ -181,7 +216,7 @@ def main():
     ap.add_argument("--cpu-slab", type=int, default=64)
     ap.add_argument("--no-e2e", action="store_true", help="skip the RL_interface end-to-end timing")
     ap.add_argument("--e2e-iters", type=int, default=10)
-    ap.add_argument("--no-batch", action="store_true", help="skip the 48-stack batch measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-stack batch and 256-stack measurements")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
     ap.add_argument("--no-pipeline", action="store_true", help="skip the 512x2048x2048 pipeline")
     args = ap.parse_args()
@@ -246,6 +281,7 @@ def main():
                "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
     batch = None if args.no_batch else batch_48_stacks(cdn, tab)
+    stacks = None if args.no_batch else stacks_256(cdn, tab)
     warm = None if args.no_warm else warm_start_eval(cdn)
     big = None if args.no_pipeline else pipeline_512x2048x2048(cdn)
 
@@ -310,6 +346,7 @@ def main():
         "passes": passes,
         "rl_interface_e2e": e2e,
         "batch_48_stacks": batch,
+        "stacks_256_of_64x128x128": stacks,
         "warm_start": warm,
         "pipeline_512x2048x2048": big,
         "setup_s": round(gen_s, 1),

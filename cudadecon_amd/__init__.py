@@ -66,6 +66,11 @@ SIGNATURES = {
                                    C.c_float, C.c_float, C.c_int]),
     "cudaDecon_rl_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int, C.c_float,
                                          C.c_float, C.c_float]),
+    "cudaDecon_rl_create_stacks": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int,
+                                                C.c_float, C.c_float, C.c_float]),
+    "cudaDecon_rl_stacks": (C.c_int, [C.c_void_p]),
+    "cudaDecon_rl_set_raw_stack": (C.c_int, [C.c_void_p, C.c_int, _f]),
+    "cudaDecon_rl_get_stack": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "cudaDecon_rl_set_raw": (C.c_int, [C.c_void_p, _f]),
     "cudaDecon_rl_set_seed": (C.c_int, [C.c_void_p, _f]),
     "cudaDecon_rl_iterate": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -331,36 +336,61 @@ def camcor(raw_u16, camparam):
 
 
 class RLEngine:
-    """RL-Biggs-Andrews on HBM-resident state (cudaDecon_rl_* handle)."""
+    """RL-Biggs-Andrews on HBM-resident state (cudaDecon_rl_* handle).  With
+    stacks > 1 the engine holds that many stacks of `shape` and iterates
+    them together (cudaDecon_rl_create_stacks): set_raw / get then take and
+    return (stacks, nz, ny, nx) arrays, iterate's factors are (n, stacks)."""
 
-    def __init__(self, shape, otf_table, kscale):
+    def __init__(self, shape, otf_table, kscale, stacks=1):
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
+        self.stacks = int(stacks)
         img = otf_image(otf_table)
         d, h, w = img.shape
         self._img = np.ascontiguousarray(img).reshape(-1)
-        self.h = lib().cudaDecon_rl_create(nx, ny, nz, self._img, w, h, d, float(kscale[0]), float(kscale[1]),
-                                           float(kscale[2]))
+        ks = (float(kscale[0]), float(kscale[1]), float(kscale[2]))
+        if self.stacks == 1:
+            self.h = lib().cudaDecon_rl_create(nx, ny, nz, self._img, w, h, d, *ks)
+        else:
+            self.h = lib().cudaDecon_rl_create_stacks(nx, ny, nz, self.stacks, self._img, w, h, d, *ks)
         if not self.h:
             raise RuntimeError(f"cudaDecon_rl_create failed: {last_error()}")
 
+    def _all(self):
+        return self.shape if self.stacks == 1 else (self.stacks,) + self.shape
+
     def set_raw(self, raw):
-        _check(lib().cudaDecon_rl_set_raw(self.h, _f32(raw)), "cudaDecon_rl_set_raw")
+        raw = _f32(raw)
+        if raw.size != int(np.prod(self._all())):
+            raise ValueError(f"set_raw: expected {self._all()} voxels")
+        _check(lib().cudaDecon_rl_set_raw(self.h, raw), "cudaDecon_rl_set_raw")
+
+    def set_raw_stack(self, s, raw):
+        raw = _f32(raw)
+        if raw.shape != self.shape:
+            raise ValueError(f"set_raw_stack: expected {self.shape}")
+        _check(lib().cudaDecon_rl_set_raw_stack(self.h, int(s), raw), "cudaDecon_rl_set_raw_stack")
 
     def set_seed(self, seed):
         _check(lib().cudaDecon_rl_set_seed(self.h, _f32(seed)), "cudaDecon_rl_set_seed")
 
     def iterate(self, n, k0=0, seeded=False):
         """Run n iterations; returns (lambdas, device ms)."""
-        lam = np.zeros(max(int(n), 1), np.float64)
+        lam = np.zeros(max(int(n), 1) * self.stacks, np.float64)
         ms = np.zeros(1, np.float64)
         _check(lib().cudaDecon_rl_iterate(self.h, int(n), int(k0), int(seeded), lam.ctypes.data, ms.ctypes.data),
                "cudaDecon_rl_iterate")
-        return lam[:n], float(ms[0])
+        lam = lam[:n * self.stacks]
+        return (lam if self.stacks == 1 else lam.reshape(n, self.stacks)), float(ms[0])
 
     def get(self):
-        out = np.empty(self.shape, np.float32)
+        out = np.empty(self._all(), np.float32)
         _check(lib().cudaDecon_rl_get(self.h, out), "cudaDecon_rl_get")
+        return out
+
+    def get_stack(self, s):
+        out = np.empty(self.shape, np.float32)
+        _check(lib().cudaDecon_rl_get_stack(self.h, int(s), out), "cudaDecon_rl_get_stack")
         return out
 
     def timing(self, enable=-1):

@@ -148,6 +148,11 @@ struct cudaDecon_rl {
   }
 };
 
+// entry points defined on one stack (the engine's other stacks would be ignored)
+static void one_stack(const cudaDecon_rl* h, const char* what) {
+  if (h && h->eng.nstacks != 1) fail(std::string(what) + ": defined for one-stack engines only");
+}
+
 extern "C" {
 
 const char* cudaDecon_last_error(void) { return g_err.c_str(); }
@@ -645,6 +650,47 @@ cudaDecon_rl* cudaDecon_rl_create(int nx, int ny, int nz, const float* otf, int 
   return out;
 }
 
+cudaDecon_rl* cudaDecon_rl_create_stacks(int nx, int ny, int nz, int nstacks, const float* otf, int otf_width,
+                                         int otf_height, int otf_depth, float kxscale, float kyscale,
+                                         float kzscale) {
+  cudaDecon_rl* out = nullptr;
+  guard([&]() -> int {
+    if (nstacks < 1) fail("cudaDecon_rl_create_stacks: nstacks < 1");
+    auto h = std::make_unique<cudaDecon_rl>();
+    h->st = make_stream();
+    OtfTable T;
+    T.set_from_image(otf, otf_width, otf_height, otf_depth, 1.f, 1.f);
+    const float ks[3] = {kxscale, kyscale, kzscale};
+    h->eng.setup(nx, ny, nz, T, ks, h->st, nstacks);
+    out = h.release();
+    return 1;
+  });
+  return out;
+}
+
+int cudaDecon_rl_stacks(cudaDecon_rl* h) { return h ? h->eng.nstacks : 0; }
+
+int cudaDecon_rl_set_raw_stack(cudaDecon_rl* h, int s, const float* raw) {
+  return guard([&]() -> int {
+    if (!h || !raw) fail("cudaDecon_rl_set_raw_stack: null argument");
+    h->eng.load_stack(s, raw, false, h->st);
+    CDN_HIP(hipStreamSynchronize(h->st));
+    h->k_next = 0;
+    return 1;
+  });
+}
+
+int cudaDecon_rl_get_stack(cudaDecon_rl* h, int s, float* X) {
+  return guard([&]() -> int {
+    if (!h || !X) fail("cudaDecon_rl_get_stack: null argument");
+    if (s < 0 || s >= h->eng.nstacks) fail("cudaDecon_rl_get_stack: stack index out of range");
+    CDN_HIP(hipMemcpyAsync(X, h->eng.result() + (size_t)h->eng.fft.nvox * s, sizeof(float) * h->eng.fft.nvox,
+                           hipMemcpyDeviceToHost, h->st));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    return 1;
+  });
+}
+
 int cudaDecon_rl_set_raw(cudaDecon_rl* h, const float* raw) {
   return guard([&]() -> int {
     if (!h || !raw) fail("cudaDecon_rl_set_raw: null argument");
@@ -683,10 +729,13 @@ int cudaDecon_rl_iterate(cudaDecon_rl* h, int n, int k0, int seeded, double* lam
     (void)hipEventDestroy(e1);
     if (elapsed_ms) *elapsed_ms = ms;
     if (lambdas) {
-      // the engine's lambda slots of this call: k < 2 carry no factor
-      std::vector<double> hl(4 * (size_t)std::max(n, 1));
-      CDN_HIP(hipMemcpy(hl.data(), h->eng.lam.p, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
-      for (int j = 0; j < n; ++j) lambdas[j] = (k0 + j >= 2) ? hl[4 * (size_t)j] : 0.0;
+      // the engine's lambda slots of this call (4 doubles per stack and
+      // iteration): k < 2 carries no factor; n x nstacks values, stack fastest
+      const size_t ns = (size_t)h->eng.nstacks;
+      std::vector<double> hl(4 * ns * (size_t)std::max(n, 1));
+      CDN_HIP(hipMemcpy(hl.data(), h->eng.lam.p, sizeof(double) * 4 * ns * (size_t)n, hipMemcpyDeviceToHost));
+      for (int j = 0; j < n; ++j)
+        for (size_t s = 0; s < ns; ++s) lambdas[j * ns + s] = (k0 + j >= 2) ? hl[4 * (j * ns + s)] : 0.0;
     }
     h->k_next = k0 + n;
     return 1;
@@ -696,7 +745,7 @@ int cudaDecon_rl_iterate(cudaDecon_rl* h, int n, int k0, int seeded, double* lam
 int cudaDecon_rl_get(cudaDecon_rl* h, float* X) {
   return guard([&]() -> int {
     if (!h || !X) fail("cudaDecon_rl_get: null argument");
-    CDN_HIP(hipMemcpyAsync(X, h->eng.result(), sizeof(float) * h->eng.fft.nvox, hipMemcpyDeviceToHost, h->st));
+    CDN_HIP(hipMemcpyAsync(X, h->eng.result(), sizeof(float) * h->eng.total_vox(), hipMemcpyDeviceToHost, h->st));
     CDN_HIP(hipStreamSynchronize(h->st));
     return 1;
   });
@@ -753,6 +802,7 @@ static void block_copy(cudaDecon_rl* h, float* dev, const float* host_in, float*
 int cudaDecon_rl_set_raw_block(cudaDecon_rl* h, const float* block, int bx, int by, int bz, int ox, int oy, int oz,
                                float background) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_set_raw_block");
     if (!h || !block) fail("cudaDecon_rl_set_raw_block: null argument");
     fill(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, background);
     block_copy(h, h->eng.raw.as<float>(), block, nullptr, bx, by, bz, ox, oy, oz);
@@ -765,6 +815,7 @@ int cudaDecon_rl_set_raw_block(cudaDecon_rl* h, const float* block, int bx, int 
 
 int cudaDecon_rl_get_block(cudaDecon_rl* h, float* out, int bx, int by, int bz, int ox, int oy, int oz) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_get_block");
     if (!h || !out) fail("cudaDecon_rl_get_block: null argument");
     block_copy(h, h->eng.result(), nullptr, out, bx, by, bz, ox, oy, oz);
     return 1;
@@ -773,6 +824,7 @@ int cudaDecon_rl_get_block(cudaDecon_rl* h, float* out, int bx, int by, int bz, 
 
 int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_stats");
     if (!h || !out6) fail("cudaDecon_rl_stats: null argument");
     h->red.ensure(sizeof(double) * 3 * 1024);
     stats(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, h->red.as<double>(), out6);
@@ -783,6 +835,7 @@ int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6) {
 
 int cudaDecon_rl_set_raw_u16(cudaDecon_rl* h, const unsigned short* raw, float offset) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_set_raw_u16");
     if (!h || !raw) fail("cudaDecon_rl_set_raw_u16: null argument");
     const Fft3& F = h->eng.fft;
     h->u16.ensure(sizeof(uint16_t) * (size_t)F.nvox);
@@ -800,6 +853,7 @@ int cudaDecon_rl_set_raw_u16(cudaDecon_rl* h, const unsigned short* raw, float o
 int cudaDecon_rl_deskew(cudaDecon_rl* h, float dz, float dr, float deskewAngle, int outputWidth, int extraShift,
                         float padVal, int* nx_out) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_deskew");
     if (!h) fail("cudaDecon_rl_deskew: null handle");
     const Fft3& F = h->eng.fft;
     double factor = 0.;
@@ -827,6 +881,7 @@ int cudaDecon_rl_deskew(cudaDecon_rl* h, float dz, float dr, float deskewAngle, 
 
 int cudaDecon_rl_affine(cudaDecon_rl* h, const float* affMat) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_affine");
     if (!h || !affMat) fail("cudaDecon_rl_affine: null argument");
     const Fft3& F = h->eng.fft;
     const float* in = h->post_cur >= 0 ? h->post[h->post_cur].as<float>() : h->eng.result();
@@ -874,6 +929,7 @@ int cudaDecon_rl_post_stats(cudaDecon_rl* h, double* out3) {
 
 int cudaDecon_rl_snapshot(cudaDecon_rl* h) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_snapshot");
     if (!h) fail("cudaDecon_rl_snapshot: null handle");
     const size_t b = sizeof(float) * (size_t)h->eng.fft.nvox;
     h->ref.ensure(b);
@@ -885,6 +941,7 @@ int cudaDecon_rl_snapshot(cudaDecon_rl* h) {
 
 int cudaDecon_rl_distance(cudaDecon_rl* h, int which, double* rel) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_distance");
     if (!h || !rel || which < 0 || which > 1) fail("cudaDecon_rl_distance: bad argument");
     if (!h->ref.p) fail("cudaDecon_rl_distance: no snapshot taken");
     h->red.ensure(sizeof(double) * 3 * 1024);
@@ -1124,6 +1181,7 @@ void cudaDecon_net_destroy(cudaDecon_net* h) { delete h; }
 
 int cudaDecon_rl_warm_start(cudaDecon_rl* h, cudaDecon_net* net, double* elapsed_ms) {
   return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_warm_start");
     if (!h || !net) fail("cudaDecon_rl_warm_start: null argument");
     const Fft3& F = h->eng.fft;
     WarmNet& N = net->net;

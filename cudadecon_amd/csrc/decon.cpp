@@ -155,26 +155,39 @@ OtfParams OtfDev::params(const OtfTable& t, double inv_n) const {
 // ---------------------------------------------------------------------------
 // RL engine
 // ---------------------------------------------------------------------------
-void RLEngine::setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st) {
+// Stacks can share every pass only if no x-update workgroup spans two of
+// them (each stack's dot products then come from its own workgroups, in the
+// order a one-stack engine sums them).
+static void check_stack_rows(int nx, long stack_rows, int nstacks) {
+  if (nstacks > 1 && stack_rows % x_update_rows_per_block(nx) != 0)
+    fail("rows per stack (ny nz = " + std::to_string(stack_rows) + ") must be a multiple of " +
+         std::to_string(x_update_rows_per_block(nx)) + " for several stacks in one engine");
+}
+
+void RLEngine::setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st,
+                     int nst) {
+  if (nst < 1) fail("an engine needs at least one stack");
+  check_stack_rows(nx, (long)ny * nz, nst);
   if (!fft.matches(nx, ny, nz)) fft.setup(nx, ny, nz, st);
+  nstacks = nst;
   table = t;
   otf.setup(table, kscale, nz, st);
-  const size_t vb = sizeof(float) * (size_t)fft.nvox;
+  const size_t vb = sizeof(float) * (size_t)fft.nvox * nstacks;
   raw.ensure(vb);
   xa.ensure(vb);
   xb.ensure(vb);
   y.ensure(vb);
   g.ensure(vb);
-  spec.ensure(sizeof(float2) * (size_t)fft.nspec);
+  spec.ensure(sizeof(float2) * (size_t)fft.nspec * nstacks);
   nb_part = x_update_blocks(nx, (long)ny * nz);
   // room for any x-pass launch shape the autotuner may install later
-  part.ensure(sizeof(double) * 2 * (size_t)x_update_blocks_max((long)ny * nz));
+  part.ensure(sizeof(double) * 2 * (size_t)x_update_blocks_max((long)ny * nz) * nstacks);
   cur = xa.as<float>();
   prev = xb.as<float>();
 }
 
 void RLEngine::load_raw(const float* in, bool on_device, hipStream_t st) {
-  const size_t vb = sizeof(float) * (size_t)fft.nvox;
+  const size_t vb = sizeof(float) * (size_t)total_vox();
   const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   CDN_HIP(hipMemcpyAsync(raw.p, in, vb, k, st));
   CDN_HIP(hipMemcpyAsync(xa.p, raw.p, vb, hipMemcpyDeviceToDevice, st));
@@ -182,25 +195,34 @@ void RLEngine::load_raw(const float* in, bool on_device, hipStream_t st) {
   prev = xb.as<float>();
 }
 
+void RLEngine::load_stack(int s, const float* in, bool on_device, hipStream_t st) {
+  if (s < 0 || s >= nstacks) fail("stack index out of range");
+  const size_t vb = sizeof(float) * (size_t)fft.nvox, off = (size_t)fft.nvox * s;
+  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  CDN_HIP(hipMemcpyAsync(raw.as<float>() + off, in, vb, k, st));
+  CDN_HIP(hipMemcpyAsync(cur + off, raw.as<float>() + off, vb, hipMemcpyDeviceToDevice, st));  // its X_0
+}
+
 void RLEngine::reset(hipStream_t st) {
-  CDN_HIP(hipMemcpyAsync(xa.p, raw.p, sizeof(float) * (size_t)fft.nvox, hipMemcpyDeviceToDevice, st));
+  CDN_HIP(hipMemcpyAsync(xa.p, raw.p, sizeof(float) * (size_t)total_vox(), hipMemcpyDeviceToDevice, st));
   cur = xa.as<float>();
   prev = xb.as<float>();
 }
 
-void RLEngine::prepare(int n) { lam.ensure(sizeof(double) * 4 * (size_t)std::max(n, 1)); }
+void RLEngine::prepare(int n) { lam.ensure(sizeof(double) * 4 * (size_t)nstacks * (size_t)std::max(n, 1)); }
 
 void RLEngine::load_seed(const float* in, bool on_device, hipStream_t st) {
-  const size_t vb = sizeof(float) * (size_t)fft.nvox;
+  const size_t vb = sizeof(float) * (size_t)total_vox();
   CDN_HIP(hipMemcpyAsync(y.p, in, vb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
 }
 
 void RLEngine::iterate(int n, int k0, bool seeded, hipStream_t st, double* lambdas) {
   if (n <= 0) return;
-  const long rows = (long)fft.ny * fft.nz;
+  const long stack_rows = (long)fft.ny * fft.nz, rows = stack_rows * nstacks;
+  check_stack_rows(fft.nx, stack_rows, nstacks);  // an autotuned x shape may have changed the blocks
   const float eps = FLT_EPSILON;  // src/RL-Biggs-Andrews.cpp:274
   const OtfParams op = otf.params(table, 1.0 / (double)fft.nvox);
-  lam.ensure(sizeof(double) * 4 * (size_t)n);
+  lam.ensure(sizeof(double) * 4 * (size_t)nstacks * (size_t)n);
   float2* S = spec.as<float2>();
   const float2* twx = fft.twx.as<float2>();
   const float2* twz = fft.twz.as<float2>();
@@ -219,11 +241,11 @@ void RLEngine::iterate(int n, int k0, bool seeded, hipStream_t st, double* lambd
   mark(-1);
   for (int j = 0; j < n; ++j) {
     const int k = k0 + j;
-    double* lamk = lamd + 4 * j;
+    double* lamk = lamd + 4 * (size_t)nstacks * j;
     int mode;
     const float* yv;
     if (k >= 2) {
-      accel_factor(st, part.as<double>(), nb_part, (double)eps, lamk);
+      accel_factor(st, part.as<double>(), nb_part, (double)eps, lamk, nstacks);
       mark(PASS_LAMBDA);
       mode = XIN_PRED;
       yv = y.as<float>();
@@ -234,14 +256,18 @@ void RLEngine::iterate(int n, int k0, bool seeded, hipStream_t st, double* lambd
       mode = XIN_PLAIN;
       yv = cur;
     }
-    launch_x_r2c(fft.nx, st, cur, prev, y.as<float>(), lamk, mode, S, rows, fft.pitch, twx);
+    launch_x_r2c(fft.nx, st, cur, prev, y.as<float>(), lamk, mode, S, rows, fft.pitch, twx, stack_rows);
     mark(PASS_XR2C);
     for (int f = 0; f < 2; ++f) {
-      fft.col(st, S, 1, false);
+      // the y transforms see the stacks as nz nstacks slabs
+      launch_col_fft(fft.ny, 0, st, S, fft.nxh, fft.pitch, (long)fft.ny * fft.pitch, (long)fft.nz * nstacks,
+                     fft.twy.as<float2>());
       mark(PASS_YFWD);
-      launch_z_filter(fft.nz, st, S, fft.nxh, (long)fft.ny * fft.pitch, fft.pitch, fft.ny, f, op, twz);
+      launch_z_filter(fft.nz, st, S, fft.nxh, (long)fft.ny * fft.pitch, fft.pitch, fft.ny, f, op, twz, nstacks,
+                      fft.nspec);
       mark(PASS_ZFILT);
-      fft.col(st, S, 1, true);
+      launch_col_fft(fft.ny, 1, st, S, fft.nxh, fft.pitch, (long)fft.ny * fft.pitch, (long)fft.nz * nstacks,
+                     fft.twy.as<float2>());
       mark(PASS_YINV);
       if (f == 0) {
         launch_x_lrcore(fft.nx, st, S, raw.as<float>(), rows, fft.pitch, twx, eps);
@@ -249,7 +275,8 @@ void RLEngine::iterate(int n, int k0, bool seeded, hipStream_t st, double* lambd
       }
     }
     nb_part = launch_x_update(fft.nx, st, S, yv, prev, g.as<float>(), part.as<double>(), k >= 1, rows, fft.pitch,
-                              twx);
+                              twx) /
+              nstacks;
     mark(PASS_XUPD);
     std::swap(cur, prev);
   }
@@ -264,9 +291,11 @@ void RLEngine::iterate(int n, int k0, bool seeded, hipStream_t st, double* lambd
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   }
   if (lambdas) {
-    std::vector<double> h(4 * (size_t)n);
+    std::vector<double> h(4 * (size_t)nstacks * n);
     CDN_HIP(hipMemcpy(h.data(), lam.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-    for (int j = 0; j < n; ++j) lambdas[j] = (k0 + j >= 2) ? h[4 * (size_t)j] : 0.0;
+    for (int j = 0; j < n; ++j)
+      for (int s = 0; s < nstacks; ++s)
+        lambdas[(size_t)j * nstacks + s] = (k0 + j >= 2) ? h[4 * ((size_t)j * nstacks + s)] : 0.0;
   }
 }
 

@@ -57,7 +57,11 @@ struct OtfDev {
 enum Pass { PASS_XR2C = 0, PASS_YFWD, PASS_ZFILT, PASS_YINV, PASS_XLR, PASS_XUPD, PASS_LAMBDA, PASS_COUNT };
 
 // RL-BA engine: all state resident in HBM; the acceleration factor stays on
-// the device so an iteration issues no host synchronisation.
+// the device so an iteration issues no host synchronisation.  It holds
+// `nstacks` stacks of one shape back to back (stack s at s nvox in the real
+// volumes, s nspec in the spectrum) and runs every pass once per iteration
+// over all of them; each stack keeps its own acceleration factor, and its
+// results are those of a one-stack engine.
 struct RLEngine {
   Fft3 fft;
   OtfDev otf;
@@ -65,16 +69,21 @@ struct RLEngine {
   DevBuf raw, xa, xb, y, g, spec, part, lam;
   float* cur = nullptr;   // X_k after the last iteration
   float* prev = nullptr;  // X_{k-1}
-  int nb_part = 0;
+  int nstacks = 1;
+  int nb_part = 0;        // dot-product partials per stack
   // per-pass timing (ms, summed) when enabled
   bool timing = false;
   double pass_ms[PASS_COUNT] = {};
   long pass_calls[PASS_COUNT] = {};
 
-  void setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st);
-  // Stage the (preprocessed) stack: R := in, X_0 := in.  `in` is a device
-  // pointer when on_device, else host.
+  void setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st, int nstacks = 1);
+  long stack_vox() const { return fft.nvox; }
+  long total_vox() const { return fft.nvox * nstacks; }
+  // Stage the (preprocessed) stacks: R := in, X_0 := in (all stacks, stack
+  // after stack).  `in` is a device pointer when on_device, else host.
   void load_raw(const float* in, bool on_device, hipStream_t st);
+  // stage stack s only
+  void load_stack(int s, const float* in, bool on_device, hipStream_t st);
   void load_seed(const float* in, bool on_device, hipStream_t st);
   // X_0 := R again (restart from the staged stack)
   void reset(hipStream_t st);
@@ -83,8 +92,8 @@ struct RLEngine {
   void prepare(int n);
   // Run iterations k = k0 .. k0+n-1 continuing the resident state (k0 > 0
   // requires the previous call to have ended at k0-1); seeded => Y_0 from the
-  // seed buffer.  lambdas (optional, host, n entries) receives the factor
-  // used at each k (0 for k < 2).
+  // seed buffer.  lambdas (optional, host, n x nstacks entries, stack
+  // fastest) receives the factor used at each k (0 for k < 2).
   void iterate(int n, int k0, bool seeded, hipStream_t st, double* lambdas);
   float* result() const { return cur; }
 };

@@ -104,19 +104,21 @@ void fft_clear_variants() {
   hipLaunchKernelGGL(kernel<N>, dim3(grid), dim3(block), 0, st, __VA_ARGS__)
 
 void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam, int mode,
-                  float2* S, long rows, long pitch, const float2* tw) {
+                  float2* S, long rows, long pitch, const float2* tw, long stack_rows) {
   FftVariant tv;
   const bool tuned = fft_get_variant(nx, &tv);
   const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
-  void* args[] = {&X, &Xp, &Y, &lam, &mode, &S, &rows, &pitch, &tw};
+  if (stack_rows <= 0) stack_rows = rows;
+  if (stack_rows != rows && stack_rows % 2) fail("launch_x_r2c: rows per stack must be even");
+  void* args[] = {&X, &Xp, &Y, &lam, &mode, &S, &rows, &pitch, &stack_rows, &tw};
   if (tuned) {
     jit_launch(nx, JK_X_R2C, grid, xs.THREADS, st, args, tv);
   } else {
     switch (nx) {
 #define CDN_CASE(N)                                                                     \
   case N:                                                                               \
-    CDN_LAUNCH(k_x_r2c, N, grid, xs.THREADS, X, Xp, Y, lam, mode, S, rows, pitch, tw); \
+    CDN_LAUNCH(k_x_r2c, N, grid, xs.THREADS, X, Xp, Y, lam, mode, S, rows, pitch, stack_rows, tw); \
     break;
       CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
@@ -154,7 +156,7 @@ void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long r
                      float eps) {
   FftVariant tv;
   const bool tuned = fft_get_variant(nx, &tv);
-  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1, true);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
   void* args[] = {&S, &raw, &rows, &pitch, &tw, &eps};
   if (tuned) {
@@ -182,6 +184,12 @@ int x_update_blocks(int nx, long rows) {
 }
 
 int x_update_blocks_max(long rows) { return (int)((rows + 1) / 2); }
+
+long x_update_rows_per_block(int nx) {
+  FftVariant tv;
+  const bool tuned = fft_get_variant(nx, &tv);
+  return 2L * x_shape(nx, tuned ? tv.xthreads : -1).NL;
+}
 
 int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, float* Xout, float* G, double* part,
                     int dots, long rows, long pitch, const float2* tw) {
@@ -239,22 +247,23 @@ void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols,
 }
 
 void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride, long ny,
-                     int conj, const OtfParams& otf, const float2* tw) {
+                     int conj, const OtfParams& otf, const float2* tw, int nstacks, long stack_stride) {
   FftVariant tv;
   const bool tuned = fft_get_variant(nz, &tv);
   const CShape cs = c_shape(nz, tuned ? tv.cols : -1);
   long ntiles = (ncols + cs.L - 1) / cs.L;
-  const unsigned grid = grid1(ntiles * ny);
+  if (nstacks < 1) fail("launch_z_filter: no stacks");
+  const unsigned grid = grid1(ntiles * ny * nstacks);
   OtfParams op = otf;
-  void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &ny, &conj, &op, &tw};
+  void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &ny, &stack_stride, &conj, &op, &tw};
   if (tuned) {
     jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args, tv);
   } else {
     switch (nz) {
 #define CDN_CASE(N)                                                                                      \
   case N:                                                                                                \
-    CDN_LAUNCH(k_z_filter, N, grid, cs.THREADS, S, ncols, line_stride, outer_stride, ntiles, ny, conj,  \
-               op, tw);                                                                                  \
+    CDN_LAUNCH(k_z_filter, N, grid, cs.THREADS, S, ncols, line_stride, outer_stride, ntiles, ny,         \
+               stack_stride, conj, op, tw);                                                              \
     break;
       CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE

@@ -25,9 +25,11 @@ void fft_prepare_length(int n);
 int fft_twiddle_count(int n);
 void fft_twiddles(int n, float2* out);
 
-// R2C along x of row pairs with the prediction fused in; writes S.
+// R2C along x of row pairs with the prediction fused in; writes S.  Rows
+// come in stacks of stack_rows (even; 0: one stack), stack s using the
+// factor lam[4 s].
 void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam,
-                  int mode, float2* S, long rows, long pitch, const float2* tw);
+                  int mode, float2* S, long rows, long pitch, const float2* tw, long stack_rows = 0);
 // plain C2R along x (ignores imag of kx = 0 and kx = nx/2).
 void launch_x_c2r(int nx, hipStream_t st, const float2* S, float* out, long rows, long pitch,
                   const float2* tw);
@@ -40,6 +42,8 @@ void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long r
 int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, float* Xout, float* G,
                     double* part, int dots, long rows, long pitch, const float2* tw);
 int x_update_blocks(int nx, long rows);
+// rows per x-update workgroup of the launch shape in use for nx
+long x_update_rows_per_block(int nx);
 // upper bound of x_update_blocks over every launch shape (one row pair per block)
 int x_update_blocks_max(long rows);
 
@@ -53,8 +57,10 @@ void fft_clear_variants();
 // line_stride, for columns [0, ncols) of each of n_outer slabs.
 void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols, long line_stride,
                     long outer_stride, long n_outer, const float2* tw);
-// z forward -> x H (or conj H) / N -> z inverse; slab index = ky.
+// z forward -> x H (or conj H) / N -> z inverse; slab index = ky, for
+// nstacks spectra stack_stride complex elements apart.
 void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride,
-                     long ny, int conj, const OtfParams& otf, const float2* tw);
+                     long ny, int conj, const OtfParams& otf, const float2* tw, int nstacks = 1,
+                     long stack_stride = 0);
 
 }  // namespace cdn

@@ -90,10 +90,17 @@ constexpr int kXTptOverride[] = {CDN_XTPT_OVERRIDE};
 
 // x passes: lines of up to 64 threads split/merge the packed row pair with
 // wavefront shuffles; longer lines (N >= 2048, measured faster with 16
-// elements per thread) use an LDS exchange
-constexpr int tpt_x(int N) {
+// elements per thread) use an LDS exchange.  Short power-of-two lines get
+// at least 32 threads, so that a wavefront's row loads and stores cover
+// whole 128-byte lines instead of 32-byte pieces of 8 rows (2-3x faster x
+// passes at 64, 128, 256 on stacks of those widths); the LRcore pass (lr),
+// two transforms per line, measured best at 16.
+constexpr int tpt_x(int N, bool lr = false) {
   if (lookup_override(kXTptOverride, N) > 0) return lookup_override(kXTptOverride, N);
-  return tpt_col(N);
+  int t = tpt_col(N);
+  const int want = lr ? 16 : 32;
+  if (N >= 64 && (N & (N - 1)) == 0 && t < want) t = want;
+  return t;
 }
 
 // columns per tile for y/z passes: fill the LDS budget, at most 64 columns
@@ -117,9 +124,9 @@ struct XShape {
   bool SHUFFLE;
 };
 // xthreads > 0 forces the workgroup size (the build-time override otherwise)
-constexpr XShape x_shape(int N, int xthreads = -1) {
+constexpr XShape x_shape(int N, int xthreads = -1, bool lr = false) {
   XShape s{};
-  s.TPT = tpt_x(N);
+  s.TPT = tpt_x(N, lr);
   s.E = N / s.TPT;
   s.SHUFFLE = plan_exact(N, s.TPT) && s.TPT <= 64 && (64 % s.TPT) == 0 && (s.E % 2) == 0;
   const int ov = xthreads >= 0 ? xthreads : lookup_override(kXThreadsOverride, N);
@@ -144,8 +151,8 @@ constexpr CShape c_shape(int N, int cols = -1) {
   return s;
 }
 
-template <int N> struct XCfg {
-  static constexpr XShape S = x_shape(N);
+template <int N, bool LR = false> struct XCfg {
+  static constexpr XShape S = x_shape(N, -1, LR);
   static constexpr int TPT = S.TPT, E = S.E, NL = S.NL, PAD = S.PAD, LS = S.LS, THREADS = S.THREADS, EH = S.EH;
   static constexpr bool SHUFFLE = S.SHUFFLE;
 };
@@ -178,10 +185,10 @@ __device__ __forceinline__ float2 shfl2(float2 v, int src) {
 
 // Half spectra A (row r0) and B (row r0+1) from the packed transform Z:
 // A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
-template <int N>
-__device__ __forceinline__ void split_store(float2 (&v)[XCfg<N>::E], float2* line, int p, float2* __restrict__ S,
-                                            long r0, long pitch, bool v0, bool v1) {
-  using C = XCfg<N>;
+template <int N, bool LR = false>
+__device__ __forceinline__ void split_store(float2 (&v)[(XCfg<N, LR>::E)], float2* line, int p,
+                                            float2* __restrict__ S, long r0, long pitch, bool v0, bool v1) {
+  using C = XCfg<N, LR>;
   constexpr int E = C::E, TPT = C::TPT;
   auto put = [&](int k, float2 zk, float2 zn) {
     const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
@@ -214,10 +221,10 @@ __device__ __forceinline__ void split_store(float2 (&v)[XCfg<N>::E], float2* lin
 
 // Packed line Z = A + i B (Hermitian extension of both rows; imaginary parts
 // of the kx = 0 and kx = N/2 bins ignored) into v.
-template <int N>
-__device__ __forceinline__ void merge_load(float2 (&v)[XCfg<N>::E], float2* line, int p, const float2* __restrict__ S,
-                                           long r0, long pitch, bool v0, bool v1) {
-  using C = XCfg<N>;
+template <int N, bool LR = false>
+__device__ __forceinline__ void merge_load(float2 (&v)[(XCfg<N, LR>::E)], float2* line, int p,
+                                           const float2* __restrict__ S, long r0, long pitch, bool v0, bool v1) {
+  using C = XCfg<N, LR>;
   constexpr int E = C::E, TPT = C::TPT;
   auto ld = [&](int k, float2& A, float2& B) {
     A = v0 ? ld_stream<NTX>(S + r0 * pitch + k) : make_float2(0.f, 0.f);
@@ -273,7 +280,7 @@ template <int N>
 __global__ void __launch_bounds__(XCfg<N>::THREADS)
     k_x_r2c(const float* __restrict__ X, const float* __restrict__ Xp, float* __restrict__ Y,
             const double* __restrict__ lam, int mode, float2* __restrict__ S, long rows, long pitch,
-            const float2* __restrict__ tw) {
+            long stack_rows, const float2* __restrict__ tw) {
   using C = XCfg<N>;
   __shared__ float2 lds[C::NL * C::LS];
   const int tid = threadIdx.x;
@@ -281,7 +288,8 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
-  const float lamf = (mode == XIN_PRED) ? (float)lam[0] : 0.f;
+  // the factor of this row pair's stack (stack_rows rows per stack, even)
+  const float lamf = (mode == XIN_PRED) ? (float)lam[4 * (v0 ? r0 / stack_rows : 0)] : 0.f;
   float2 v[C::E];
   // all loads are issued before any store
   const float* src = (mode == XIN_SEED) ? Y : X;
@@ -351,10 +359,10 @@ __device__ __forceinline__ float lrcore(float c, float raw, float eps) {
 }
 
 template <int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
+__global__ void __launch_bounds__((XCfg<N, true>::THREADS))
     k_x_lrcore(float2* __restrict__ S, const float* __restrict__ raw, long rows, long pitch,
                const float2* __restrict__ tw, float eps) {
-  using C = XCfg<N>;
+  using C = XCfg<N, true>;
   __shared__ float2 lds[C::NL * C::LS];
   const int tid = threadIdx.x;
   const int l = tid / C::TPT, p = tid % C::TPT;
@@ -362,7 +370,7 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
   float2 v[C::E];
-  merge_load<N>(v, line, p, S, r0, pitch, v0, v1);
+  merge_load<N, true>(v, line, p, S, r0, pitch, v0, v1);
   fft_line<N, C::TPT, 1, true>(v, line, p, tw);
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
@@ -374,7 +382,7 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   __syncthreads();
   fft_line<N, C::TPT, 1, false>(v, line, p, tw);
   __syncthreads();
-  split_store<N>(v, line, p, S, r0, pitch, v0, v1);
+  split_store<N, true>(v, line, p, S, r0, pitch, v0, v1);
 }
 
 // Deterministic workgroup sum of two doubles; out[0], out[1] written by
@@ -520,16 +528,18 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
 template <int N>
 __global__ void __launch_bounds__(CCfg<N>::THREADS)
     k_z_filter(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles, long ny,
-               int conj, OtfParams otf, const float2* __restrict__ tw) {
+               long stack_stride, int conj, OtfParams otf, const float2* __restrict__ tw) {
   using C = CCfg<N>;
   __shared__ float2 lds[N * C::L];
   const int tid = threadIdx.x;
   const int l = tid % C::L, p = tid / C::L;
   const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
-  const long outer = lb / ntiles, tile = lb - outer * ntiles;
+  // slabs are (stack, ky) pairs, ky fastest
+  const long slab = lb / ntiles, tile = lb - slab * ntiles;
+  const long stack = slab / ny, outer = slab - stack * ny;
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
-  float2* base = S + outer * outer_stride + c;
+  float2* base = S + stack * stack_stride + outer * outer_stride + c;
   float2 v[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m)

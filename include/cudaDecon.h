@@ -143,23 +143,42 @@ int cudaDecon_fft_variant(int n, int* cols, int* xthreads, int* col_nt_load);
 typedef struct cudaDecon_rl cudaDecon_rl;
 cudaDecon_rl* cudaDecon_rl_create(int nx, int ny, int nz, const float* otf, int otf_width, int otf_height,
                                   int otf_depth, float kxscale, float kyscale, float kzscale);
-/* preprocessed stack (also X_0), host memory */
+/* One engine over nstacks stacks of one shape (tiles, patches, time
+ * points): stored back to back, every pass of an iteration is one launch
+ * over all of them, each stack keeping its own acceleration factor; each
+ * stack's result equals a one-stack engine's.  ny nz must be a multiple of
+ * the x passes' rows per workgroup (the call fails naming it otherwise).
+ * No counterpart in the reference (its library holds one stack). */
+cudaDecon_rl* cudaDecon_rl_create_stacks(int nx, int ny, int nz, int nstacks, const float* otf, int otf_width,
+                                         int otf_height, int otf_depth, float kxscale, float kyscale,
+                                         float kzscale);
+int cudaDecon_rl_stacks(cudaDecon_rl* h);
+/* stage / read stack s alone (host memory, nx ny nz floats) */
+int cudaDecon_rl_set_raw_stack(cudaDecon_rl* h, int s, const float* raw);
+int cudaDecon_rl_get_stack(cudaDecon_rl* h, int s, float* X);
+/* preprocessed stack (also X_0), host memory; all stacks, one after another */
 int cudaDecon_rl_set_raw(cudaDecon_rl* h, const float* raw);
 /* seed for Y_0 (the FlatStart hook, src/RL-Biggs-Andrews.cpp:305-316) */
 int cudaDecon_rl_set_seed(cudaDecon_rl* h, const float* seed);
-/* iterations k0 .. k0+n-1; lambdas (n, nullable) receives the factor used
- * at each k; elapsed_ms (nullable) the device time of the n iterations. */
+/* iterations k0 .. k0+n-1 of every stack; lambdas (n x nstacks, stack
+ * fastest, nullable) receives the factor used at each k; elapsed_ms
+ * (nullable) the device time of the n iterations. */
 int cudaDecon_rl_iterate(cudaDecon_rl* h, int n, int k0, int seeded, double* lambdas, double* elapsed_ms);
+/* X of all stacks, one after another */
 int cudaDecon_rl_get(cudaDecon_rl* h, float* X);
 /* per-pass device time: enable != 0 turns event timing on (and clears the
  * sums); pass_ms / pass_calls (7 entries: x-R2C, y-fwd, z-filter, y-inv,
  * x-LRcore, x-update, lambda) receive the sums so far (nullable). */
 int cudaDecon_rl_timing(cudaDecon_rl* h, int enable, double* pass_ms, long* pass_calls);
-/* bytes of one half-spectrum row pitch (complex elements) and volume sizes */
+/* bytes of one half-spectrum row pitch (complex elements) and volume sizes
+ * (of one stack) */
 int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pitch);
 /* Stage a sparse stack without a full-volume transfer: every voxel is set
  * to `background`, then the host block (bz, by, bx) is placed at offset
- * (oz, oy, ox); X_0 := the stack.  get_block reads a block of X back. */
+ * (oz, oy, ox); X_0 := the stack.  get_block reads a block of X back.
+ * These, stats, the pipeline below (set_raw_u16, deskew, affine),
+ * snapshot / distance and warm_start are defined for one-stack engines and
+ * fail on others. */
 int cudaDecon_rl_set_raw_block(cudaDecon_rl* h, const float* block, int bx, int by, int bz, int ox, int oy, int oz,
                                float background);
 int cudaDecon_rl_get_block(cudaDecon_rl* h, float* out, int bx, int by, int bz, int ox, int oy, int oz);
