@@ -54,6 +54,8 @@ void Fft3::setup(int x, int y, int z, hipStream_t st) {
   upload_twiddles(twx, nx, st);
   upload_twiddles(twy, ny, st);
   upload_twiddles(twz, nz, st);
+  fft_prepare_cols(ny, nxh);
+  fft_prepare_cols(nz, nxh);
 }
 
 void Fft3::col(hipStream_t st, float2* S, int axis, bool inverse) const {

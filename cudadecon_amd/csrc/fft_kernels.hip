@@ -216,10 +216,41 @@ int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, flo
   return nb;
 }
 
+// Column tiles: when a slab's columns need two built-in tiles, the second
+// one a quarter or less full (65 columns of a 128 x 128 plane in tiles of
+// 64), one tile holding all of them instead, if it fits (threads <= 1024,
+// LDS within the built-in budget class): a run-time compiled variant,
+// bitwise the same results.  (Measured: 8% on stacks of 64 x 128 x 128; a
+// wider single tile, 129 columns for 3 built-in tiles, was slower.)
+bool fft_balanced_cols(int len, long ncols, FftVariant* v) {
+  const CShape d = c_shape(len);
+  if (ncols <= d.L || ncols >= 2 * d.L || 4 * (ncols - d.L) > d.L) return false;
+  const long budget = len >= 768 ? 160L * 1024 : 80L * 1024;
+  if (ncols * c_shape(len, 1).TPT > 1024 || (long)len * ncols * 8 > budget) return false;
+  if (v) {
+    *v = FftVariant();
+    v->cols = (int)ncols;
+  }
+  return true;
+}
+
+// tuned variant of the length, else one tile of all columns, else built in
+static bool col_variant(int len, long ncols, FftVariant* v) {
+  if (fft_get_variant(len, v)) return true;
+  return fft_balanced_cols(len, ncols, v);
+}
+
+void fft_prepare_cols(int len, long ncols) {
+  FftVariant v;
+  if (fft_get_variant(len, &v) || !fft_balanced_cols(len, ncols, &v)) return;
+  // compile and load (grid 0: no launch) before any stream capture
+  for (JitKernel k : {JK_COL_F, JK_COL_I, JK_Z_FILTER}) jit_launch(len, k, 0, 1, nullptr, nullptr, v);
+}
+
 void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols, long line_stride,
                     long outer_stride, long n_outer, const float2* tw) {
   FftVariant tv;
-  const bool tuned = fft_get_variant(len, &tv);
+  const bool tuned = col_variant(len, ncols, &tv);
   const CShape cs = c_shape(len, tuned ? tv.cols : -1);
   long ntiles = (ncols + cs.L - 1) / cs.L;
   const unsigned grid = grid1(ntiles * n_outer);
@@ -249,7 +280,7 @@ void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols,
 void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride, long ny,
                      int conj, const OtfParams& otf, const float2* tw, int nstacks, long stack_stride) {
   FftVariant tv;
-  const bool tuned = fft_get_variant(nz, &tv);
+  const bool tuned = col_variant(nz, ncols, &tv);
   const CShape cs = c_shape(nz, tuned ? tv.cols : -1);
   long ntiles = (ncols + cs.L - 1) / cs.L;
   if (nstacks < 1) fail("launch_z_filter: no stacks");

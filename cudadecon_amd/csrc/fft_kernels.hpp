@@ -50,6 +50,13 @@ int x_update_blocks_max(long rows);
 // Tuned launch shapes: a length with a variant runs that variant's run-time
 // compiled kernels (results are bitwise those of the default shapes).
 void fft_set_variant(int n, const FftVariant& v);
+// One column tile of all `ncols` columns of length `len` when they would
+// need two built-in tiles, the second a quarter full or less, and the tile
+// fits (v: the variant); the column launchers use it unless the length has
+// a tuned variant.
+bool fft_balanced_cols(int len, long ncols, FftVariant* v);
+// compile and load the kernels those launches will need (no GPU work)
+void fft_prepare_cols(int len, long ncols);
 bool fft_get_variant(int n, FftVariant* v);
 void fft_clear_variants();
 
