@@ -1181,11 +1181,10 @@ void cudaDecon_net_destroy(cudaDecon_net* h) { delete h; }
 
 int cudaDecon_rl_warm_start(cudaDecon_rl* h, cudaDecon_net* net, double* elapsed_ms) {
   return guard([&]() -> int {
-    one_stack(h, "cudaDecon_rl_warm_start");
     if (!h || !net) fail("cudaDecon_rl_warm_start: null argument");
     const Fft3& F = h->eng.fft;
     WarmNet& N = net->net;
-    N.geometry(1, F.nz, F.ny, F.nx, false);
+    N.geometry(h->eng.nstacks, F.nz, F.ny, F.nx, false);  // one sample per stack, each normalised by its mean
     CDN_HIP(hipStreamSynchronize(net->st));  // parameter uploads of the network's stream
     hipEvent_t e0, e1;
     CDN_HIP(hipEventCreate(&e0));

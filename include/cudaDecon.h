@@ -176,9 +176,9 @@ int cudaDecon_rl_geometry(cudaDecon_rl* h, long* nvox, long* nspec_alg, long* pi
 /* Stage a sparse stack without a full-volume transfer: every voxel is set
  * to `background`, then the host block (bz, by, bx) is placed at offset
  * (oz, oy, ox); X_0 := the stack.  get_block reads a block of X back.
- * These, stats, the pipeline below (set_raw_u16, deskew, affine),
- * snapshot / distance and warm_start are defined for one-stack engines and
- * fail on others. */
+ * These, stats, the pipeline below (set_raw_u16, deskew, affine) and
+ * snapshot / distance are defined for one-stack engines and fail on others
+ * (warm_start seeds every stack). */
 int cudaDecon_rl_set_raw_block(cudaDecon_rl* h, const float* block, int bx, int by, int bz, int ox, int oy, int oz,
                                float background);
 int cudaDecon_rl_get_block(cudaDecon_rl* h, float* out, int bx, int by, int bz, int ox, int oy, int oz);
