@@ -90,16 +90,24 @@ constexpr int kXTptOverride[] = {CDN_XTPT_OVERRIDE};
 
 // x passes: lines of up to 64 threads split/merge the packed row pair with
 // wavefront shuffles; longer lines (N >= 2048, measured faster with 16
-// elements per thread) use an LDS exchange.  Short power-of-two lines get
-// at least 32 threads, so that a wavefront's row loads and stores cover
-// whole 128-byte lines instead of 32-byte pieces of 8 rows (2-3x faster x
-// passes at 64, 128, 256 on stacks of those widths); the LRcore pass (lr),
-// two transforms per line, measured best at 16.
+// elements per thread) use an LDS exchange.  Short lines (16 threads or
+// fewer by the radix rule) get the smallest divisor of N from 32 up, so
+// that a wavefront's row loads and stores cover whole 128-byte lines
+// instead of 32-byte pieces of several rows (x passes 2-8x faster on stacks
+// 64 to 384 wide); the LRcore pass (lr), two transforms per line, measured
+// best from 16 up.  CDN_XTPT_RULE 1 keeps the rule to powers of two.
+#ifndef CDN_XTPT_RULE
+#define CDN_XTPT_RULE 2
+#endif
 constexpr int tpt_x(int N, bool lr = false) {
   if (lookup_override(kXTptOverride, N) > 0) return lookup_override(kXTptOverride, N);
   int t = tpt_col(N);
   const int want = lr ? 16 : 32;
-  if (N >= 64 && (N & (N - 1)) == 0 && t < want) t = want;
+  if (t >= want || t > 16 || N < 2 * want) return t;
+  if (CDN_XTPT_RULE == 1 && (N & (N - 1)) != 0) return t;
+  // the smallest divisor of N from `want` up (N itself is one)
+  for (int d = want; d <= N / 2; ++d)
+    if (N % d == 0) return d;
   return t;
 }
 
