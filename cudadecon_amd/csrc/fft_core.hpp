@@ -316,6 +316,91 @@ __device__ __forceinline__ void reg_stage(float2 (&v)[N / TPT], float2* line, in
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident variant with the exchanges in halves: real parts through
+// LDS, then imaginary parts, so a line needs N floats of LDS instead of N
+// complex values (more workgroups per CU) for two more barriers per
+// exchange.  `in` holds stage S's inputs, in[u R + t] = element j + t B with
+// j = p + u TPT; each exchange reads the next stage's inputs straight into
+// registers.  Same arithmetic, same order as reg_stage.
+// ---------------------------------------------------------------------------
+template <int N, int TPT, int SE, bool INV, int S>
+__device__ __forceinline__ void reg_stage_half(float2 (&in)[N / TPT], float2 (&v)[N / TPT], float* line, int p,
+                                               const float2* __restrict__ tw) {
+  constexpr Plan P = make_plan(N);
+  constexpr int R = P.r[S], LP = P.lp[S], B = N / R, U = B / TPT;
+  constexpr bool FIRST = (S == 0), LAST = (S == P.ns - 1);
+  float2 w[U * R];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = p + u * TPT;
+    float2 x[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) x[t] = in[u * R + t];
+    if constexpr (LP > 1) {
+      const int k = j % LP;
+      const float2* wt = tw + P.twoff[S] + k * (R - 1);
+#pragma unroll
+      for (int t = 1; t < R; ++t) x[t] = INV ? cmulc(x[t], wt[t - 1]) : cmul(x[t], wt[t - 1]);
+    }
+    Dft<R, INV>::run(x);
+#pragma unroll
+    for (int q = 0; q < R; ++q) w[u * R + q] = x[q];
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[u + q * U] = w[u * R + q];
+  } else {
+    constexpr int R2 = P.r[S + 1], B2 = N / R2, U2 = B2 / TPT;
+    float2 nx[N / TPT];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      // the previous reads of the line precede these writes
+      if (!FIRST || half == 1) __syncthreads();
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = p + u * TPT;
+        const int k = j % LP;
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) line[(base + q * LP) * SE] = half ? w[u * R + q].y : w[u * R + q].x;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int j = p + u * TPT;
+#pragma unroll
+        for (int t = 0; t < R2; ++t) {
+          const float f = line[(j + t * B2) * SE];
+          if (half)
+            nx[u * R2 + t].y = f;
+          else
+            nx[u * R2 + t].x = f;
+        }
+      }
+    }
+    reg_stage_half<N, TPT, SE, INV, S + 1>(nx, v, line, p, tw);
+  }
+}
+
+// fft_line (below) for exact plans with the exchanges in halves; `line` is
+// the float LDS line (N floats at stride SE).  Same contract as fft_line.
+template <int N, int TPT, int SE, bool INV>
+__device__ __forceinline__ void fft_line_half(float2 (&v)[N / TPT], float* line, int p,
+                                              const float2* __restrict__ tw) {
+  static_assert(plan_exact(N, TPT), "the half exchange needs an exact plan");
+  constexpr Plan P = make_plan(N);
+  constexpr int R = P.r[0], U = N / R / TPT;
+  float2 in[N / TPT];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int t = 0; t < R; ++t) in[u * R + t] = v[u + t * U];
+  reg_stage_half<N, TPT, SE, INV, 0>(in, v, line, p, tw);
+}
+
 // FFT of one line whose thread p holds elements p + m TPT in v[m] (m < N/TPT)
 // on entry and on exit.  Contract: the caller has made the LDS line free
 // (a barrier after its last use) and must barrier again before reusing it.

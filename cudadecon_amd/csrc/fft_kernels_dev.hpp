@@ -165,9 +165,16 @@ template <int N, bool LR = false> struct XCfg {
   static constexpr bool SHUFFLE = S.SHUFFLE;
 };
 
+// Column passes exchange between FFT stages in halves (real, then imaginary
+// parts) when the plan is exact: half the LDS per workgroup.
+#ifndef CDN_SPLIT_LDS
+#define CDN_SPLIT_LDS 1
+#endif
 template <int N> struct CCfg {
   static constexpr CShape S = c_shape(N);
   static constexpr int TPT = S.TPT, L = S.L, THREADS = S.THREADS, E = S.E;
+  static constexpr bool HALF = CDN_SPLIT_LDS != 0 && plan_exact(N, S.TPT);
+  static constexpr int LDS = HALF ? (N * L + 1) / 2 : N * L;  // float2 elements
 };
 
 // ---------------------------------------------------------------------------
@@ -507,12 +514,24 @@ __device__ __forceinline__ long xcd_logical_block(long b, long G) {
 // ---------------------------------------------------------------------------
 // y / z column passes: thread (l, p) holds column l's elements p + m TPT
 // ---------------------------------------------------------------------------
+// the transform of column l of a tile (LDS as declared by the kernel)
+template <int N, bool INV>
+__device__ __forceinline__ void col_line(float2 (&v)[CCfg<N>::E], float2* lds, int l, int p,
+                                         const float2* __restrict__ tw) {
+  using C = CCfg<N>;
+  if constexpr (C::HALF) {
+    fft_line_half<N, C::TPT, C::L, INV>(v, reinterpret_cast<float*>(lds) + l, p, tw);
+  } else {
+    fft_line<N, C::TPT, C::L, INV>(v, lds + l, p, tw);
+  }
+}
+
 template <int N, bool INV>
 __global__ void __launch_bounds__(CCfg<N>::THREADS)
     k_col(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles,
           const float2* __restrict__ tw) {
   using C = CCfg<N>;
-  __shared__ float2 lds[N * C::L];
+  __shared__ float2 lds[C::LDS];
   const int tid = threadIdx.x;
   const int l = tid % C::L, p = tid / C::L;
   const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
@@ -524,7 +543,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
 #pragma unroll
   for (int m = 0; m < C::E; ++m)
     v[m] = valid ? ld_stream<NTCL>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
-  fft_line<N, C::TPT, C::L, INV>(v, lds + l, p, tw);
+  col_line<N, INV>(v, lds, l, p, tw);
   if (valid) {
 #pragma unroll
     for (int m = 0; m < C::E; ++m) st_stream<NTCS>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
@@ -533,12 +552,26 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
 
 // filter_kernel / filterConj_kernel (src/RLgpuImpl.cu:331-350) with the OTF
 // value computed on the fly as in dev_otfinterpolate (src/RLgpuImpl.cu:228-329)
+// Waves per SIMD the z filter is compiled for (1: the compiler's choice).
+// Length 256 (32-column tiles of 512 threads, 32 KB of LDS) at 6 waves, three
+// workgroups per CU instead of two, for 80 registers and 15 spilled dwords:
+// 4% faster at 256 x 1024 x 1024; the other lengths spill far more and
+// measured slower (64: 0.018 -> 0.023 ms at 64 x 256 x 256).
+#ifndef CDN_ZF_WAVES
+#define CDN_ZF_WAVES 0
+#endif
+constexpr int zf_waves(int N) {
+  if (CDN_ZF_WAVES > 0) return CDN_ZF_WAVES;
+  const CShape c = c_shape(N);
+  const bool half = CDN_SPLIT_LDS != 0 && plan_exact(N, c.TPT);
+  return (N == 256 && c.THREADS == 512 && half && N * c.L * 4 <= 32768) ? 6 : 1;
+}
 template <int N>
-__global__ void __launch_bounds__(CCfg<N>::THREADS)
+__global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(zf_waves(N))))
     k_z_filter(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles, long ny,
                long stack_stride, int conj, OtfParams otf, const float2* __restrict__ tw) {
   using C = CCfg<N>;
-  __shared__ float2 lds[N * C::L];
+  __shared__ float2 lds[C::LDS];
   const int tid = threadIdx.x;
   const int l = tid % C::L, p = tid / C::L;
   const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
@@ -552,7 +585,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
 #pragma unroll
   for (int m = 0; m < C::E; ++m)
     v[m] = valid ? ld_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
-  fft_line<N, C::TPT, C::L, false>(v, lds + l, p, tw);
+  col_line<N, false>(v, lds, l, p, tw);
 
   // signed ky (k > n>>1 -> k - n), scaled k coordinates
   const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
@@ -620,7 +653,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
     }
   }
   __syncthreads();
-  fft_line<N, C::TPT, C::L, true>(v, lds + l, p, tw);
+  col_line<N, true>(v, lds, l, p, tw);
   if (valid) {
 #pragma unroll
     for (int m = 0; m < C::E; ++m) st_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
