@@ -550,8 +550,6 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   }
 }
 
-// filter_kernel / filterConj_kernel (src/RLgpuImpl.cu:331-350) with the OTF
-// value computed on the fly as in dev_otfinterpolate (src/RLgpuImpl.cu:228-329)
 // Waves per SIMD the z filter is compiled for (1: the compiler's choice).
 // Length 256 (32-column tiles of 512 threads, 32 KB of LDS) at 6 waves, three
 // workgroups per CU instead of two, for 80 registers and 15 spilled dwords:
@@ -566,12 +564,29 @@ constexpr int zf_waves(int N) {
   const bool half = CDN_SPLIT_LDS != 0 && plan_exact(N, c.TPT);
   return (N == 256 && c.THREADS == 512 && half && N * c.L * 4 <= 32768) ? 6 : 1;
 }
+// Rows of the rotationally averaged OTF table staged in LDS per tile.  A
+// tile is one ky and L adjacent kx, so its radii span a few table rows.
+// Measured: z length 128 (128 x 512 x 810) 0.117 -> 0.104 ms; 64 no change;
+// 256 (256 x 1024 x 1024, at 6 waves with 15 spilled dwords) 0.508 -> 0.527,
+// so only 128 stages.  0 disables.
+#ifndef CDN_ZF_ROWS
+#define CDN_ZF_ROWS 6
+#endif
+template <int N> struct ZRows {
+  static constexpr int R = N == 128 ? CDN_ZF_ROWS : 0;
+};
+
+// filter_kernel / filterConj_kernel (src/RLgpuImpl.cu:331-350) with the OTF
+// value computed on the fly as in dev_otfinterpolate (src/RLgpuImpl.cu:228-329)
 template <int N>
 __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(zf_waves(N))))
     k_z_filter(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles, long ny,
                long stack_stride, int conj, OtfParams otf, const float2* __restrict__ tw) {
   using C = CCfg<N>;
+  constexpr int RS = ZRows<N>::R;
+  constexpr int PER = RS > 0 ? (RS * N + C::THREADS - 1) / C::THREADS : 1;
   __shared__ float2 lds[C::LDS];
+  __shared__ float2 otf_rows[RS > 0 ? RS * N : 1];
   const int tid = threadIdx.x;
   const int l = tid % C::L, p = tid / C::L;
   const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
@@ -580,17 +595,48 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
   const long stack = slab / ny, outer = slab - stack * ny;
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
+  // signed ky (k > n>>1 -> k - n), scaled k coordinates
+  const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
+  const float ky = (float)kyi * otf.kyscale;
+
+  // table rows r_lo .. r_lo + nrows - 1 cover the tile's radii (with a row of
+  // margin on each side); their loads go out before the stack's
+  int r_lo = 0, nrows = 0;
+  float2 stage[PER];
+  if constexpr (RS > 0) {
+    if (otf.ra) {
+      const float kx0 = (float)(tile * C::L) * otf.kxscale, kx1 = (float)(tile * C::L + C::L - 1) * otf.kxscale;
+      const float kr0 = sqrtf(kx0 * kx0 + ky * ky), kr1 = sqrtf(kx1 * kx1 + ky * ky);
+      const int nr = otf.ra_tab.nr;
+      if (kr0 < (float)(nr - 1)) {
+        r_lo = max(0, (int)floorf(kr0) - 1);
+        const int r_hi = min(nr - 1, (int)floorf(fminf(kr1, (float)nr)) + 2);
+        nrows = min(RS, r_hi - r_lo + 1);
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = tid + k * C::THREADS;
+        stage[k] = e < nrows * N ? otf.ra_tab.tz[(long)r_lo * N + e] : make_float2(0.f, 0.f);
+      }
+    }
+  }
+
   float2* base = S + stack * stack_stride + outer * outer_stride + c;
   float2 v[C::E];
 #pragma unroll
   for (int m = 0; m < C::E; ++m)
     v[m] = valid ? ld_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
+  if constexpr (RS > 0) {
+    // visible after the transform's barriers
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * C::THREADS;
+      if (e < nrows * N) otf_rows[e] = stage[k];
+    }
+  }
   col_line<N, false>(v, lds, l, p, tw);
 
-  // signed ky (k > n>>1 -> k - n), scaled k coordinates
-  const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
   const float kx = (float)c * otf.kxscale;
-  const float ky = (float)kyi * otf.kyscale;
   const float sg = conj ? -1.f : 1.f;
   if (otf.ra) {
     const float kr = sqrtf(kx * kx + ky * ky);
@@ -599,17 +645,38 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
     const float ar = kr - (float)ir;
     const float2* t0 = otf.ra_tab.tz + (long)ir * N;
     const float2* t1 = t0 + N;
+    const int rr = ir - r_lo;
+    // one source per wavefront: the staged rows when they hold every row
+    // the wavefront's lanes interpolate between
+    bool from_lds = false;
+    if constexpr (RS > 0) from_lds = __all(!inside || (rr >= 0 && rr + 1 < nrows));
+    const float2* s0 = otf_rows + (rr >= 0 && rr < RS ? rr : 0) * N;
+    if (from_lds) {
 #pragma unroll
-    for (int m = 0; m < C::E; ++m) {
-      const int i = p + m * C::TPT;
-      float2 r = make_float2(0.f, 0.f);
-      if (inside) {
-        const float2 a = t0[i], b = t1[i];
-        float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
-        h.y *= sg;
-        r = cscale(cmul(h, v[m]), otf.inv_n);
+      for (int m = 0; m < C::E; ++m) {
+        const int i = p + m * C::TPT;
+        float2 r = make_float2(0.f, 0.f);
+        if (inside) {
+          const float2 a = s0[i], b = s0[N + i];
+          float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
+          h.y *= sg;
+          r = cscale(cmul(h, v[m]), otf.inv_n);
+        }
+        v[m] = r;
       }
-      v[m] = r;
+    } else {
+#pragma unroll
+      for (int m = 0; m < C::E; ++m) {
+        const int i = p + m * C::TPT;
+        float2 r = make_float2(0.f, 0.f);
+        if (inside) {
+          const float2 a = t0[i], b = t1[i];
+          float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
+          h.y *= sg;
+          r = cscale(cmul(h, v[m]), otf.inv_n);
+        }
+        v[m] = r;
+      }
     }
   } else {
     const Otf3d& T = otf.tab3;
