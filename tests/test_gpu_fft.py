@@ -56,7 +56,10 @@ def _ra_table(nr=17, nz=33, seed=0):
 
 
 @pytest.mark.parametrize("conj", [False, True])
-@pytest.mark.parametrize("shape", [(64, 128, 128), (128, 64, 810), (64, 96, 192)], ids=lambda s: str(s))
+# 256 x 16 x 1024: z length 256 (6-wave z filter, 32-column tiles) with the
+# kx scale of a 1024-wide stack (1/32)
+@pytest.mark.parametrize("shape", [(64, 128, 128), (128, 64, 810), (64, 96, 192), (256, 16, 1024)],
+                         ids=lambda s: str(s))
 def test_filter_ra_matches_oracle(cdn, orc, shape, conj):
     print("filter", shape, conj, flush=True)
     tab = _ra_table()

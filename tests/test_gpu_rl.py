@@ -32,8 +32,8 @@ def _relmax(a, b):
 
 
 @pytest.mark.parametrize("shape,iters", [((64, 128, 128), 10), ((64, 96, 810), 6), ((96, 64, 64), 12),
-                                         ((70, 72, 150), 8)],
-                         ids=["beads_64x128x128", "x810", "z96", "runtime-compiled"])
+                                         ((70, 72, 150), 8), ((256, 16, 1024), 6)],
+                         ids=["beads_64x128x128", "x810", "z96", "runtime-compiled", "z256"])
 def test_engine_matches_oracle(cdn, orc, shape, iters):
     print("engine", shape, iters, flush=True)
     from cudadecon_amd.synth import make_stack

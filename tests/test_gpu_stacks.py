@@ -64,6 +64,37 @@ def test_stacks_equal_single_engines_bitwise(cdn):
     m.close()
 
 
+def test_z256_stacks(cdn, orc):
+    """z length 256 (the 6-wave z filter) with the kx scale of a wide stack:
+    three stacks in one engine equal one-stack engines bitwise, and one of
+    them the oracle."""
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import gaussian_psf
+    tab = make_otf(gaussian_psf(1.5, 3.0))
+    shape = (256, 16, 128)
+    ks = (np.float32(0.03), np.float32(0.3), np.float32(0.13))
+    rng = np.random.default_rng(21)
+    stacks = rng.gamma(2.0, 40.0, (3,) + shape).astype(np.float32)
+    stacks[1] *= 2.5
+    print("z256 stacks", shape, flush=True)
+    m = cdn.RLEngine(shape, tab, ks, stacks=3)
+    m.set_raw(stacks)
+    lam, _ = m.iterate(5)
+    X = m.get()
+    m.close()
+    for s in range(3):
+        e = cdn.RLEngine(shape, tab, ks)
+        e.set_raw(stacks[s])
+        ls, _ = e.iterate(5)
+        assert np.array_equal(X[s], e.get()), s
+        assert np.array_equal(lam[:, s], ls), s
+        e.close()
+    nz, ny, nx = shape
+    H = orc.otf_interp(tab, (tab.shape[0], 1, tab.shape[1]), (nx, ny, nz), ks)
+    ref, _ = orc.rl_ba(stacks[1].astype(np.float64), H, 5)
+    assert np.abs(X[1] - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
 def test_stack_shape_and_single_stack_entry_points_are_checked(cdn):
     tab, ks, _ = _setup(cdn)
     with pytest.raises(RuntimeError, match="multiple"):
