@@ -129,8 +129,11 @@ def stacks_256(cdn, tab, iters=10, nstreams=8):
     return out
 
 
-def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int):
-    """FP32 oracle RL-BA (OpenMP) on the first `slab` z-planes of the stack."""
+def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int, warm: int = 2):
+    """FP32 oracle RL-BA (OpenMP) on the first `slab` z-planes of the stack:
+    steady-state iterations k = warm .. warm + iters - 1, timed as a run of
+    warm + iters iterations less a run of the `warm` first ones (k = 0, 1
+    have no prediction and k = 0 no dot products)."""
     import oracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
@@ -141,15 +144,20 @@ def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int):
                           (2 * tab.shape[1], tab.shape[0], 1))
     H = orc.otf_interp(tab, g["otf_dims"], g["fft_dims"], ks).astype(np.complex64)
     t0 = time.perf_counter()
-    orc.rl_ba_f32(sub, H, iters)
-    dt = time.perf_counter() - t0
+    orc.rl_ba_f32(sub, H, warm)
+    t1 = time.perf_counter()
+    orc.rl_ba_f32(sub, H, warm + iters)
+    t2 = time.perf_counter()
+    dt = (t2 - t1) - (t1 - t0)
     return {
         "value": round(sub.size * iters / dt / 1e9, 5),
         "unit": "Gvoxel/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP32 oracle RL-BA (own mixed-radix FFT, OpenMP), {iters} iterations k=0..{iters - 1} "
-                  f"on the first {slab} z-planes ({nz}x{ny}x{nx}) of the stack, {dt:.1f} s",
+        "sample": f"FP32 oracle RL-BA (own mixed-radix FFT, OpenMP) on the first {slab} z-planes "
+                  f"({nz}x{ny}x{nx}) of the stack: {iters} steady iterations k={warm}..{warm + iters - 1}, "
+                  f"mean {dt / iters:.3f} s each (a {warm + iters}-iteration run, {t2 - t1:.1f} s, less a "
+                  f"{warm}-iteration run, {t1 - t0:.1f} s)",
     }
 
 
@@ -212,7 +220,7 @@ def main():
     ap.add_argument("--workload", default="beads_256x1024x1024", choices=["beads_256x1024x1024", "beads_64x128x128", "lightsheet_128x512x512"])
     ap.add_argument("--pass-iters", type=int, default=3, help="iterations with per-pass event timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--cpu-slab", type=int, default=64)
     ap.add_argument("--no-e2e", action="store_true", help="skip the RL_interface end-to-end timing")
     ap.add_argument("--e2e-iters", type=int, default=10)
