@@ -96,11 +96,21 @@ constexpr int kXTptOverride[] = {CDN_XTPT_OVERRIDE};
 // instead of 32-byte pieces of several rows (x passes 2-8x faster on stacks
 // 64 to 384 wide); the LRcore pass (lr), two transforms per line, measured
 // best from 16 up.  CDN_XTPT_RULE 1 keeps the rule to powers of two.
+//
+// Rule 3 adds: lines whose plan still leaves more than 16 elements per
+// thread (radix lcm 20 to 30: 768, 810) held them in up to 256 registers,
+// one or two waves per SIMD; they take the smallest thread count with at
+// most 6 elements per thread and the all-LDS transform.  The LRcore pass
+// (two transforms per line) does so only above 24 elements per thread, to
+// at most 15: at 24 (384, 768) the register-resident transform measured
+// faster (x_lrcore 0.102 vs 0.120 ms at 64 x 768 x 768).  At 128 x 512 x 810
+// the x passes went from 0.25 / 0.47 / 0.39 ms (R2C / LRcore / update) to
+// 0.18 / 0.26 / 0.24; at 64 x 768 x 768 R2C and update from 0.117 / 0.132
+// to 0.106 / 0.125.
 #ifndef CDN_XTPT_RULE
-#define CDN_XTPT_RULE 2
+#define CDN_XTPT_RULE 3
 #endif
-constexpr int tpt_x(int N, bool lr = false) {
-  if (lookup_override(kXTptOverride, N) > 0) return lookup_override(kXTptOverride, N);
+constexpr int tpt_x_short(int N, bool lr) {
   int t = tpt_col(N);
   const int want = lr ? 16 : 32;
   if (t >= want || t > 16 || N < 2 * want) return t;
@@ -108,6 +118,15 @@ constexpr int tpt_x(int N, bool lr = false) {
   // the smallest divisor of N from `want` up (N itself is one)
   for (int d = want; d <= N / 2; ++d)
     if (N % d == 0) return d;
+  return t;
+}
+constexpr int tpt_x(int N, bool lr = false) {
+  if (lookup_override(kXTptOverride, N) > 0) return lookup_override(kXTptOverride, N);
+  const int t = tpt_x_short(N, lr);
+  if (CDN_XTPT_RULE < 3 || N / t <= (lr ? 24 : 16)) return t;
+  const int emax = lr ? 15 : 6;
+  for (int d = t; d <= N && d <= 1024; ++d)
+    if (N % d == 0 && N / d <= emax) return d;
   return t;
 }
 
