@@ -169,9 +169,13 @@ constexpr XShape x_shape(int N, int xthreads = -1, bool lr = false) {
 struct CShape {
   int TPT, L, THREADS, E;
 };
+#ifndef CDN_CTPT_OVERRIDE
+#define CDN_CTPT_OVERRIDE 0, 0
+#endif
+constexpr int kCTptOverride[] = {CDN_CTPT_OVERRIDE};
 constexpr CShape c_shape(int N, int cols = -1) {
   CShape s{};
-  s.TPT = tpt_col(N);
+  s.TPT = lookup_override(kCTptOverride, N) > 0 ? lookup_override(kCTptOverride, N) : tpt_col(N);
   s.L = lc_for(N, s.TPT, cols);
   s.THREADS = s.TPT * s.L;
   s.E = N / s.TPT;
