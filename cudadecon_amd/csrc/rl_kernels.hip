@@ -239,31 +239,21 @@ void accel_factor(hipStream_t st, const double* part, int nb, double eps, double
 }
 
 // ---------------------------------------------------------------------------
-// Plain streaming copy (16-byte non-temporal vectors, grid-stride): the bandwidth
-// reference the iteration's passes are compared with.
+// Plain streaming copy: the bandwidth reference the iteration's passes are
+// compared with.  One 16-byte non-temporal vector per thread, one workgroup
+// per 4 KB: of the forms measured on a 1 GiB copy this was the fastest (6.7
+// TB/s; 4 vectors per thread, grid-stride 6.4, consecutive 5.9; cached 5.7).
 // ---------------------------------------------------------------------------
-#ifndef CDN_COPY_UNROLL
-#define CDN_COPY_UNROLL 1
-#endif
 __global__ void __launch_bounds__(BS) k_copy16(const float4* __restrict__ in, float4* __restrict__ out, long n4) {
-  constexpr int U = CDN_COPY_UNROLL;
   const long stride = (long)gridDim.x * BS;
-  long i = (long)blockIdx.x * BS + threadIdx.x;
   const f4v* src = reinterpret_cast<const f4v*>(in);
   f4v* dst = reinterpret_cast<f4v*>(out);
-  for (; i + (U - 1) * stride < n4; i += U * stride) {
-    f4v v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
-#pragma unroll
-    for (int u = 0; u < U; ++u) __builtin_nontemporal_store(v[u], dst + i + u * stride);
-  }
-  for (; i < n4; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n4; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes) {
   const long n4 = (long)(bytes / 16);
-  hipLaunchKernelGGL(k_copy16, dim3(nblocks(n4, 4 * CDN_COPY_UNROLL)), dim3(BS), 0, st, (const float4*)in,
-                     (float4*)out, n4);
+  hipLaunchKernelGGL(k_copy16, dim3(nblocks(n4)), dim3(BS), 0, st, (const float4*)in, (float4*)out, n4);
   CDN_HIP(hipGetLastError());
 }
 
