@@ -595,6 +595,12 @@ constexpr int zf_waves(int N) {
 #ifndef CDN_ZF_ROWS
 #define CDN_ZF_ROWS 6
 #endif
+// scaled signed ky of slab row `outer` (k > n>>1 -> k - n)
+__device__ __forceinline__ float zf_ky(long outer, long ny, float kyscale) {
+  const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
+  return (float)kyi * kyscale;
+}
+
 template <int N> struct ZRows {
   static constexpr int R = N == 128 ? CDN_ZF_ROWS : 0;
 };
@@ -618,9 +624,6 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
   const long stack = slab / ny, outer = slab - stack * ny;
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
-  // signed ky (k > n>>1 -> k - n), scaled k coordinates
-  const int kyi = (outer > (ny >> 1)) ? (int)(outer - ny) : (int)outer;
-  const float ky = (float)kyi * otf.kyscale;
 
   // table rows r_lo .. r_lo + nrows - 1 cover the tile's radii (with a row of
   // margin on each side); their loads go out before the stack's
@@ -628,6 +631,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
   float2 stage[PER];
   if constexpr (RS > 0) {
     if (otf.ra) {
+      const float ky = zf_ky(outer, ny, otf.kyscale);
       const float kx0 = (float)(tile * C::L) * otf.kxscale, kx1 = (float)(tile * C::L + C::L - 1) * otf.kxscale;
       const float kr0 = sqrtf(kx0 * kx0 + ky * ky), kr1 = sqrtf(kx1 * kx1 + ky * ky);
       const int nr = otf.ra_tab.nr;
@@ -660,6 +664,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
   col_line<N, false>(v, lds, l, p, tw);
 
   const float kx = (float)c * otf.kxscale;
+  const float ky = zf_ky(outer, ny, otf.kyscale);
   const float sg = conj ? -1.f : 1.f;
   if (otf.ra) {
     const float kr = sqrtf(kx * kx + ky * ky);
