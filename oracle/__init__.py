@@ -149,14 +149,25 @@ def otf_interp(raw_otf: np.ndarray, otf_dims, grid, kscale) -> np.ndarray:
 def filt(v: np.ndarray, H: np.ndarray, conj: bool = False) -> np.ndarray:
     """One filterGPU call: C2R(H . R2C(v / N)) (src/RLgpuImpl.cu:156-225)."""
     v = _c(v, np.float64).copy()
+    _check_H(v.shape, H)
     nz, ny, nx = v.shape
     lib().orc_filter_f64(v, nx, ny, nz, _c(H, np.complex128).view(np.float64).reshape(-1), int(conj))
     return v
 
 
+def _check_H(shape, H):
+    """The half-spectrum OTF must match the volume (the C routines index it
+    by the volume's shape)."""
+    nz, ny, nx = shape
+    if tuple(H.shape) != (nz, ny, nx // 2 + 1):
+        raise ValueError(f"OTF shape {tuple(H.shape)} does not match the volume {tuple(shape)}: "
+                         f"expected {(nz, ny, nx // 2 + 1)}")
+
+
 def rl_ba(raw: np.ndarray, H: np.ndarray, n_iter: int, seed: np.ndarray | None = None):
     """FP64 RL-Biggs-Andrews; returns (X, lambdas)."""
     raw = _c(raw, np.float64)
+    _check_H(raw.shape, H)
     nz, ny, nx = raw.shape
     X = np.empty_like(raw)
     lam = np.zeros(max(n_iter, 1), np.float64)
@@ -172,6 +183,7 @@ def rl_ba(raw: np.ndarray, H: np.ndarray, n_iter: int, seed: np.ndarray | None =
 def rl_ba_f32(raw: np.ndarray, H: np.ndarray, n_iter: int, seed: np.ndarray | None = None):
     """FP32 flavour (timed CPU baseline); returns (X, lambdas)."""
     raw = _c(raw, np.float32)
+    _check_H(raw.shape, H)
     nz, ny, nx = raw.shape
     X = np.empty_like(raw)
     lam = np.zeros(max(n_iter, 1), np.float64)

@@ -152,6 +152,56 @@ def test_rl_interface_preprocessing_only_with_rotation(cdn, orc, tmp_path):
     assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
+# RL_interface options against the oracle's restatement of RichardsonLucy_GPU
+# (src/RL-Biggs-Andrews.cpp:56-450): skewed decon (iterate on the raw grid,
+# deskew after), z blend, the doubled reversed stack, the deskewed raw output,
+# and the bleach rescale, whose first call fixes the reference intensity.
+# 64 x 64 x 100 raw at 31.5 deg, dz 0.3: deskewed width 256 (built in).
+OPTION_CASES = {
+    "skewed_decon": dict(skewed=True, kw=dict()),
+    "z_blend_4": dict(skewed=False, kw=dict(nzblend=4)),
+    "dup_rev_stack": dict(skewed=False, kw=dict(dup=True)),
+    "save_deskewed": dict(skewed=False, kw=dict(save=True)),
+    "rescale_second_call": dict(skewed=False, kw=dict(rescale=True)),
+}
+
+
+@pytest.mark.parametrize("case", list(OPTION_CASES), ids=list(OPTION_CASES))
+def test_rl_interface_options(cdn, orc, tmp_path, case):
+    print("RL_interface option", case, flush=True)
+    from cudadecon_amd.synth import make_stack
+    raw, _, _ = make_stack("lightsheet_128x512x512", shape=(64, 64, 100))
+    tab, _ = _otf_and_geom(orc, raw.shape, 1.3, 4.0, 0.104, 0.3)
+    nz, ny, nx = raw.shape
+    c = OPTION_CASES[case]
+    skewed, kw = c["skewed"], c["kw"]
+    p = _write_otf(tmp_path, tab)
+    cdn.rl_cleanup()
+    cdn.rl_init(raw.shape, p, dzdata=0.3, dxdata=0.104, dzpsf=0.3, dxpsf=0.104, deskew=31.5, rotate=0.0,
+                skewed_decon=skewed)
+    g = orc.init_geometry(nx, ny, nz, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, 0, skewed, (2 * tab.shape[1], tab.shape[0], 1))
+    assert cdn.output_shape() == (g["out_dims"][2], g["out_dims"][1], g["out_dims"][0])
+    sess = orc.Session(g, tab)
+    iters = 4
+    save = kw.get("save", False)
+    resc = kw.get("rescale", False)
+    args = dict(napodize=0, nzblend=kw.get("nzblend", 0), pad_val=0.0, dup_rev_stack=kw.get("dup", False),
+                skewed_decon=skewed)
+    gargs = dict(napodize=0, nz_blend=kw.get("nzblend", 0), pad_val=0.0, dup_reversed_z=kw.get("dup", False),
+                 skewed_decon=skewed)
+    calls = [raw, np.clip(raw.astype(np.int32) * 3 // 4 + 25, 0, 65535).astype(np.uint16)] if resc else [raw]
+    for im in calls:
+        ref, ref_desk, _ = orc.rl_interface(sess, im, 100.0, resc, save, iters, 0, **args)
+        got = cdn.rl_decon(im, background=100.0, n_iters=iters, save_deskewed=save, rescale=resc, **gargs)
+        if save:
+            got, got_desk = got
+            assert got_desk.shape == ref_desk.shape
+            assert np.abs(got_desk - ref_desk).max() <= 1e-5 * np.abs(ref_desk).max()
+        assert got.shape == ref.shape
+        assert _relmax(got, ref) <= TOL, (case, _relmax(got, ref))
+    cdn.rl_cleanup()
+
+
 def test_deskew_interface(cdn, orc):
     rng = np.random.default_rng(11)
     im = rng.uniform(0, 1000, (32, 40, 100)).astype(np.float32)

@@ -399,3 +399,16 @@ def test_wiener_closed_forms(orc):
     # each plane becomes its mean
     out = orc.wiener(v, tab, (40, 1, 8), ks, (1.0, 1.0), 0.5, 0.0)
     assert np.abs(out - v.mean(axis=(1, 2), keepdims=True)).max() < 1e-9
+
+
+def test_otf_shape_mismatch_raises():
+    """The C routines index the OTF by the volume's shape: a mismatched OTF is
+    refused before the call (a skewed decon OTF on a deskewed volume, say)."""
+    import oracle as orc
+    v = np.zeros((4, 4, 8))
+    with pytest.raises(ValueError):
+        orc.rl_ba(v, np.zeros((4, 4, 4), np.complex128), 1)
+    with pytest.raises(ValueError):
+        orc.rl_ba_f32(v.astype(np.float32), np.zeros((4, 4, 6), np.complex64), 1)
+    with pytest.raises(ValueError):
+        orc.filt(v, np.zeros((4, 3, 5), np.complex128))
