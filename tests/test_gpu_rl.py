@@ -163,6 +163,8 @@ OPTION_CASES = {
     "dup_rev_stack": dict(skewed=False, kw=dict(dup=True)),
     "save_deskewed": dict(skewed=False, kw=dict(save=True)),
     "rescale_second_call": dict(skewed=False, kw=dict(rescale=True)),
+    "shift_and_pad": dict(skewed=False, kw=dict(shift=7, pad=50.0)),
+    "output_width_300": dict(skewed=False, kw=dict(width=300)),
 }
 
 
@@ -177,21 +179,25 @@ def test_rl_interface_options(cdn, orc, tmp_path, case):
     skewed, kw = c["skewed"], c["kw"]
     p = _write_otf(tmp_path, tab)
     cdn.rl_cleanup()
+    width = kw.get("width", 0)
     cdn.rl_init(raw.shape, p, dzdata=0.3, dxdata=0.104, dzpsf=0.3, dxpsf=0.104, deskew=31.5, rotate=0.0,
-                skewed_decon=skewed)
-    g = orc.init_geometry(nx, ny, nz, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, 0, skewed, (2 * tab.shape[1], tab.shape[0], 1))
+                width=width, skewed_decon=skewed)
+    g = orc.init_geometry(nx, ny, nz, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, width, skewed,
+                          (2 * tab.shape[1], tab.shape[0], 1))
     assert cdn.output_shape() == (g["out_dims"][2], g["out_dims"][1], g["out_dims"][0])
     sess = orc.Session(g, tab)
     iters = 4
     save = kw.get("save", False)
     resc = kw.get("rescale", False)
-    args = dict(napodize=0, nzblend=kw.get("nzblend", 0), pad_val=0.0, dup_rev_stack=kw.get("dup", False),
+    pad = kw.get("pad", 0.0)
+    shift = kw.get("shift", 0)
+    args = dict(napodize=0, nzblend=kw.get("nzblend", 0), pad_val=pad, dup_rev_stack=kw.get("dup", False),
                 skewed_decon=skewed)
-    gargs = dict(napodize=0, nz_blend=kw.get("nzblend", 0), pad_val=0.0, dup_reversed_z=kw.get("dup", False),
-                 skewed_decon=skewed)
+    gargs = dict(napodize=0, nz_blend=kw.get("nzblend", 0), pad_val=pad, dup_reversed_z=kw.get("dup", False),
+                 skewed_decon=skewed, shift=shift)
     calls = [raw, np.clip(raw.astype(np.int32) * 3 // 4 + 25, 0, 65535).astype(np.uint16)] if resc else [raw]
     for im in calls:
-        ref, ref_desk, _ = orc.rl_interface(sess, im, 100.0, resc, save, iters, 0, **args)
+        ref, ref_desk, _ = orc.rl_interface(sess, im, 100.0, resc, save, iters, shift, **args)
         got = cdn.rl_decon(im, background=100.0, n_iters=iters, save_deskewed=save, rescale=resc, **gargs)
         if save:
             got, got_desk = got
