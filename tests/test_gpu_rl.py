@@ -282,3 +282,46 @@ def test_device_pipeline_matches_host_entry_points(cdn, orc):
     s = eng.post_stats()
     assert abs(s[0] - float(a.astype(np.float64).sum())) <= 1e-9 * abs(s[0]) and s[1] == a.min() and s[2] == a.max()
     eng.close()
+
+
+def test_engine_random_shapes_match_oracle(cdn, orc):
+    """Seeded random 2,3,5,7-smooth shapes (72 and 100 compiled at run time),
+    3 iterations each, against the FP64 oracle."""
+    rng = np.random.default_rng(2024)
+    lens = [64, 96, 128, 192, 72, 100]
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import gaussian_psf
+    tab = make_otf(gaussian_psf(1.5, 3.0))
+    for _ in range(5):
+        shape = tuple(int(v) for v in rng.choice(lens, 3))
+        print("random shape", shape, flush=True)
+        nz, ny, nx = shape
+        g = orc.init_geometry(nx, ny, nz, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False,
+                              (2 * tab.shape[1], tab.shape[0], 1))
+        R = rng.gamma(2.0, 40.0, shape)
+        H = orc.otf_interp(tab, g["otf_dims"], g["fft_dims"], g["kscale"])
+        ref, lam_ref = orc.rl_ba(R, H, 3)
+        eng = cdn.RLEngine(shape, tab, g["kscale"])
+        eng.set_raw(R.astype(np.float32))
+        lam, _ = eng.iterate(3)
+        got = eng.get()
+        eng.close()
+        assert _relmax(got, ref) <= TOL, (shape, _relmax(got, ref))
+        assert np.abs(lam - lam_ref).max() < 1e-4
+
+
+def test_engine_larger_than_memory_fails_cleanly(cdn):
+    """40 stacks of 512 x 2048 x 2048 need 344 GB per volume, more than the
+    GPU holds: creation reports the failed allocation (no exit, no crash) and
+    keeps nothing; an ordinary engine works afterwards."""
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import gaussian_psf
+    tab = make_otf(gaussian_psf(1.5, 3.0))
+    ks = (np.float32(0.25), np.float32(0.25), np.float32(0.5))
+    with pytest.raises(RuntimeError, match="(?i)memory"):
+        cdn.RLEngine((512, 2048, 2048), tab, ks, stacks=40)
+    e = cdn.RLEngine((64, 128, 128), tab, ks)
+    e.set_raw(np.random.default_rng(1).uniform(0, 100, (64, 128, 128)).astype(np.float32))
+    e.iterate(2)
+    assert np.isfinite(e.get()).all()
+    e.close()
