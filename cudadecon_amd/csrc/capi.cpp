@@ -8,6 +8,7 @@
 
 #include "batch.hpp"
 #include "decon.hpp"
+#include "hostcopy.hpp"
 #include "net.hpp"
 #include "rl_kernels.hpp"
 #include "tiff.hpp"
@@ -285,7 +286,7 @@ int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, f
     const hipStream_t st = S.st;
     const size_t nin = (size_t)nx * ny * nz;
     S.u16.ensure(nin * sizeof(uint16_t));
-    CDN_HIP(hipMemcpyAsync(S.u16.p, raw_data, nin * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    copy_to_device(st, S.u16.p, raw_data, nin * sizeof(uint16_t));
     unsigned cx = S.output_nx, cy = S.output_ny, cz = S.output_nz;
     auto vox = [&]() { return (size_t)cx * cy * cz; };
     S.bufA.ensure(vox() * sizeof(float));
@@ -321,11 +322,10 @@ int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, f
         cx = S.deskewedXdim;
         if (bSaveDeskewedRaw) {
           if (!raw_deskewed_result) fail("RL_interface: raw_deskewed_result is null");
-          CDN_HIP(hipMemcpyAsync(raw_deskewed_result, X, vox() * sizeof(float), hipMemcpyDeviceToHost, st));
+          copy_to_host(st, raw_deskewed_result, X, vox() * sizeof(float));
           if (nIters == 0) {
             // the reference returns uninitialised memory in `result` here
-            CDN_HIP(hipMemcpyAsync(result, X, vox() * sizeof(float), hipMemcpyDeviceToHost, st));
-            CDN_HIP(hipStreamSynchronize(st));
+            copy_to_host(st, result, X, vox() * sizeof(float));
             return 1;
           }
         }
@@ -356,8 +356,7 @@ int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, f
       cx = (unsigned)nxo;
       cz = (unsigned)nzo;
     }
-    CDN_HIP(hipMemcpyAsync(result, X, vox() * sizeof(float), hipMemcpyDeviceToHost, st));
-    CDN_HIP(hipStreamSynchronize(st));
+    copy_to_host(st, result, X, vox() * sizeof(float));
     return 1;
   });
 }
@@ -393,10 +392,9 @@ int Deskew_interface(const float* const raw_data, int nx, int ny, int nz, float 
     try {
       din.ensure(sizeof(float) * (size_t)nx * ny * nz);
       dout.ensure(sizeof(float) * (size_t)xdim * ny * nz);
-      CDN_HIP(hipMemcpyAsync(din.p, raw_data, sizeof(float) * (size_t)nx * ny * nz, hipMemcpyHostToDevice, st));
+      copy_to_device(st, din.p, raw_data, sizeof(float) * (size_t)nx * ny * nz);
       deskew(st, din.as<float>(), nx, ny, nz, factor, dout.as<float>(), (int)xdim, extraShift, padVal);
-      CDN_HIP(hipMemcpyAsync(result, dout.p, sizeof(float) * (size_t)xdim * ny * nz, hipMemcpyDeviceToHost, st));
-      CDN_HIP(hipStreamSynchronize(st));
+      copy_to_host(st, result, dout.p, sizeof(float) * (size_t)xdim * ny * nz);
     } catch (...) {
       (void)hipStreamDestroy(st);
       throw;
@@ -416,10 +414,9 @@ static int affine_common(const float* raw_data, int nx, int ny, int nz, float* r
     try {
       din.ensure(sizeof(float) * n);
       dout.ensure(sizeof(float) * n);
-      CDN_HIP(hipMemcpyAsync(din.p, raw_data, sizeof(float) * n, hipMemcpyHostToDevice, st));
+      copy_to_device(st, din.p, raw_data, sizeof(float) * n);
       affine(st, din.as<float>(), nx, ny, nz, dout.as<float>(), affMat, ra, dx, dy, dz);
-      CDN_HIP(hipMemcpyAsync(result, dout.p, sizeof(float) * n, hipMemcpyDeviceToHost, st));
-      CDN_HIP(hipStreamSynchronize(st));
+      copy_to_host(st, result, dout.p, sizeof(float) * n);
     } catch (...) {
       (void)hipStreamDestroy(st);
       throw;
@@ -463,10 +460,9 @@ int camcor_interface(const unsigned short* const raw_data, int nx, int ny, int n
     const size_t n = (size_t)nx * ny * nz;
     C.in.ensure(sizeof(uint16_t) * n);
     C.out.ensure(sizeof(uint16_t) * n);
-    CDN_HIP(hipMemcpyAsync(C.in.p, raw_data, sizeof(uint16_t) * n, hipMemcpyHostToDevice, C.st));
+    copy_to_device(C.st, C.in.p, raw_data, sizeof(uint16_t) * n);
     camcor(C.st, C.in.as<uint16_t>(), C.cam.as<float>(), nx, ny, nz, C.out.as<uint16_t>());
-    CDN_HIP(hipMemcpyAsync(result, C.out.p, sizeof(uint16_t) * n, hipMemcpyDeviceToHost, C.st));
-    CDN_HIP(hipStreamSynchronize(C.st));
+    copy_to_host(C.st, result, C.out.p, sizeof(uint16_t) * n);
     return 1;
   });
 }
@@ -745,8 +741,7 @@ int cudaDecon_rl_iterate(cudaDecon_rl* h, int n, int k0, int seeded, double* lam
 int cudaDecon_rl_get(cudaDecon_rl* h, float* X) {
   return guard([&]() -> int {
     if (!h || !X) fail("cudaDecon_rl_get: null argument");
-    CDN_HIP(hipMemcpyAsync(X, h->eng.result(), sizeof(float) * h->eng.total_vox(), hipMemcpyDeviceToHost, h->st));
-    CDN_HIP(hipStreamSynchronize(h->st));
+    copy_to_host(h->st, X, h->eng.result(), sizeof(float) * h->eng.total_vox());
     return 1;
   });
 }
@@ -839,7 +834,7 @@ int cudaDecon_rl_set_raw_u16(cudaDecon_rl* h, const unsigned short* raw, float o
     if (!h || !raw) fail("cudaDecon_rl_set_raw_u16: null argument");
     const Fft3& F = h->eng.fft;
     h->u16.ensure(sizeof(uint16_t) * (size_t)F.nvox);
-    CDN_HIP(hipMemcpyAsync(h->u16.p, raw, sizeof(uint16_t) * (size_t)F.nvox, hipMemcpyHostToDevice, h->st));
+    copy_to_device(h->st, h->u16.p, raw, sizeof(uint16_t) * (size_t)F.nvox);
     u16_to_f32_crop(h->st, h->u16.as<uint16_t>(), F.nx, F.ny, F.nz, h->eng.raw.as<float>(), F.nx, F.ny, F.nz);
     bgsub(h->st, h->eng.raw.as<float>(), F.nvox, offset);  // src/RLgpuImpl.cu:124-132 with bg = offset
     h->eng.reset(h->st);
@@ -912,8 +907,7 @@ int cudaDecon_rl_post_dims(cudaDecon_rl* h, int* nx, int* ny, int* nz) {
 int cudaDecon_rl_post_get(cudaDecon_rl* h, float* out) {
   return guard([&]() -> int {
     if (!h || !out || h->post_cur < 0) fail("cudaDecon_rl_post_get: no deskew or affine result");
-    CDN_HIP(hipMemcpy(out, h->post[h->post_cur].p, sizeof(float) * (size_t)h->pnx * h->pny * h->pnz,
-                      hipMemcpyDeviceToHost));
+    copy_to_host(h->st, out, h->post[h->post_cur].p, sizeof(float) * (size_t)h->pnx * h->pny * h->pnz);
     return 1;
   });
 }

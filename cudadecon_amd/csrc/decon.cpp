@@ -2,6 +2,7 @@
 // src/RL-Biggs-Andrews.cpp:211-367 as 9 fused passes + a device-side
 // acceleration factor; no per-iteration host synchronisation).
 #include "decon.hpp"
+#include "hostcopy.hpp"
 
 #include "rl_kernels.hpp"
 
@@ -190,8 +191,10 @@ void RLEngine::setup(int nx, int ny, int nz, const OtfTable& t, const float ksca
 
 void RLEngine::load_raw(const float* in, bool on_device, hipStream_t st) {
   const size_t vb = sizeof(float) * (size_t)total_vox();
-  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  CDN_HIP(hipMemcpyAsync(raw.p, in, vb, k, st));
+  if (on_device)
+    CDN_HIP(hipMemcpyAsync(raw.p, in, vb, hipMemcpyDeviceToDevice, st));
+  else
+    copy_to_device(st, raw.p, in, vb);
   CDN_HIP(hipMemcpyAsync(xa.p, raw.p, vb, hipMemcpyDeviceToDevice, st));
   cur = xa.as<float>();
   prev = xb.as<float>();
@@ -200,8 +203,10 @@ void RLEngine::load_raw(const float* in, bool on_device, hipStream_t st) {
 void RLEngine::load_stack(int s, const float* in, bool on_device, hipStream_t st) {
   if (s < 0 || s >= nstacks) fail("stack index out of range");
   const size_t vb = sizeof(float) * (size_t)fft.nvox, off = (size_t)fft.nvox * s;
-  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  CDN_HIP(hipMemcpyAsync(raw.as<float>() + off, in, vb, k, st));
+  if (on_device)
+    CDN_HIP(hipMemcpyAsync(raw.as<float>() + off, in, vb, hipMemcpyDeviceToDevice, st));
+  else
+    copy_to_device(st, raw.as<float>() + off, in, vb);
   CDN_HIP(hipMemcpyAsync(cur + off, raw.as<float>() + off, vb, hipMemcpyDeviceToDevice, st));  // its X_0
 }
 

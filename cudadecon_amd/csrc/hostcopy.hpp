@@ -1,0 +1,24 @@
+// Copies between caller-owned (pageable) host buffers and the device.
+//
+// Measured on the MI355X box for 1 GiB: pageable hipMemcpy 26 GB/s to the
+// device and 17 GB/s back, pinned 57 GB/s both ways, one host thread's
+// memcpy 29 GB/s.  Large copies therefore go through two pinned staging
+// buffers: the DMA of one chunk overlaps several host threads copying the
+// other chunk between the caller's memory and the pinned buffer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace cdn {
+
+// host -> device.  Returns once the host buffer has been read; the last DMAs
+// may still be in flight, ordered on `st` before any later work there.
+void copy_to_device(hipStream_t st, void* dev, const void* host, size_t bytes);
+
+// device -> host, after the work already queued on `st`.  Returns once the
+// data is in the host buffer.
+void copy_to_host(hipStream_t st, void* host, const void* dev, size_t bytes);
+
+}  // namespace cdn
