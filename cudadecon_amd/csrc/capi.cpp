@@ -484,7 +484,7 @@ int cudaDecon_fft_r2c(const float* in, int nx, int ny, int nz, float* out) {
       F.setup(nx, ny, nz, st);
       dv.ensure(sizeof(float) * F.nvox);
       ds.ensure(sizeof(float2) * F.nspec);
-      CDN_HIP(hipMemcpyAsync(dv.p, in, sizeof(float) * F.nvox, hipMemcpyHostToDevice, st));
+      copy_to_device(st, dv.p, in, sizeof(float) * F.nvox);
       F.r2c(st, dv.as<float>(), ds.as<float2>());
       CDN_HIP(hipMemcpy2DAsync(out, sizeof(float2) * F.nxh, ds.p, sizeof(float2) * F.pitch, sizeof(float2) * F.nxh,
                                (size_t)ny * nz, hipMemcpyDeviceToHost, st));
@@ -510,8 +510,7 @@ int cudaDecon_fft_c2r(const float* in, int nx, int ny, int nz, float* out) {
       CDN_HIP(hipMemcpy2DAsync(ds.p, sizeof(float2) * F.pitch, in, sizeof(float2) * F.nxh, sizeof(float2) * F.nxh,
                                (size_t)ny * nz, hipMemcpyHostToDevice, st));
       F.c2r(st, ds.as<float2>(), dv.as<float>());
-      CDN_HIP(hipMemcpyAsync(out, dv.p, sizeof(float) * F.nvox, hipMemcpyDeviceToHost, st));
-      CDN_HIP(hipStreamSynchronize(st));
+      copy_to_host(st, out, dv.p, sizeof(float) * F.nvox);
     } catch (...) {
       (void)hipStreamDestroy(st);
       throw;
@@ -536,7 +535,7 @@ int cudaDecon_filter(float* vol, int nx, int ny, int nz, const float* otf, int o
       DevBuf dv, ds;
       dv.ensure(sizeof(float) * F.nvox);
       ds.ensure(sizeof(float2) * F.nspec);
-      CDN_HIP(hipMemcpyAsync(dv.p, vol, sizeof(float) * F.nvox, hipMemcpyHostToDevice, st));
+      copy_to_device(st, dv.p, vol, sizeof(float) * F.nvox);
       float2* S = ds.as<float2>();
       launch_x_r2c(nx, st, dv.as<float>(), nullptr, nullptr, nullptr, XIN_PLAIN, S, (long)ny * nz, F.pitch,
                    F.twx.as<float2>());
@@ -545,8 +544,7 @@ int cudaDecon_filter(float* vol, int nx, int ny, int nz, const float* otf, int o
                       F.twz.as<float2>());
       F.col(st, S, 1, true);
       launch_x_c2r(nx, st, S, dv.as<float>(), (long)ny * nz, F.pitch, F.twx.as<float2>());
-      CDN_HIP(hipMemcpyAsync(vol, dv.p, sizeof(float) * F.nvox, hipMemcpyDeviceToHost, st));
-      CDN_HIP(hipStreamSynchronize(st));
+      copy_to_host(st, vol, dv.p, sizeof(float) * F.nvox);
     } catch (...) {
       (void)hipStreamDestroy(st);
       throw;
@@ -575,7 +573,7 @@ int cudaDecon_wiener(float* vol, int nx, int ny, int nz, const float* otf, int o
       DevBuf dv, ds;
       dv.ensure(sizeof(float) * F.nvox);
       ds.ensure(sizeof(float2) * F.nspec);
-      CDN_HIP(hipMemcpyAsync(dv.p, vol, sizeof(float) * F.nvox, hipMemcpyHostToDevice, st));
+      copy_to_device(st, dv.p, vol, sizeof(float) * F.nvox);
       if (background != 0.f) add_scalar(st, dv.as<float>(), F.nvox, -background);
       float2* S = ds.as<float2>();
       launch_x_r2c(nx, st, dv.as<float>(), nullptr, nullptr, nullptr, XIN_PLAIN, S, (long)ny * nz, F.pitch,
@@ -587,8 +585,7 @@ int cudaDecon_wiener(float* vol, int nx, int ny, int nz, const float* otf, int o
       F.col(st, S, 2, true);
       F.col(st, S, 1, true);
       launch_x_c2r(nx, st, S, dv.as<float>(), (long)ny * nz, F.pitch, F.twx.as<float2>());
-      CDN_HIP(hipMemcpyAsync(vol, dv.p, sizeof(float) * F.nvox, hipMemcpyDeviceToHost, st));
-      CDN_HIP(hipStreamSynchronize(st));
+      copy_to_host(st, vol, dv.p, sizeof(float) * F.nvox);
     } catch (...) {
       (void)hipStreamDestroy(st);
       throw;
@@ -680,9 +677,7 @@ int cudaDecon_rl_get_stack(cudaDecon_rl* h, int s, float* X) {
   return guard([&]() -> int {
     if (!h || !X) fail("cudaDecon_rl_get_stack: null argument");
     if (s < 0 || s >= h->eng.nstacks) fail("cudaDecon_rl_get_stack: stack index out of range");
-    CDN_HIP(hipMemcpyAsync(X, h->eng.result() + (size_t)h->eng.fft.nvox * s, sizeof(float) * h->eng.fft.nvox,
-                           hipMemcpyDeviceToHost, h->st));
-    CDN_HIP(hipStreamSynchronize(h->st));
+    copy_to_host(h->st, X, h->eng.result() + (size_t)h->eng.fft.nvox * s, sizeof(float) * h->eng.fft.nvox);
     return 1;
   });
 }
@@ -1073,10 +1068,10 @@ void stage_batch(cudaDecon_net* h, const float* raw, const float* tgt, int nb, i
   h->net.geometry(nb, nz, ny, nx, training);
   const size_t n = (size_t)nb * nz * ny * nx;
   h->net.b_raw.ensure(sizeof(float) * n);
-  CDN_HIP(hipMemcpyAsync(h->net.b_raw.p, raw, sizeof(float) * n, hipMemcpyHostToDevice, h->st));
+  copy_to_device(h->st, h->net.b_raw.p, raw, sizeof(float) * n);
   if (tgt) {
     h->net.b_tgt.ensure(sizeof(float) * n);
-    CDN_HIP(hipMemcpyAsync(h->net.b_tgt.p, tgt, sizeof(float) * n, hipMemcpyHostToDevice, h->st));
+    copy_to_device(h->st, h->net.b_tgt.p, tgt, sizeof(float) * n);
   }
 }
 }  // namespace
@@ -1090,8 +1085,7 @@ int cudaDecon_net_forward(cudaDecon_net* h, const float* raw, int nb, int nz, in
     o.ensure(sizeof(float) * n);
     h->net.forward(h->st, h->net.b_raw.as<float>(), mode < 2);
     h->net.head(h->st, h->net.b_raw.as<float>(), mode & 1, o.as<float>());
-    CDN_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->st));
-    CDN_HIP(hipStreamSynchronize(h->st));
+    copy_to_host(h->st, out, o.p, sizeof(float) * n);
     return 1;
   });
 }
