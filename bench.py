@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--no-pipeline", action="store_true", help="skip the 512x2048x2048 pipeline")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU; --gpus must be 1 "
+        print("bench.py: this benchmark runs on exactly one GPU; --gpus must be 1 "
               "(the multi-GPU slab decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))

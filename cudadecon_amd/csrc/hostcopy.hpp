@@ -1,6 +1,6 @@
 // Copies between caller-owned (pageable) host buffers and the device.
 //
-// Measured on the MI355X box for 1 GiB: pageable hipMemcpy 26 GB/s to the
+// Measured on an MI355X host for 1 GiB: pageable hipMemcpy 26 GB/s to the
 // device and 17 GB/s back, pinned 57 GB/s both ways, one host thread's
 // memcpy 29 GB/s.  Large copies therefore go through two pinned staging
 // buffers: the DMA of one chunk overlaps several host threads copying the
