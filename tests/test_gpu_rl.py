@@ -218,6 +218,27 @@ def test_deskew_interface(cdn, orc):
         assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
+def test_deskew_large_buffers_round_trip_exactly(cdn):
+    """Host buffers of 16 MB and more cross through two 32 MB pinned staging
+    buffers (src/RL-Biggs-Andrews.cpp:724-773 copies them with cudaMemcpy).
+    A zero-angle deskew is a copy with the last column padded (xin = x +
+    extraShift exactly, interpolation weight 0), so the result is bitwise
+    the input: 72 MB each way is three chunks, the last partial, which takes
+    both pipelines through buffer reuse.  A small call in between goes
+    straight through the runtime; the large call after it reuses the
+    staging buffers."""
+    rng = np.random.default_rng(21)
+    big = rng.uniform(-1000, 1000, (67, 263, 1021)).astype(np.float32)
+    small = rng.uniform(-1000, 1000, (5, 31, 77)).astype(np.float32)
+    assert big.nbytes > 2 * (32 << 20) and big.nbytes % (32 << 20) != 0
+    for im, shift in [(big, 0), (small, 0), (big, 2)]:
+        nx = im.shape[2]
+        got = cdn.deskew(im, 0.0, 0.3, 0.104, shift=shift, pad_val=-7.0)
+        assert got.shape == im.shape
+        assert np.array_equal(got[..., : nx - 1 - shift], im[..., shift : nx - 1])
+        assert np.all(got[..., nx - 1 - shift :] == np.float32(-7.0))
+
+
 def test_affine_interfaces(cdn, orc):
     rng = np.random.default_rng(12)
     im = rng.uniform(0, 1000, (24, 48, 64)).astype(np.float32)
