@@ -2,13 +2,11 @@
 #include "hostcopy.hpp"
 
 #include "common.hpp"
+#include "copypool.hpp"
 
 #include <algorithm>
-#include <cstring>
 #include <map>
 #include <mutex>
-#include <thread>
-#include <vector>
 
 namespace cdn {
 namespace {
@@ -16,29 +14,9 @@ namespace {
 constexpr size_t kChunk = 32UL << 20;   // bytes per staging buffer
 constexpr size_t kDirect = 16UL << 20;  // smaller copies go straight through the runtime
 
-// memcpy split over host threads (slices of at least 2 MB, at most 8 threads)
-void par_memcpy(void* dst, const void* src, size_t n) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>({8, hw, std::max<size_t>(1, n / (2UL << 20))});
-  if (nt <= 1) {
-    std::memcpy(dst, src, n);
-    return;
-  }
-  const size_t per = (n + nt - 1) / nt;
-  std::vector<std::thread> th;
-  th.reserve(nt - 1);
-  for (size_t t = 1; t < nt; ++t) {
-    const size_t a = t * per;
-    if (a >= n) break;
-    const size_t len = std::min(per, n - a);
-    th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, len); });
-  }
-  std::memcpy(dst, src, std::min(per, n));
-  for (auto& t : th) t.join();
-}
-
 struct Stage {
   std::mutex mu;
+  CopyPool* pool = nullptr;
   void* buf[2] = {nullptr, nullptr};      // pinned, portable across devices
   std::map<int, hipEvent_t> ev[2];        // last DMA through buf[i], per device
   // the event for buf[i] on the device of stream st (created there)
@@ -60,6 +38,7 @@ struct Stage {
   void ensure() {
     for (int i = 0; i < 2; ++i)
       if (!buf[i]) CDN_HIP(hipHostMalloc(&buf[i], kChunk, hipHostMallocPortable));
+    if (!pool) pool = new CopyPool;
   }
   // every DMA through the staging buffers, on any device, has finished
   void drain() {
@@ -91,7 +70,7 @@ void copy_to_device(hipStream_t st, void* dev, const void* host, size_t bytes) {
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
     hipEvent_t e = S.event(b, st);
     if (i >= 2) CDN_HIP(hipEventSynchronize(e));  // the DMA of chunk i - 2 left buf[b]
-    par_memcpy(S.buf[b], static_cast<const char*>(host) + off, len);
+    S.pool->copy(S.buf[b], static_cast<const char*>(host) + off, len);
     CDN_HIP(hipMemcpyAsync(static_cast<char*>(dev) + off, S.buf[b], len, hipMemcpyHostToDevice, st));
     CDN_HIP(hipEventRecord(e, st));
   }
@@ -121,7 +100,7 @@ void copy_to_host(hipStream_t st, void* host, const void* dev, size_t bytes) {
     const int b = (int)(i & 1);
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
     CDN_HIP(hipEventSynchronize(S.event(b, st)));
-    par_memcpy(static_cast<char*>(host) + off, S.buf[b], len);
+    S.pool->copy(static_cast<char*>(host) + off, S.buf[b], len);
   }
 }
 
