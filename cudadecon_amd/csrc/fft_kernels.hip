@@ -107,7 +107,7 @@ void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float
                   float2* S, long rows, long pitch, const float2* tw, long stack_rows) {
   FftVariant tv;
   const bool tuned = fft_get_variant(nx, &tv);
-  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1, XK_R2C);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
   if (stack_rows <= 0) stack_rows = rows;
   if (stack_rows != rows && stack_rows % 2) fail("launch_x_r2c: rows per stack must be even");
@@ -156,7 +156,7 @@ void launch_x_lrcore(int nx, hipStream_t st, float2* S, const float* raw, long r
                      float eps) {
   FftVariant tv;
   const bool tuned = fft_get_variant(nx, &tv);
-  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1, true);
+  const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1, XK_LR);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
   void* args[] = {&S, &raw, &rows, &pitch, &tw, &eps};
   if (tuned) {

@@ -120,8 +120,22 @@ constexpr int tpt_x_short(int N, bool lr) {
     if (N % d == 0) return d;
   return t;
 }
-constexpr int tpt_x(int N, bool lr = false) {
+// Kinds of x pass with their own line shape: the inverse and update passes,
+// the LRcore pass (two transforms per line), the R2C pass with the prediction.
+enum XKind : int { XK_PLAIN = 0, XK_LR = 1, XK_R2C = 2 };
+// Threads per line of the R2C pass where they differ from the other x
+// passes (pairs length, threads).  1024-point lines with 256 threads (4
+// elements per thread, all-LDS transform) took the R2C pass from 0.742 to
+// 0.710 ms at 256 x 1024 x 1024; the same shape made LRcore slower (0.60 ->
+// 0.76 ms) and the update no faster, so they keep theirs.
+#ifndef CDN_R2C_TPT
+#define CDN_R2C_TPT 1024, 256
+#endif
+constexpr int kR2cTpt[] = {CDN_R2C_TPT};
+constexpr int tpt_x(int N, int kind = XK_PLAIN) {
   if (lookup_override(kXTptOverride, N) > 0) return lookup_override(kXTptOverride, N);
+  if (kind == XK_R2C && lookup_override(kR2cTpt, N) > 0) return lookup_override(kR2cTpt, N);
+  const bool lr = kind == XK_LR;
   const int t = tpt_x_short(N, lr);
   if (CDN_XTPT_RULE < 3 || N / t <= (lr ? 24 : 16)) return t;
   const int emax = lr ? 15 : 6;
@@ -151,9 +165,9 @@ struct XShape {
   bool SHUFFLE;
 };
 // xthreads > 0 forces the workgroup size (the build-time override otherwise)
-constexpr XShape x_shape(int N, int xthreads = -1, bool lr = false) {
+constexpr XShape x_shape(int N, int xthreads = -1, int kind = XK_PLAIN) {
   XShape s{};
-  s.TPT = tpt_x(N, lr);
+  s.TPT = tpt_x(N, kind);
   s.E = N / s.TPT;
   s.SHUFFLE = plan_exact(N, s.TPT) && s.TPT <= 64 && (64 % s.TPT) == 0 && (s.E % 2) == 0;
   const int ov = xthreads >= 0 ? xthreads : lookup_override(kXThreadsOverride, N);
@@ -182,8 +196,8 @@ constexpr CShape c_shape(int N, int cols = -1) {
   return s;
 }
 
-template <int N, bool LR = false> struct XCfg {
-  static constexpr XShape S = x_shape(N, -1, LR);
+template <int N, int K = XK_PLAIN> struct XCfg {
+  static constexpr XShape S = x_shape(N, -1, K);
   static constexpr int TPT = S.TPT, E = S.E, NL = S.NL, PAD = S.PAD, LS = S.LS, THREADS = S.THREADS, EH = S.EH;
   static constexpr bool SHUFFLE = S.SHUFFLE;
 };
@@ -223,10 +237,10 @@ __device__ __forceinline__ float2 shfl2(float2 v, int src) {
 
 // Half spectra A (row r0) and B (row r0+1) from the packed transform Z:
 // A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
-template <int N, bool LR = false>
-__device__ __forceinline__ void split_store(float2 (&v)[(XCfg<N, LR>::E)], float2* line, int p,
+template <int N, int K = XK_PLAIN>
+__device__ __forceinline__ void split_store(float2 (&v)[(XCfg<N, K>::E)], float2* line, int p,
                                             float2* __restrict__ S, long r0, long pitch, bool v0, bool v1) {
-  using C = XCfg<N, LR>;
+  using C = XCfg<N, K>;
   constexpr int E = C::E, TPT = C::TPT;
   auto put = [&](int k, float2 zk, float2 zn) {
     const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
@@ -259,10 +273,10 @@ __device__ __forceinline__ void split_store(float2 (&v)[(XCfg<N, LR>::E)], float
 
 // Packed line Z = A + i B (Hermitian extension of both rows; imaginary parts
 // of the kx = 0 and kx = N/2 bins ignored) into v.
-template <int N, bool LR = false>
-__device__ __forceinline__ void merge_load(float2 (&v)[(XCfg<N, LR>::E)], float2* line, int p,
+template <int N, int K = XK_PLAIN>
+__device__ __forceinline__ void merge_load(float2 (&v)[(XCfg<N, K>::E)], float2* line, int p,
                                            const float2* __restrict__ S, long r0, long pitch, bool v0, bool v1) {
-  using C = XCfg<N, LR>;
+  using C = XCfg<N, K>;
   constexpr int E = C::E, TPT = C::TPT;
   auto ld = [&](int k, float2& A, float2& B) {
     A = v0 ? ld_stream<NTX>(S + r0 * pitch + k) : make_float2(0.f, 0.f);
@@ -315,11 +329,11 @@ __device__ __forceinline__ void merge_load(float2 (&v)[(XCfg<N, LR>::E)], float2
 }
 
 template <int N>
-__global__ void __launch_bounds__(XCfg<N>::THREADS)
+__global__ void __launch_bounds__((XCfg<N, XK_R2C>::THREADS))
     k_x_r2c(const float* __restrict__ X, const float* __restrict__ Xp, float* __restrict__ Y,
             const double* __restrict__ lam, int mode, float2* __restrict__ S, long rows, long pitch,
             long stack_rows, const float2* __restrict__ tw) {
-  using C = XCfg<N>;
+  using C = XCfg<N, XK_R2C>;
   __shared__ float2 lds[C::NL * C::LS];
   const int tid = threadIdx.x;
   const int l = tid / C::TPT, p = tid % C::TPT;
@@ -362,7 +376,7 @@ __global__ void __launch_bounds__(XCfg<N>::THREADS)
   for (int m = 0; m < C::E; ++m) v[m] = make_float2(a[m], b[m]);
   fft_line<N, C::TPT, 1, false>(v, line, p, tw);
   __syncthreads();
-  split_store<N>(v, line, p, S, r0, pitch, v0, v1);
+  split_store<N, XK_R2C>(v, line, p, S, r0, pitch, v0, v1);
 }
 
 template <int N>
@@ -397,10 +411,10 @@ __device__ __forceinline__ float lrcore(float c, float raw, float eps) {
 }
 
 template <int N>
-__global__ void __launch_bounds__((XCfg<N, true>::THREADS))
+__global__ void __launch_bounds__((XCfg<N, XK_LR>::THREADS))
     k_x_lrcore(float2* __restrict__ S, const float* __restrict__ raw, long rows, long pitch,
                const float2* __restrict__ tw, float eps) {
-  using C = XCfg<N, true>;
+  using C = XCfg<N, XK_LR>;
   __shared__ float2 lds[C::NL * C::LS];
   const int tid = threadIdx.x;
   const int l = tid / C::TPT, p = tid % C::TPT;
@@ -408,7 +422,7 @@ __global__ void __launch_bounds__((XCfg<N, true>::THREADS))
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
   float2 v[C::E];
-  merge_load<N, true>(v, line, p, S, r0, pitch, v0, v1);
+  merge_load<N, XK_LR>(v, line, p, S, r0, pitch, v0, v1);
   fft_line<N, C::TPT, 1, true>(v, line, p, tw);
 #pragma unroll
   for (int m = 0; m < C::E; ++m) {
@@ -420,7 +434,7 @@ __global__ void __launch_bounds__((XCfg<N, true>::THREADS))
   __syncthreads();
   fft_line<N, C::TPT, 1, false>(v, line, p, tw);
   __syncthreads();
-  split_store<N, true>(v, line, p, S, r0, pitch, v0, v1);
+  split_store<N, XK_LR>(v, line, p, S, r0, pitch, v0, v1);
 }
 
 // Deterministic workgroup sum of two doubles; out[0], out[1] written by
