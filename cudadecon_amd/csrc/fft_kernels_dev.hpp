@@ -124,12 +124,16 @@ constexpr int tpt_x_short(int N, bool lr) {
 // the LRcore pass (two transforms per line), the R2C pass with the prediction.
 enum XKind : int { XK_PLAIN = 0, XK_LR = 1, XK_R2C = 2 };
 // Threads per line of the R2C pass where they differ from the other x
-// passes (pairs length, threads).  1024-point lines with 256 threads (4
-// elements per thread, all-LDS transform) took the R2C pass from 0.742 to
-// 0.710 ms at 256 x 1024 x 1024; the same shape made LRcore slower (0.60 ->
-// 0.76 ms) and the update no faster, so they keep theirs.
+// passes (pairs length, threads).  More threads per line, fewer elements
+// each (all-LDS transform): 1024 points with 256 threads took the R2C pass
+// from 0.742 to 0.712 ms at 256 x 1024 x 1024 (the same shape made LRcore
+// slower, 0.60 -> 0.76 ms, and the update no faster, so they keep theirs);
+// 512 with 128 from 0.098-0.100 to 0.094-0.096 ms at 128 x 512 x 512 (three
+// alternating runs each; 64 threads: 0.098).  2048-point lines with 256 or
+// 512 threads measured like the 128 they keep (5.4-6.2 ms either way at
+// 512 x 2048 x 2048).
 #ifndef CDN_R2C_TPT
-#define CDN_R2C_TPT 1024, 256
+#define CDN_R2C_TPT 512, 128, 1024, 256
 #endif
 constexpr int kR2cTpt[] = {CDN_R2C_TPT};
 constexpr int tpt_x(int N, int kind = XK_PLAIN) {
