@@ -132,6 +132,17 @@ void OtfDev::setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t
     }
     tz.ensure(sizeof(float2) * h.size());
     CDN_HIP(hipMemcpyAsync(tz.p, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, st));
+    // the two rows a radius interpolates between, side by side
+    if (nr >= 2) {
+      std::vector<float4> h2((size_t)(nr - 1) * nz);
+      for (int r = 0; r + 1 < nr; ++r)
+        for (int kz = 0; kz < nz; ++kz) {
+          const float2 a = h[(size_t)r * nz + kz], b = h[(size_t)(r + 1) * nz + kz];
+          h2[(size_t)r * nz + kz] = make_float4(a.x, a.y, b.x, b.y);
+        }
+      tz2.ensure(sizeof(float4) * h2.size());
+      CDN_HIP(hipMemcpyAsync(tz2.p, h2.data(), sizeof(float4) * h2.size(), hipMemcpyHostToDevice, st));
+    }
   } else {
     raw.ensure(sizeof(float) * t.data.size());
     CDN_HIP(hipMemcpyAsync(raw.p, t.data.data(), sizeof(float) * t.data.size(), hipMemcpyHostToDevice, st));
@@ -143,6 +154,7 @@ OtfParams OtfDev::params(const OtfTable& t, double inv_n) const {
   OtfParams p{};
   p.ra = ra;
   p.ra_tab.tz = tz.as<float2>();
+  p.ra_tab.tz2 = tz2.as<float4>();
   p.ra_tab.nr = t.nxotf;
   p.tab3.raw = raw.as<float2>();
   p.tab3.nxotf = t.nxotf;

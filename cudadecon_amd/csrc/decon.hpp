@@ -49,7 +49,7 @@ struct OtfTable {
 struct OtfDev {
   int ra = 1, nz_for = -1;
   float ks[3] = {0, 0, 0};
-  DevBuf raw, tz;
+  DevBuf raw, tz, tz2;
   void setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st);
   OtfParams params(const OtfTable& t, double inv_n) const;
 };

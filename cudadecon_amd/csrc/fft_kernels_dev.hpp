@@ -689,8 +689,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
     const bool inside = kr < (float)(otf.ra_tab.nr - 1);
     const int ir = inside ? (int)floorf(kr) : 0;
     const float ar = kr - (float)ir;
-    const float2* t0 = otf.ra_tab.tz + (long)ir * N;
-    const float2* t1 = t0 + N;
+    const float4* t01 = otf.ra_tab.tz2 + (long)ir * N;
     const int rr = ir - r_lo;
     // one source per wavefront: the staged rows when they hold every row
     // the wavefront's lanes interpolate between
@@ -716,8 +715,8 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
         const int i = p + m * C::TPT;
         float2 r = make_float2(0.f, 0.f);
         if (inside) {
-          const float2 a = t0[i], b = t1[i];
-          float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
+          const float4 ab = t01[i];
+          float2 h = make_float2((1.f - ar) * ab.x + ar * ab.z, (1.f - ar) * ab.y + ar * ab.w);
           h.y *= sg;
           r = cscale(cmul(h, v[m]), otf.inv_n);
         }

@@ -16,8 +16,9 @@ enum XInMode : int {
 };
 
 struct RaOtf {
-  const float2* tz;  // z-interpolated radial table, (nr, nz) kz fastest
-  int nr;            // radial samples (nxotf); valid iff kr < nr - 1
+  const float2* tz;   // z-interpolated radial table, (nr, nz) kz fastest
+  const float4* tz2;  // rows r and r + 1 side by side, (nr - 1, nz): one load per lookup
+  int nr;             // radial samples (nxotf); valid iff kr < nr - 1
 };
 
 struct Otf3d {
