@@ -58,6 +58,7 @@ SIGNATURES = {
     "cudaDecon_last_error": (C.c_char_p, []),
     "cudaDecon_fft_supported": (C.c_int, [C.c_int]),
     "cudaDecon_find_optimal_dimension": (C.c_uint, [C.c_uint, C.c_int]),
+    "cudaDecon_deskew_width": (C.c_uint, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_int]),
     "cudaDecon_get_session_dims": (C.c_uint, [C.c_int]),
     "cudaDecon_get_session_kscale": (C.c_int, [_f]),
     "cudaDecon_fft_r2c": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f]),
@@ -210,16 +211,17 @@ def rl_cleanup():
 
 
 def deskew(im, angle, dz, dx, width=0, shift=0, pad_val=0.0, out_width=None):
-    """Deskew_interface on a float (nz, ny, nx) stack.  out_width must be
-    given when width == 0 and angle != 0 (the library then sizes the output
-    from its session globals, src/RL-Biggs-Andrews.cpp:736-744); with a zero
-    angle the output keeps the input width."""
+    """Deskew_interface on a float (nz, ny, nx) stack.  The result's width is
+    the one the library writes (cudaDecon_deskew_width: `width`, or with
+    width == 0 the reference's rule from the session's output nz,
+    src/RL-Biggs-Andrews.cpp:736-744); out_width, if given, must equal it."""
     im = _f32(im)
     nz, ny, nx = im.shape
-    if not abs(float(np.float32(angle))) > 0.0:
-        w = nx
-    else:
-        w = int(width) if width else int(out_width)
+    w = int(lib().cudaDecon_deskew_width(nx, float(dz), float(dx), float(angle), int(width)))
+    if out_width is not None and int(out_width) != w:
+        raise ValueError(f"deskew: out_width {out_width} differs from the width Deskew_interface writes ({w})")
+    if w <= 0:
+        raise ValueError("deskew: zero output width")
     out = np.empty((nz, ny, w), np.float32)
     _check(lib().Deskew_interface(im, nx, ny, nz, float(dz), float(dx), float(angle), out, int(width), int(shift),
                                   float(pad_val)), "Deskew_interface")

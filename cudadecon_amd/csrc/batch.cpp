@@ -49,14 +49,19 @@ void Batch::enqueue(hipStream_t st, int i, int iters) {
 }
 
 void Batch::capture(int iters) {
-  if (exec && graph_iters == iters) return;
+  for (auto& e : engines) e->prepare(iters);  // no allocation while capturing
+  // a graph replays the addresses it was captured with: reuse it only while
+  // no engine buffer has moved (a longer mode 0/1 run regrows lam, say)
+  std::vector<std::vector<const void*>> addr;
+  for (auto& e : engines) addr.push_back(e->device_addresses());
+  if (exec && graph_iters == iters && addr == graph_addr) return;
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   exec = nullptr;
   graph = nullptr;
+  graph_iters = -1;
   const int S = (int)streams.size() - 1;
   hipStream_t root = streams[S];
-  for (auto& e : engines) e->prepare(iters);  // no allocation while capturing
   CDN_HIP(hipDeviceSynchronize());
   CDN_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
   CDN_HIP(hipEventRecord(events[0], root));
@@ -69,6 +74,11 @@ void Batch::capture(int iters) {
   CDN_HIP(hipStreamEndCapture(root, &graph));
   CDN_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   graph_iters = iters;
+  graph_addr = std::move(addr);
+  // capturing ran each engine's host bookkeeping of the schedule: its X_k and
+  // X_{k-1} pointers are now where a replay leaves them
+  graph_post.clear();
+  for (auto& e : engines) graph_post.emplace_back(e->cur, e->prev);
 }
 
 double Batch::run(int iters, int mode) {
@@ -93,6 +103,10 @@ double Batch::run(int iters, int mode) {
     }
   } else {
     CDN_HIP(hipGraphLaunch(exec, root));
+    for (size_t i = 0; i < engines.size(); ++i) {
+      engines[i]->cur = graph_post[i].first;
+      engines[i]->prev = graph_post[i].second;
+    }
   }
   CDN_HIP(hipEventRecord(t1, root));
   CDN_HIP(hipEventSynchronize(t1));

@@ -15,6 +15,10 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_iters = -1;
+  // what the captured graph baked in: every engine's device addresses, and
+  // each engine's X_k / X_{k-1} pointers after a replay
+  std::vector<std::vector<const void*>> graph_addr;
+  std::vector<std::pair<float*, float*>> graph_post;
 
   ~Batch();
   // dims_zyx: 3 ints per stack; kscales: 3 floats per stack

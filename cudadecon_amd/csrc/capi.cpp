@@ -8,6 +8,7 @@
 
 #include "batch.hpp"
 #include "decon.hpp"
+#include "geometry.hpp"
 #include "hostcopy.hpp"
 #include "net.hpp"
 #include "rl_kernels.hpp"
@@ -27,36 +28,9 @@ namespace {
 
 thread_local std::string g_err;
 
-// findOptimalDimension / notGoodDimension (src/RL-Biggs-Andrews.cpp:18-43);
-// n <= 1 is returned as is (the reference recurses forever at 1).
-bool not_good(unsigned n) {
-  while (true) {
-    if (n <= 1) return false;
-    if (n == 2 || n == 3 || n == 5 || n == 7) return false;
-    if (n % 2 == 0) { n /= 2; continue; }
-    if (n % 3 == 0) { n /= 3; continue; }
-    if (n % 5 == 0) { n /= 5; continue; }
-    if (n % 7 == 0) { n /= 7; continue; }
-    return true;
-  }
-}
-unsigned find_optimal_dimension(unsigned n, int step = -1) {
-  if (n <= 1) return n;
-  unsigned o = n;
-  while (not_good(o)) o += step;
-  return o;
-}
-
-struct Session {
+// the session's sizes and matrices (SessionGeometry) plus its device state
+struct Session : SessionGeometry {
   hipStream_t st = nullptr;
-  unsigned output_nx = 0, output_ny = 0, output_nz = 0;
-  bool crop = false;
-  double deskewFactor = 0.0;
-  unsigned deskewedXdim = 0;
-  bool rot_on = false;
-  float rot[4] = {0, 0, 0, 0};
-  float dr = 0, dz = 0;  // dz after the |sin| factor when deskewing
-  unsigned fft_nx = 0;   // x length of the iteration grid
   OtfTable otf;
   RLEngine eng;
   DevBuf u16, bufA, bufB, red_part, red_acc, red_mean;
@@ -79,21 +53,7 @@ hipStream_t make_stream() {
   return s;
 }
 
-// k-scales for an FFT grid of x-length fft_nx (src/RL-Biggs-Andrews.cpp:634-642)
-void kscales(const Session& S, unsigned fft_nx, float ks[3]) {
-  const float dkx = (float)(1.0 / (S.dr * (float)fft_nx));
-  const float dky = (float)(1.0 / (S.dr * (float)S.output_ny));
-  const float dkz = (float)(1.0 / (S.dz * (float)S.output_nz));
-  ks[0] = dkx / S.otf.dkx;
-  ks[1] = dky / S.otf.dky;
-  ks[2] = dkz / S.otf.dkz;
-}
-
-// rotated sizes (src/RL-Biggs-Andrews.cpp:410-411), float arithmetic
-void rotated_dims(const float* p, unsigned nx, unsigned nz, int* nx_o, int* nz_o) {
-  *nz_o = (int)((float)nz * p[3] / p[0]);
-  *nx_o = (int)((float)nx * p[3] + (float)nz * p[2] * p[2] / std::fabs(p[1]));
-}
+void kscales(const Session& S, unsigned fft_nx, float ks[3]) { session_kscales(S, S.otf, fft_nx, ks); }
 
 // camera-correction parameters of camcor_interface_init (process global, as
 // the reference's intended context, src/llspy.cpp:70-77)
@@ -119,6 +79,15 @@ template <class F> int guard(F&& f) {
   return 0;
 }
 
+// A NaN or Inf anywhere in a stack spreads through the FFT to every voxel of
+// the result, so float inputs of the RL path are checked on the device
+// (one read of the staged stack) and rejected with a message.
+void check_finite(hipStream_t st, const float* dev, long n, DevBuf& red, const std::string& what) {
+  red.ensure(sizeof(double) * 3 * 1024);
+  const long bad = count_nonfinite(st, dev, n, red.as<double>());
+  if (bad > 0) fail(what + ": " + std::to_string(bad) + " NaN or Inf value(s) in the stack");
+}
+
 }  // namespace
 }  // namespace cdn
 
@@ -126,6 +95,7 @@ using namespace cdn;
 
 struct cudaDecon_batch {
   Batch b;
+  DevBuf red;  // partials of the input checks
 };
 
 struct cudaDecon_net {
@@ -141,6 +111,7 @@ struct cudaDecon_rl {
   RLEngine eng;
   int k_next = 0;
   DevBuf red, ref;  // reduction partials of the diagnostics (not the engine's), snapshot of X
+  DevBuf probe;     // destination of the copy-rate probe (never an engine buffer)
   DevBuf u16;       // staging of uint16 stacks
   DevBuf post[2];   // device-resident deskew / affine results
   int post_cur = -1, pnx = 0, pny = 0, pnz = 0;
@@ -164,59 +135,30 @@ int cudaDecon_fft_supported(int n) { return fft_length_supported(n) ? 1 : 0; }
 
 unsigned cudaDecon_find_optimal_dimension(unsigned n, int step) { return find_optimal_dimension(n, step); }
 
+unsigned cudaDecon_deskew_width(int nx, float dz, float dr, float deskewAngle, int outputWidth) {
+  return deskew_interface_width(nx, dz, dr, deskewAngle, outputWidth, g_output_nz);
+}
+
 int RL_interface_init(int nx, int ny, int nz, float dr, float dz, float dr_psf, float dz_psf, float deskewAngle,
                       float rotationAngle, int outputWidth, bool bSkewedDecon, char* OTF_file_name) {
   return guard([&]() -> int {
-    if (nx <= 0 || ny <= 0 || nz <= 0) fail("RL_interface_init: non-positive dimensions");
     if (!OTF_file_name) fail("RL_interface_init: OTF file name is null");
     auto S = std::make_unique<Session>();
-    S->output_ny = find_optimal_dimension(ny);
-    S->output_nz = find_optimal_dimension(nz);
-    const bool has_deskew = std::fabs(deskewAngle) > 0.0;
-    // the CLI also optimises the width for skewed decon
-    // (src/linearDecon.cpp:643); the library path leaves it unoptimised and
-    // then plans an FFT of the deskewed width over the raw width
-    S->output_nx = (has_deskew && !bSkewedDecon) ? (unsigned)nx : find_optimal_dimension(nx);
-    S->crop = (S->output_nx != (unsigned)nx) || (S->output_ny != (unsigned)ny) || (S->output_nz != (unsigned)nz);
-
-    const TiffImage img = tiff_read(OTF_file_name);
+    try {
+      static_cast<SessionGeometry&>(*S) =
+          session_geometry(nx, ny, nz, dr, dz, deskewAngle, rotationAngle, outputWidth, bSkewedDecon);
+    } catch (const std::exception& e) {
+      fail(std::string("RL_interface_init: ") + e.what());
+    }
     if (bSkewedDecon) dz_psf *= std::fabs(std::sin(deskewAngle * M_PI / 180.));
-    S->otf.set_from_image(img.data.data(), img.width, img.height, img.depth, dr_psf, dz_psf);
-
-    S->deskewFactor = 0.;
-    S->deskewedXdim = S->output_nx;
-    if (has_deskew) {
-      if (deskewAngle < 0) deskewAngle += 180.;
-      S->deskewFactor = std::cos(deskewAngle * M_PI / 180.) * dz / dr;
-      if (outputWidth == 0)
-        S->deskewedXdim = (unsigned)(S->deskewedXdim +
-                                     std::floor(S->output_nz * dz * std::fabs(std::cos(deskewAngle * M_PI / 180.)) / dr));
-      else
-        S->deskewedXdim = (unsigned)outputWidth;
-      S->deskewedXdim = find_optimal_dimension(S->deskewedXdim);
-      dz *= std::fabs(std::sin(deskewAngle * M_PI / 180.));
-    }
-    if (std::fabs(rotationAngle) > 0.0) {
-      rotationAngle *= M_PI / 180;
-      const float stretch = dr / dz;
-      S->rot_on = true;
-      S->rot[0] = std::cos(rotationAngle) * stretch;
-      S->rot[1] = std::sin(rotationAngle) * stretch;
-      S->rot[2] = -std::sin(rotationAngle);
-      S->rot[3] = std::cos(rotationAngle);
-      if (S->rot[1] == 0.f || S->rot[0] == 0.f) fail("RL_interface_init: degenerate rotation angle");
-    }
-    S->dr = dr;
-    S->dz = dz;
-    const unsigned fft_nx = bSkewedDecon ? S->output_nx : S->deskewedXdim;
+    S->otf = load_otf_file(OTF_file_name, dr_psf, dz_psf);
     // precompiled lengths need nothing; others are compiled now (like a
     // cuFFT plan, src/RL-Biggs-Andrews.cpp:622-631), without touching the GPU
-    for (unsigned n : {fft_nx, S->output_ny, S->output_nz}) {
+    for (unsigned n : {S->fft_nx, S->output_ny, S->output_nz}) {
       if (!fft_length_supported((int)n))
         fail("RL_interface_init: FFT length " + std::to_string(n) + " is not supported (2,3,5,7-smooth, at most 16384)");
       fft_prepare_length((int)n);
     }
-    S->fft_nx = fft_nx;
     g_output_nz = S->output_nz;  // the stream is created on first use
     g_s = std::move(S);
     return 1;
@@ -274,6 +216,8 @@ int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, f
     if (!g_s) fail("RL_interface called before RL_interface_init");
     Session& S = *g_s;
     if (!raw_data || !result) fail("RL_interface: null buffer");
+    if (!std::isfinite(background) || !std::isfinite(padVal)) fail("RL_interface: non-finite background or pad value");
+    if (nIters < 0) fail("RL_interface: negative iteration count");
     if (!S.st) S.st = make_stream();
     if ((unsigned)nx < S.output_nx || (unsigned)ny < S.output_ny || (unsigned)nz < S.output_nz)
       fail("RL_interface: stack smaller than the initialised size");
@@ -356,6 +300,9 @@ int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, f
       cx = (unsigned)nxo;
       cz = (unsigned)nzo;
     }
+    // uint16 input and a finite OTF cannot give NaN or Inf; report it if the
+    // arithmetic ever does, rather than return a poisoned stack as a success
+    check_finite(st, X, (long)vox(), S.red_part, "RL_interface result");
     copy_to_host(st, result, X, vox() * sizeof(float));
     return 1;
   });
@@ -377,15 +324,13 @@ int Deskew_interface(const float* const raw_data, int nx, int ny, int nz, float 
                      float* const result, int outputWidth, int extraShift, float padVal) {
   return guard([&]() -> int {
     if (!raw_data || !result) fail("Deskew_interface: null buffer");
+    if (nx <= 0 || ny <= 0 || nz <= 0) fail("Deskew_interface: non-positive dimensions");
+    const unsigned xdim = deskew_interface_width(nx, dz, dr, deskewAngle, outputWidth, g_output_nz);
+    if (xdim == 0) fail("Deskew_interface: zero output width");
     double factor = 0.;
-    unsigned xdim = (unsigned)nx;
     if (std::fabs(deskewAngle) > 0.0) {
       if (deskewAngle < 0) deskewAngle += 180.;
       factor = std::cos(deskewAngle * M_PI / 180.) * dz / dr;
-      if (outputWidth == 0)
-        xdim = (unsigned)(xdim + std::floor(g_output_nz * dz * std::fabs(std::cos(deskewAngle * M_PI / 180.)) / dr) / 4.);
-      else
-        xdim = (unsigned)outputWidth;
     }
     hipStream_t st = make_stream();
     DevBuf din, dout;
@@ -667,6 +612,9 @@ int cudaDecon_rl_set_raw_stack(cudaDecon_rl* h, int s, const float* raw) {
   return guard([&]() -> int {
     if (!h || !raw) fail("cudaDecon_rl_set_raw_stack: null argument");
     h->eng.load_stack(s, raw, false, h->st);
+    if (s >= 0 && s < h->eng.nstacks)
+      check_finite(h->st, h->eng.raw.as<float>() + (size_t)h->eng.fft.nvox * s, h->eng.fft.nvox, h->red,
+                   "cudaDecon_rl_set_raw_stack");
     CDN_HIP(hipStreamSynchronize(h->st));
     h->k_next = 0;
     return 1;
@@ -686,6 +634,7 @@ int cudaDecon_rl_set_raw(cudaDecon_rl* h, const float* raw) {
   return guard([&]() -> int {
     if (!h || !raw) fail("cudaDecon_rl_set_raw: null argument");
     h->eng.load_raw(raw, false, h->st);
+    check_finite(h->st, h->eng.raw.as<float>(), h->eng.total_vox(), h->red, "cudaDecon_rl_set_raw");
     CDN_HIP(hipStreamSynchronize(h->st));
     h->k_next = 0;
     return 1;
@@ -696,6 +645,7 @@ int cudaDecon_rl_set_seed(cudaDecon_rl* h, const float* seed) {
   return guard([&]() -> int {
     if (!h || !seed) fail("cudaDecon_rl_set_seed: null argument");
     h->eng.load_seed(seed, false, h->st);
+    check_finite(h->st, h->eng.y.as<float>(), h->eng.total_vox(), h->red, "cudaDecon_rl_set_seed");
     CDN_HIP(hipStreamSynchronize(h->st));
     return 1;
   });
@@ -704,7 +654,12 @@ int cudaDecon_rl_set_seed(cudaDecon_rl* h, const float* seed) {
 int cudaDecon_rl_iterate(cudaDecon_rl* h, int n, int k0, int seeded, double* lambdas, double* elapsed_ms) {
   return guard([&]() -> int {
     if (!h) fail("cudaDecon_rl_iterate: null handle");
+    if (n < 0) fail("cudaDecon_rl_iterate: negative iteration count");
     if (k0 != 0 && k0 != h->k_next) fail("cudaDecon_rl_iterate: k0 must be 0 or continue the previous call");
+    if (n == 0) {
+      if (elapsed_ms) *elapsed_ms = 0.0;
+      return 1;
+    }
     if (k0 == 0) h->eng.reset(h->st);
     hipEvent_t e0, e1;
     CDN_HIP(hipEventCreate(&e0));
@@ -795,7 +750,9 @@ int cudaDecon_rl_set_raw_block(cudaDecon_rl* h, const float* block, int bx, int 
     one_stack(h, "cudaDecon_rl_set_raw_block");
     if (!h || !block) fail("cudaDecon_rl_set_raw_block: null argument");
     fill(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, background);
+    if (!std::isfinite(background)) fail("cudaDecon_rl_set_raw_block: non-finite background");
     block_copy(h, h->eng.raw.as<float>(), block, nullptr, bx, by, bz, ox, oy, oz);
+    check_finite(h->st, h->eng.raw.as<float>(), h->eng.fft.nvox, h->red, "cudaDecon_rl_set_raw_block");
     h->eng.reset(h->st);
     CDN_HIP(hipStreamSynchronize(h->st));
     h->k_next = 0;
@@ -946,11 +903,12 @@ int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms) {
   return guard([&]() -> int {
     if (!h || n <= 0) fail("cudaDecon_rl_copy_probe: bad argument");
     const size_t bytes = sizeof(float) * (size_t)h->eng.fft.nvox / 16 * 16;
+    h->probe.ensure(bytes);
     hipEvent_t e0, e1;
     CDN_HIP(hipEventCreate(&e0));
     CDN_HIP(hipEventCreate(&e1));
     CDN_HIP(hipEventRecord(e0, h->st));
-    for (int i = 0; i < n; ++i) stream_copy(h->st, h->eng.raw.p, h->eng.y.p, bytes);
+    for (int i = 0; i < n; ++i) stream_copy(h->st, h->eng.raw.p, h->probe.p, bytes);
     CDN_HIP(hipEventRecord(e1, h->st));
     CDN_HIP(hipEventSynchronize(e1));
     float t = 0.f;
@@ -983,7 +941,9 @@ cudaDecon_batch* cudaDecon_batch_create(int n, const int* dims_zyx, const float*
 int cudaDecon_batch_set_raw(cudaDecon_batch* h, int i, const float* raw) {
   return guard([&]() -> int {
     if (!h || !raw || i < 0 || i >= (int)h->b.engines.size()) fail("cudaDecon_batch_set_raw: bad argument");
-    h->b.engines[i]->load_raw(raw, false, h->b.streams[0]);
+    RLEngine& e = *h->b.engines[i];
+    e.load_raw(raw, false, h->b.streams[0]);
+    check_finite(h->b.streams[0], e.raw.as<float>(), e.total_vox(), h->red, "cudaDecon_batch_set_raw");
     CDN_HIP(hipStreamSynchronize(h->b.streams[0]));
     return 1;
   });
@@ -1004,6 +964,7 @@ int cudaDecon_batch_set_raw_block(cudaDecon_batch* h, int i, const float* block,
       CDN_HIP(hipMemcpy2DAsync(e.raw.as<float>() + ((size_t)(oz + z) * F.ny + oy) * F.nx + ox, sizeof(float) * F.nx,
                                block + (size_t)z * bx * by, sizeof(float) * bx, sizeof(float) * bx, by,
                                hipMemcpyHostToDevice, st));
+    check_finite(st, e.raw.as<float>(), F.nvox, h->red, "cudaDecon_batch_set_raw_block");
     e.reset(st);
     CDN_HIP(hipStreamSynchronize(st));
     return 1;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common_dev.hpp"
+#include "host_common.hpp"
 
 #include <cstdint>
 #include <stdexcept>
@@ -25,7 +26,5 @@ namespace cdn {
                                " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
     }                                                                                   \
   } while (0)
-
-inline void fail(const std::string& msg) { throw std::runtime_error(msg); }
 
 }  // namespace cdn

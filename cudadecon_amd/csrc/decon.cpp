@@ -81,24 +81,6 @@ void Fft3::c2r(hipStream_t st, float2* S, float* out) const {
 // ---------------------------------------------------------------------------
 // OTF
 // ---------------------------------------------------------------------------
-void OtfTable::set_from_image(const float* img, int width, int height, int depth, float dr_psf, float dz_psf) {
-  // determine_OTF_dimensions (src/RL-Biggs-Andrews.cpp:506-529), float math
-  if (depth > 1) {
-    nxotf = width / 2;
-    nyotf = height;
-    nzotf = depth;
-  } else {
-    nxotf = height;
-    nyotf = 1;
-    nzotf = width / 2;
-  }
-  if (nxotf < 2 || nzotf < 1) fail("OTF table is too small");
-  dkx = 1.0f / ((float)((nxotf - 1) * 2) * dr_psf);
-  dky = (nyotf > 1) ? 1.0f / ((float)nyotf * dr_psf) : dkx;
-  dkz = 1.0f / ((float)nzotf * dz_psf);
-  data.assign(img, img + (size_t)2 * nxotf * nyotf * nzotf);
-}
-
 void OtfDev::setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st) {
   ra = (t.nyotf == 1);
   ks[0] = kscale[0];
@@ -229,6 +211,11 @@ void RLEngine::reset(hipStream_t st) {
 }
 
 void RLEngine::prepare(int n) { lam.ensure(sizeof(double) * 4 * (size_t)nstacks * (size_t)std::max(n, 1)); }
+
+std::vector<const void*> RLEngine::device_addresses() const {
+  return {raw.p, xa.p, xb.p, y.p, g.p, spec.p, part.p, lam.p, otf.raw.p, otf.tz.p, otf.tz2.p,
+          fft.twx.p, fft.twy.p, fft.twz.p};
+}
 
 void RLEngine::load_seed(const float* in, bool on_device, hipStream_t st) {
   const size_t vb = sizeof(float) * (size_t)total_vox();

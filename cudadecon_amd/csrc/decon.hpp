@@ -3,6 +3,7 @@
 
 #include "common.hpp"
 #include "fft_kernels.hpp"
+#include "geometry.hpp"
 
 #include <string>
 #include <vector>
@@ -35,15 +36,6 @@ struct Fft3 {
   void r2c(hipStream_t st, const float* in, float2* S) const;   // unnormalised
   void c2r(hipStream_t st, float2* S, float* out) const;        // unnormalised, consumes S
   void col(hipStream_t st, float2* S, int axis, bool inverse) const;  // axis 1 = y, 2 = z
-};
-
-// Raw OTF as loaded (complex64 interleaved) and determine_OTF_dimensions
-// (src/RL-Biggs-Andrews.cpp:506-529).
-struct OtfTable {
-  std::vector<float> data;
-  int nxotf = 0, nyotf = 0, nzotf = 0;
-  float dkx = 0, dky = 0, dkz = 0;
-  void set_from_image(const float* img, int width, int height, int depth, float dr_psf, float dz_psf);
 };
 
 struct OtfDev {
@@ -96,6 +88,10 @@ struct RLEngine {
   // fastest) receives the factor used at each k (0 for k < 2).
   void iterate(int n, int k0, bool seeded, hipStream_t st, double* lambdas);
   float* result() const { return cur; }
+  // every device address an iteration reads or writes (buffers, tables,
+  // twiddles): a captured graph of the iteration stays valid while these
+  // are unchanged
+  std::vector<const void*> device_addresses() const;
 };
 
 }  // namespace cdn

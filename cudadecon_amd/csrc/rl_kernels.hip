@@ -319,6 +319,26 @@ void stats(hipStream_t st, const float* img, long n, double* d_part, double out[
   }
 }
 
+// NaN / Inf count (exponent bits all set), per-block partials in FP64
+__global__ void __launch_bounds__(BS) k_nonfinite_part(const float* __restrict__ img, long n, double* part) {
+  double c = 0;
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS)
+    if ((__float_as_uint(img[i]) & 0x7f800000u) == 0x7f800000u) c += 1.0;
+  block_reduce2(c, 0.0, part + 2 * blockIdx.x);
+}
+long count_nonfinite(hipStream_t st, const float* img, long n, double* d_part) {
+  if (n <= 0) return 0;
+  const int nb = (int)std::min<long>(RED_BLOCKS, (n + BS - 1) / BS);
+  hipLaunchKernelGGL(k_nonfinite_part, dim3(nb), dim3(BS), 0, st, img, n, d_part);
+  CDN_HIP(hipGetLastError());
+  std::vector<double> h(2 * (size_t)nb);
+  CDN_HIP(hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  CDN_HIP(hipStreamSynchronize(st));
+  double c = 0;
+  for (int b = 0; b < nb; ++b) c += h[2 * b];
+  return (long)c;
+}
+
 // camcor_kernel (src/camcor.cu:80-108): per pixel a, b, offset (three nx*ny
 // planes), previous plane z-1 (z = 0 takes plane nz-1, the texture's wrap
 // for the normalised coordinate (z - 0.5) / nz), and the float -> uint16

@@ -20,6 +20,9 @@ void fill(hipStream_t st, float* p, long n, float v);
 void stats(hipStream_t st, const float* img, long n, double* d_part, double out[3]);
 // sum (a-b)^2 and sum b^2 of n floats (FP64); d_part holds >= 2 * 1024 doubles.  Synchronises.
 void diff_norm(hipStream_t st, const float* a, const float* b, long n, double* d_part, double out[2]);
+// number of NaN / Inf values among n floats; d_part holds >= 2 * 1024
+// doubles.  Synchronises.
+long count_nonfinite(hipStream_t st, const float* img, long n, double* d_part);
 // camera correction of a uint16 stack with 3 parameter planes (a, b, offset)
 void camcor(hipStream_t st, const uint16_t* raw, const float* cam, int nx, int ny, int nz, uint16_t* out);
 void add_scalar(hipStream_t st, float* p, long n, float c);

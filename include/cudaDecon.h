@@ -91,6 +91,11 @@ int cudaDecon_fft_supported(int n);
 /* findOptimalDimension (src/RL-Biggs-Andrews.cpp:33-43); n <= 1 returned as is. */
 unsigned cudaDecon_find_optimal_dimension(unsigned n, int step);
 
+/* Width of the result Deskew_interface writes for these arguments (its
+ * sizing rule, src/RL-Biggs-Andrews.cpp:736-744, including the session's
+ * output nz when outputWidth is 0), so callers can allocate nz*ny*width. */
+unsigned cudaDecon_deskew_width(int nx, float dz, float dr, float deskewAngle, int outputWidth);
+
 /* Session sizes: which = 0 output nz (before rotation), 1 deskewed nx,
  * 2 cropped nx, 3 FFT nx of the iteration.  0 without a session. */
 unsigned cudaDecon_get_session_dims(int which);

@@ -173,3 +173,43 @@ def test_negative_rotation_sizes_use_abs(cdn, orc, tmp_path):
     assert g["rot"][1] < 0
     assert L.get_output_nx() == g["out_dims"][0] > 0
     L.RL_cleanup()
+
+
+def test_init_rejects_nan_otf(cdn, tmp_path):
+    """One NaN in the OTF would make every filtered voxel NaN: init refuses
+    the file and says why (no GPU work happens before RL_interface)."""
+    import numpy as np
+    from cudadecon_amd.otf import write_otf_tiff
+    tab = np.ones((17, 33), np.complex64)
+    tab[3, 4] = np.complex64(complex(np.nan, 0.0))
+    p = tmp_path / "otf_nan.tif"
+    write_otf_tiff(str(p), tab)
+    L = cdn.lib()
+    L.RL_cleanup()
+    assert L.RL_interface_init(128, 128, 64, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False, str(p).encode()) == 0
+    assert "NaN or Inf" in cdn.last_error()
+    assert L.get_output_nx() == 0  # no session was set up
+
+
+def test_deskew_width_rule(cdn, tmp_path):
+    """cudaDecon_deskew_width gives the width Deskew_interface writes
+    (src/RL-Biggs-Andrews.cpp:736-744): the caller's width, nx at a zero
+    angle, else nx + floor(output_nz dz |cos| / dr) / 4 from the session."""
+    import math
+
+    import numpy as np
+    from cudadecon_amd.otf import write_otf_tiff
+    L = cdn.lib()
+    assert L.cudaDecon_deskew_width(512, 0.3, 0.104, 31.5, 700) == 700
+    assert L.cudaDecon_deskew_width(512, 0.3, 0.104, 0.0, 0) == 512
+    assert L.cudaDecon_deskew_width(512, float("nan"), 0.104, 31.5, 0) == 0  # no valid width
+    p = tmp_path / "otf.tif"
+    write_otf_tiff(str(p), np.ones((17, 33), np.complex64))
+    assert L.RL_interface_init(512, 512, 128, 0.104, 0.3, 0.104, 0.3, 31.5, 0.0, 0, False, str(p).encode()) == 1
+    L.RL_cleanup()  # output_nz survives cleanup, as in the reference
+    want = int(512 + math.floor(np.float32(128 * np.float32(0.3)) * abs(math.cos(math.radians(31.5)))
+                                / np.float32(0.104)) / 4.0)
+    got = L.cudaDecon_deskew_width(512, 0.3, 0.104, 31.5, 0)
+    assert abs(got - want) <= 1 and got > 512, (got, want)
+    with pytest.raises(ValueError):
+        cdn.deskew(np.zeros((4, 4, 512), np.float32), 31.5, 0.3, 0.104, out_width=got + 1)

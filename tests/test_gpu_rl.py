@@ -346,3 +346,56 @@ def test_engine_larger_than_memory_fails_cleanly(cdn):
     e.iterate(2)
     assert np.isfinite(e.get()).all()
     e.close()
+
+
+def test_nan_inputs_are_rejected(cdn):
+    """A NaN or Inf in a staged stack or seed is reported (it would spread
+    through the FFT to every voxel), and the engine stays usable."""
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import gaussian_psf
+    tab = make_otf(gaussian_psf(1.5, 3.0))
+    shape = (64, 64, 64)
+    eng = cdn.RLEngine(shape, tab, (0.5, 0.5, 0.5))
+    R = np.full(shape, 10.0, np.float32)
+    R[3, 4, 5] = np.nan
+    with pytest.raises(RuntimeError, match="NaN or Inf"):
+        eng.set_raw(R)
+    R[3, 4, 5] = np.inf
+    with pytest.raises(RuntimeError, match="1 NaN or Inf"):
+        eng.set_raw(R)
+    R[3, 4, 5] = 10.0
+    eng.set_raw(R)
+    S = R.copy()
+    S[0, 0, 0] = np.nan
+    with pytest.raises(RuntimeError, match="NaN or Inf"):
+        eng.set_seed(S)
+    with pytest.raises(RuntimeError, match="negative iteration count"):
+        eng.iterate(-1)
+    lam, ms = eng.iterate(0)
+    assert ms == 0.0
+    eng.iterate(2)
+    assert np.isfinite(eng.get()).all()
+    eng.close()
+
+
+def test_copy_probe_keeps_a_loaded_seed(cdn):
+    """The copy-rate probe writes a buffer of its own: a seed loaded before
+    it still starts the seeded iteration."""
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import gaussian_psf
+    tab = make_otf(gaussian_psf(1.5, 3.0))
+    shape = (64, 64, 64)
+    rng = np.random.default_rng(5)
+    R = rng.uniform(1, 100, shape).astype(np.float32)
+    seed = rng.uniform(1, 100, shape).astype(np.float32)
+    outs = []
+    for probe in (False, True):
+        eng = cdn.RLEngine(shape, tab, (0.5, 0.5, 0.5))
+        eng.set_raw(R)
+        eng.set_seed(seed)
+        if probe:
+            eng.copy_probe(2)
+        eng.iterate(3, seeded=True)
+        outs.append(eng.get())
+        eng.close()
+    assert np.array_equal(outs[0], outs[1])
