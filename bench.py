@@ -54,43 +54,71 @@ def pmc_traffic(kernel: str):
         return None
 
 
-def batch_48_stacks(cdn, tab, iters=10, nstreams=8):
-    """48 stacks of uneven shape (z in {64..256}, y, x in {256..1024}, drawn
-    with seed 2): iterations of all stacks one at a time vs one captured
-    hipGraph over nstreams streams.  Stacks are staged on the device (a bead
-    block on a constant background), so no full-volume transfers."""
-    rng = np.random.default_rng(2)
-    zs, xys = [64, 96, 128, 192, 256], [256, 384, 512, 768, 1024]
-    shapes = [(int(rng.choice(zs)), int(rng.choice(xys)), int(rng.choice(xys))) for _ in range(48)]
-    ks = [(np.float32(0.25 * 128 / s[2]), np.float32(0.25 * 128 / s[1]), np.float32(0.515625 * 64 / s[0]))
-          for s in shapes]
-    b = cdn.Batch(shapes, tab, ks, nstreams=nstreams)
-    blk = np.random.default_rng(0).uniform(0, 100, (32, 64, 64)).astype(np.float32)
-    for i in range(len(shapes)):
-        b.set_raw_block(i, blk, (8, 8, 8), 20.0)
-    vox = int(sum(np.prod(s) for s in shapes))
-    out = {"stacks": len(shapes), "voxels": vox, "iterations": iters, "streams": nstreams}
-    for mode in ("sequential", "graph"):
+def _batch_modes(b, vox, iters, modes=("sequential", "graph", "grouped_graph")):
+    out = {}
+    for mode in modes:
         b.run(iters, mode)
         ms = min(b.run(iters, mode) for _ in range(2))
         out[mode] = {"ms": round(ms, 2), "Gvoxel_iter_per_s": round(vox * iters / (ms * 1e-3) / 1e9, 2)}
-    out["speedup"] = round(out["sequential"]["ms"] / out["graph"]["ms"], 3)
-    b.close()
-    # the same batch after the autotuner picked launch shapes for its lengths
+    return out
+
+
+def batch_48_stacks(cdn, iters=10, nstreams=8):
+    """The 48-stack workload (cudadecon_amd.synth batch_shapes / batch_stack):
+    shapes drawn with seed 2 from z in {64..256}, y, x in {256..1024}; each
+    stack its own seeded bead field imaged through channel PSF i mod 3.
+    10 iterations of every stack: one stack at a time (each stack's engine),
+    the engines round-robin over nstreams streams in one captured hipGraph,
+    and the grouped schedule (one launch per pass and transform length,
+    spanning stacks) in one captured hipGraph."""
+    from cudadecon_amd import synth, warmstart
+    from cudadecon_amd.otf import make_otf
     t0 = time.perf_counter()
-    rep = cdn.autotune(shapes, reps=3, use_cache=True)
-    out["autotune_s"] = round(time.perf_counter() - t0, 1)
-    out["tuned_lengths"] = {str(e["n"]): e["chosen"] for e in rep["lengths"]
-                            if any(v not in (0, -1) for v in e["chosen"].values())}
-    b = cdn.Batch(shapes, tab, ks, nstreams=nstreams)
-    for i in range(len(shapes)):
-        b.set_raw_block(i, blk, (8, 8, 8), 20.0)
-    b.run(iters, "graph")
-    ms = min(b.run(iters, "graph") for _ in range(2))
-    out["graph_tuned"] = {"ms": round(ms, 2), "Gvoxel_iter_per_s": round(vox * iters / (ms * 1e-3) / 1e9, 2)}
-    out["speedup_tuned"] = round(out["sequential"]["ms"] / ms, 3)
+    shapes = synth.batch_shapes()
+    tabs = [make_otf(synth.gaussian_psf(sxy, sz)) for sxy, sz in synth.BATCH_CHANNELS]
+    table_of = [synth.batch_channel(i)[0] for i in range(len(shapes))]
+    ks = [warmstart.kscale_for(s, tabs[table_of[i]], dr=synth.BATCH_DR, dz=synth.BATCH_DZ)
+          for i, s in enumerate(shapes)]
+    b = cdn.Batch(shapes, tabs, ks, nstreams=nstreams, table_of=table_of)
+    for i, s in enumerate(shapes):
+        raw = synth.batch_stack(i, s)
+        b.set_raw(i, np.maximum(raw.astype(np.float32) - np.float32(100.0), np.float32(0.0)))
+    setup = time.perf_counter() - t0
+    vox = int(sum(np.prod(s) for s in shapes))
+    out = {"stacks": len(shapes), "voxels": vox, "iterations": iters, "streams": nstreams,
+           "channels": [list(c) for c in synth.BATCH_CHANNELS], "grouped_launches_per_iter": b.launches(),
+           "per_stack_launches_per_iter": 10 * len(shapes)}
+    out.update(_batch_modes(b, vox, iters))
+    seq = out["sequential"]["ms"]
+    out["speedup_graph"] = round(seq / out["graph"]["ms"], 3)
+    out["speedup_grouped_graph"] = round(seq / out["grouped_graph"]["ms"], 3)
     b.close()
-    cdn.autotune_clear()
+    out["setup_s"] = round(setup, 1)
+    return out
+
+
+def batch_small_uneven(cdn, tab, n=256, iters=10, nstreams=8):
+    """256 stacks with z, y, x each drawn (seed 5) from {64, 96, 128, 192}:
+    small uneven stacks, where launch latency dominates.  One at a time,
+    per-stack engines in one captured hipGraph, grouped schedule in one
+    captured hipGraph.  Each stack is a bead block on a flat background."""
+    rng = np.random.default_rng(5)
+    sizes = [64, 96, 128, 192]
+    shapes = [(int(rng.choice(sizes)), int(rng.choice(sizes)), int(rng.choice(sizes))) for _ in range(n)]
+    ks = [(np.float32(0.25 * 128 / s[2]), np.float32(0.25 * 128 / s[1]), np.float32(0.515625 * 64 / s[0]))
+          for s in shapes]
+    b = cdn.Batch(shapes, tab, ks, nstreams=nstreams)
+    blk = np.random.default_rng(0).uniform(0, 100, (32, 48, 48)).astype(np.float32)
+    for i in range(n):
+        b.set_raw_block(i, blk, (8, 8, 8), 20.0)
+    vox = int(sum(np.prod(s) for s in shapes))
+    out = {"stacks": n, "voxels": vox, "iterations": iters, "streams": nstreams,
+           "grouped_launches_per_iter": b.launches(), "per_stack_launches_per_iter": 10 * n}
+    out.update(_batch_modes(b, vox, iters))
+    seq = out["sequential"]["ms"]
+    out["speedup_graph"] = round(seq / out["graph"]["ms"], 3)
+    out["speedup_grouped_graph"] = round(seq / out["grouped_graph"]["ms"], 3)
+    b.close()
     return out
 
 
@@ -130,34 +158,30 @@ def stacks_256(cdn, tab, iters=10, nstreams=8):
 
 
 def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int, warm: int = 2):
-    """FP32 oracle RL-BA (OpenMP) on the first `slab` z-planes of the stack:
-    steady-state iterations k = warm .. warm + iters - 1, timed as a run of
-    warm + iters iterations less a run of the `warm` first ones (k = 0, 1
-    have no prediction and k = 0 no dot products)."""
+    """FP32 oracle RL-BA (OpenMP, the oracle's own FFT) on the stack (its
+    first `slab` z-planes when slab > 0): `warm` + `iters` iterations, the
+    median wall time of the steady ones k = warm .. warm + iters - 1 (k = 0,
+    1 have no prediction and k = 0 no dot products)."""
     import oracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
     orc.set_threads(threads)
-    sub = np.ascontiguousarray(R_full[:slab])
+    sub = np.ascontiguousarray(R_full[:slab] if slab > 0 else R_full)
     nz, ny, nx = sub.shape
     g = orc.init_geometry(nx, ny, nz, 0.104, 0.2, 0.104, 0.2, 0.0, 0.0, 0, False,
                           (2 * tab.shape[1], tab.shape[0], 1))
     H = orc.otf_interp(tab, g["otf_dims"], g["fft_dims"], ks).astype(np.complex64)
-    t0 = time.perf_counter()
-    orc.rl_ba_f32(sub, H, warm)
-    t1 = time.perf_counter()
-    orc.rl_ba_f32(sub, H, warm + iters)
-    t2 = time.perf_counter()
-    dt = (t2 - t1) - (t1 - t0)
+    _, _, sec = orc.rl_ba_f32_timed(sub, H, warm + iters)
+    steady = sec[warm:]
+    med = float(np.median(steady))
     return {
-        "value": round(sub.size * iters / dt / 1e9, 5),
+        "value": round(sub.size / med / 1e9, 5),
         "unit": "Gvoxel/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP32 oracle RL-BA (own mixed-radix FFT, OpenMP) on the first {slab} z-planes "
-                  f"({nz}x{ny}x{nx}) of the stack: {iters} steady iterations k={warm}..{warm + iters - 1}, "
-                  f"mean {dt / iters:.3f} s each (a {warm + iters}-iteration run, {t2 - t1:.1f} s, less a "
-                  f"{warm}-iteration run, {t1 - t0:.1f} s)",
+        "sample": f"FP32 oracle RL-BA (own mixed-radix FFT, OpenMP) on the {nz}x{ny}x{nx} stack: median of "
+                  f"{iters} steady iterations k={warm}..{warm + iters - 1} ({med:.3f} s; all: "
+                  f"{', '.join(f'{t:.3f}' for t in steady)} s) after {warm} warm-up iterations",
     }
 
 
@@ -221,10 +245,10 @@ def main():
     ap.add_argument("--pass-iters", type=int, default=3, help="iterations with per-pass event timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=5)
-    ap.add_argument("--cpu-slab", type=int, default=64)
+    ap.add_argument("--cpu-slab", type=int, default=0, help="z-planes of the CPU baseline sample (0: the whole stack)")
     ap.add_argument("--no-e2e", action="store_true", help="skip the RL_interface end-to-end timing")
     ap.add_argument("--e2e-iters", type=int, default=10)
-    ap.add_argument("--no-batch", action="store_true", help="skip the 48-stack batch and 256-stack measurements")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch measurements (48 uneven stacks, 256 small)")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
     ap.add_argument("--no-pipeline", action="store_true", help="skip the 512x2048x2048 pipeline")
     args = ap.parse_args()
@@ -288,7 +312,8 @@ def main():
         e2e = {"entry": "RL_interface", "iterations": args.e2e_iters, "seconds": round(e2e_s, 4),
                "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
-    batch = None if args.no_batch else batch_48_stacks(cdn, tab)
+    batch = None if args.no_batch else batch_48_stacks(cdn)
+    small = None if args.no_batch else batch_small_uneven(cdn, tab)
     stacks = None if args.no_batch else stacks_256(cdn, tab)
     warm = None if args.no_warm else warm_start_eval(cdn)
     big = None if args.no_pipeline else pipeline_512x2048x2048(cdn)
@@ -354,6 +379,7 @@ def main():
         "passes": passes,
         "rl_interface_e2e": e2e,
         "batch_48_stacks": batch,
+        "batch_256_small_uneven": small,
         "stacks_256_of_64x128x128": stacks,
         "warm_start": warm,
         "pipeline_512x2048x2048": big,

@@ -59,6 +59,7 @@ SIGNATURES = {
     "cudaDecon_fft_supported": (C.c_int, [C.c_int]),
     "cudaDecon_find_optimal_dimension": (C.c_uint, [C.c_uint, C.c_int]),
     "cudaDecon_deskew_width": (C.c_uint, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_int]),
+    "cudaDecon_set_flat_start": (C.c_int, [C.c_int, C.c_float]),
     "cudaDecon_get_session_dims": (C.c_uint, [C.c_int]),
     "cudaDecon_get_session_kscale": (C.c_int, [_f]),
     "cudaDecon_fft_r2c": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f]),
@@ -91,6 +92,11 @@ SIGNATURES = {
     "cudaDecon_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "cudaDecon_batch_get": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "cudaDecon_batch_destroy": (None, [C.c_void_p]),
+    "cudaDecon_batch_create_tables": (C.c_void_p, [C.c_int, C.c_void_p, C.c_int, _f, C.c_void_p, C.c_void_p, _f,
+                                                   C.c_int]),
+    "cudaDecon_batch_lambdas": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d]),
+    "cudaDecon_batch_set_grid_cap": (C.c_int, [C.c_void_p, C.c_long]),
+    "cudaDecon_batch_launches": (C.c_int, [C.c_void_p]),
     "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
     "cudaDecon_rl_set_raw_u16": (C.c_int, [C.c_void_p, _u16, C.c_float]),
     "cudaDecon_rl_deskew": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
@@ -121,6 +127,9 @@ SIGNATURES = {
     "cudaDecon_net_destroy": (None, [C.c_void_p]),
     "cudaDecon_rl_warm_start": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cudaDecon_device_sync": (C.c_int, []),
+    "cudaDecon_device_malloc": (C.c_void_p, [C.c_size_t]),
+    "cudaDecon_device_free": (C.c_int, [C.c_void_p]),
+    "cudaDecon_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
 }
 
 PASS_NAMES = ["x_r2c", "y_fwd", "z_filter", "y_inv", "x_lrcore", "x_update", "lambda"]
@@ -337,6 +346,41 @@ def camcor(raw_u16, camparam):
     return out
 
 
+class DeviceBuffer:
+    """Device memory for the entry points that take device pointers
+    (cudaDecon_device_malloc / copy / free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().cudaDecon_device_malloc(self.nbytes)
+        if not self.ptr:
+            raise RuntimeError(f"cudaDecon_device_malloc failed: {last_error()}")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        _check(lib().cudaDecon_device_copy(self.ptr, a.ctypes.data, a.nbytes, 1), "cudaDecon_device_copy")
+
+    def download(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        _check(lib().cudaDecon_device_copy(out.ctypes.data, self.ptr, out.nbytes, 2), "cudaDecon_device_copy")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().cudaDecon_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class RLEngine:
     """RL-Biggs-Andrews on HBM-resident state (cudaDecon_rl_* handle).  With
     stacks > 1 the engine holds that many stacks of `shape` and iterates
@@ -503,21 +547,48 @@ class RLEngine:
 class Batch:
     """Many independent stacks of uneven shape (cudaDecon_batch_* handle).
 
-    shapes: list of (nz, ny, nx); kscales: one (kx, ky, kz) per stack."""
+    shapes: list of (nz, ny, nx); kscales: one (kx, ky, kz) per stack;
+    otf_table: one OTF table for every stack, or a list of tables with
+    table_of giving each stack's (channels)."""
 
-    MODES = {"sequential": 0, "streams": 1, "graph": 2}
+    MODES = {"sequential": 0, "streams": 1, "graph": 2, "grouped": 3, "grouped_graph": 4}
 
-    def __init__(self, shapes, otf_table, kscales, nstreams=8):
+    def __init__(self, shapes, otf_table, kscales, nstreams=8, table_of=None):
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
         dims = np.ascontiguousarray(np.array(self.shapes, np.int32).reshape(-1))
         ks = _f32(np.array(kscales, np.float32).reshape(-1))
-        img = otf_image(otf_table)
-        d, h, w = img.shape
-        self._img = np.ascontiguousarray(img).reshape(-1)
-        self.h = lib().cudaDecon_batch_create(len(self.shapes), dims.ctypes.data, self._img, w, h, d, ks,
-                                              int(nstreams))
+        if isinstance(otf_table, (list, tuple)):
+            imgs = [otf_image(t) for t in otf_table]
+            self._img = np.ascontiguousarray(np.concatenate([i.reshape(-1) for i in imgs]))
+            whd = np.ascontiguousarray(np.array([(i.shape[2], i.shape[1], i.shape[0]) for i in imgs], np.int32))
+            tof = np.ascontiguousarray(np.array(table_of if table_of is not None else [0] * len(self.shapes),
+                                                np.int32))
+            if tof.size != len(self.shapes):
+                raise ValueError("table_of needs one table index per stack")
+            self.h = lib().cudaDecon_batch_create_tables(len(self.shapes), dims.ctypes.data, len(imgs), self._img,
+                                                         whd.ctypes.data, tof.ctypes.data, ks, int(nstreams))
+        else:
+            img = otf_image(otf_table)
+            d, h, w = img.shape
+            self._img = np.ascontiguousarray(img).reshape(-1)
+            self.h = lib().cudaDecon_batch_create(len(self.shapes), dims.ctypes.data, self._img, w, h, d, ks,
+                                                  int(nstreams))
         if not self.h:
             raise RuntimeError(f"cudaDecon_batch_create failed: {last_error()}")
+
+    def lambdas(self, i, iters):
+        """Acceleration factors of stack i in the last run (0 for k < 2)."""
+        out = np.zeros(int(iters), np.float64)
+        _check(lib().cudaDecon_batch_lambdas(self.h, int(i), int(iters), out), "cudaDecon_batch_lambdas")
+        return out
+
+    def set_grid_cap(self, max_workgroups):
+        """Grouped modes: persistent column-pass grids of at most this many workgroups (0: one per tile)."""
+        _check(lib().cudaDecon_batch_set_grid_cap(self.h, int(max_workgroups)), "cudaDecon_batch_set_grid_cap")
+
+    def launches(self):
+        """Kernel launches per iteration of the grouped modes."""
+        return int(lib().cudaDecon_batch_launches(self.h))
 
     def set_raw(self, i, raw):
         _check(lib().cudaDecon_batch_set_raw(self.h, int(i), _f32(raw)), "cudaDecon_batch_set_raw")

@@ -1,29 +1,67 @@
 // Batched RL-BA over many independent stacks of uneven shape on one GPU.
 //
-// Each stack has its own RL engine (plan, OTF, HBM buffers).  Three ways to
-// run n iterations of every stack:
-//   0  one stack at a time on one stream (the reference's usage pattern);
-//   1  stacks spread round-robin over S streams, so small kernels of
-//      different stacks overlap and fill the chip;
-//   2  the same schedule captured once into a hipGraph (fork/join over the S
-//      streams) and replayed: one graph launch per batch run.
+// Every stack's volumes and spectrum live in shared arenas, stacks ordered by
+// x length, and each stack also has an engine (its own FFT plan, OTF table at
+// its own k-scales, dot-product partials and factors) whose buffers are views
+// of its slices.  Five ways to run n iterations of every stack:
+//   0  each engine, one stack at a time on one stream (the reference's usage);
+//   1  each engine, stacks round-robin over S streams;
+//   2  schedule 1 captured once into a hipGraph (fork/join) and replayed;
+//   3  grouped: every pass is one launch per distinct transform length,
+//      spanning all stacks of that length whatever their other sizes.  The x
+//      passes see the contiguous rows of all stacks of one x length (a stack
+//      table gives each row pair its factor); the y passes and the z filter
+//      take a table of per-stack segments (spectrum, column geometry, OTF) and
+//      find each workgroup's stack by a search over it; the factors of all
+//      stacks come from one launch over per-stack ranges of the partials.
+//      About 3 nx + 4 ny + 2 nz + 1 launches per iteration (the distinct
+//      lengths), instead of 9 or 10 per stack;
+//   4  schedule 3 captured once into a hipGraph and replayed.
+// Every mode gives every stack bitwise the result of a one-stack engine: the
+// arithmetic per column, row and partial block is the same, and each stack's
+// partials are summed over the same blocks in the same order.
 // The iteration issues no host synchronisation and no memset, so it can be
 // captured; a stack restarts from its staged stack (X_0 = R) at each run.
 #include "batch.hpp"
 
+#include "rl_kernels.hpp"
+
 #include <algorithm>
+#include <cfloat>
+#include <map>
 
 namespace cdn {
 
-Batch::~Batch() {
+void CapturedRun::reset() {
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  iters = -1;
+  addr.clear();
+  post.clear();
+}
+
+Batch::~Batch() {
+  cap_streams.reset();
+  cap_grouped.reset();
+  engines.clear();  // views of the arenas: release before the arenas
   for (hipEvent_t e : events) (void)hipEventDestroy(e);
   for (hipStream_t s : streams) (void)hipStreamDestroy(s);
 }
 
-void Batch::setup(int n, const int* dims_zyx, const OtfTable& t, const float* kscales, int nstreams) {
+void Batch::setup(int n, const int* dims_zyx, std::vector<OtfTable> tabs, const int* table_of, const float* kscales,
+                  int nstreams) {
   if (n <= 0) fail("batch: no stacks");
+  if (tabs.empty()) fail("batch: no OTF tables");
+  for (int i = 0; i < n; ++i) {
+    if (dims_zyx[3 * i] <= 0 || dims_zyx[3 * i + 1] <= 0 || dims_zyx[3 * i + 2] <= 0)
+      fail("batch: stack " + std::to_string(i) + " has a non-positive size");
+    if (table_of[i] < 0 || table_of[i] >= (int)tabs.size())
+      fail("batch: stack " + std::to_string(i) + " names OTF table " + std::to_string(table_of[i]) + " of " +
+           std::to_string(tabs.size()));
+  }
+  tables = std::move(tabs);
   nstreams = std::max(1, std::min(nstreams, 32));
   for (int i = 0; i < nstreams + 1; ++i) {
     hipStream_t s;
@@ -35,77 +73,283 @@ void Batch::setup(int n, const int* dims_zyx, const OtfTable& t, const float* ks
     CDN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     events.push_back(e);
   }
+  // arena layout: stacks by x length, then input order; every stack of one
+  // x length back to back (their rows contiguous), groups 256-byte aligned
+  order.resize(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return dims_zyx[3 * a + 2] < dims_zyx[3 * b + 2]; });
+  pos.assign(n, 0);
+  voff.assign(n, 0);
+  soff.assign(n, 0);
+  long v = 0, c = 0;
+  for (int p = 0; p < n; ++p) {
+    const int i = order[p];
+    const int nz = dims_zyx[3 * i], ny = dims_zyx[3 * i + 1], nx = dims_zyx[3 * i + 2];
+    if (p > 0 && nx != dims_zyx[3 * order[p - 1] + 2]) {
+      v = (v + 63) / 64 * 64;
+      c = (c + 31) / 32 * 32;
+    }
+    pos[i] = p;
+    voff[i] = v;
+    soff[i] = c;
+    v += (long)nx * ny * nz;
+    c += (long)nz * ny * Fft3::pitch_of(nx);
+  }
+  a_raw.ensure(sizeof(float) * (size_t)v);
+  a_xa.ensure(sizeof(float) * (size_t)v);
+  a_xb.ensure(sizeof(float) * (size_t)v);
+  a_y.ensure(sizeof(float) * (size_t)v);
+  a_g.ensure(sizeof(float) * (size_t)v);
+  a_spec.ensure(sizeof(float2) * (size_t)c);
   engines.resize(n);
   for (int i = 0; i < n; ++i) {
+    const EngineViews ev{a_raw.as<float>() + voff[i], a_xa.as<float>() + voff[i], a_xb.as<float>() + voff[i],
+                         a_y.as<float>() + voff[i],   a_g.as<float>() + voff[i],  a_spec.as<float2>() + soff[i]};
     engines[i] = std::make_unique<RLEngine>();
-    engines[i]->setup(dims_zyx[3 * i + 2], dims_zyx[3 * i + 1], dims_zyx[3 * i], t, kscales + 3 * i, streams[0]);
+    engines[i]->setup(dims_zyx[3 * i + 2], dims_zyx[3 * i + 1], dims_zyx[3 * i], tables[table_of[i]],
+                      kscales + 3 * i, streams[0], 1, &ev);
   }
+  cur = a_xa.as<float>();
+  prev = a_xb.as<float>();
+  build_groups();
 }
 
-void Batch::enqueue(hipStream_t st, int i, int iters) {
+void Batch::build_groups() {
+  const int n = (int)engines.size();
+  xg.clear();
+  yg.clear();
+  zg.clear();
+  // x passes: runs of stacks with one x length; a run ends after a stack
+  // whose rows would put a row pair or an update workgroup across the next
+  // stack (ny nz odd, or not a multiple of the workgroup's rows)
+  std::vector<int> ranges(2 * (size_t)n);
+  long part_total = 0;
+  for (int p = 0; p < n;) {
+    const RLEngine& e0 = *engines[order[p]];
+    XGroup g;
+    g.nx = e0.fft.nx;
+    g.first = p;
+    g.voff = voff[order[p]];
+    g.soff = soff[order[p]];
+    g.pitch = e0.fft.pitch;
+    g.tw = e0.fft.twx.as<float2>();
+    g.rpb = x_update_rows_per_block(g.nx);
+    std::vector<long> starts;
+    while (p < n && engines[order[p]]->fft.nx == g.nx) {
+      const long r = (long)engines[order[p]]->fft.ny * engines[order[p]]->fft.nz;
+      starts.push_back(g.rows);
+      ranges[2 * (size_t)p] = (int)(part_total + g.rows / g.rpb);
+      ranges[2 * (size_t)p + 1] = (int)((r + g.rpb - 1) / g.rpb);
+      g.rows += r;
+      ++p;
+      if (r % 2 != 0 || r % g.rpb != 0) break;
+    }
+    g.count = p - g.first;
+    g.nb = x_update_blocks(g.nx, g.rows);
+    g.part0 = part_total;
+    part_total += g.nb;
+    g.row_starts.ensure(sizeof(long) * starts.size());
+    CDN_HIP(hipMemcpy(g.row_starts.p, starts.data(), sizeof(long) * starts.size(), hipMemcpyHostToDevice));
+    xg.push_back(std::move(g));
+  }
+  g_part.ensure(sizeof(double) * 2 * (size_t)std::max(part_total, 1L));
+  g_ranges.ensure(sizeof(int) * ranges.size());
+  CDN_HIP(hipMemcpy(g_ranges.p, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice));
+
+  // y passes by y length, z filter by z length (stacks in arena order)
+  std::map<int, std::vector<ColSeg>> ys;
+  std::map<int, std::vector<ZSeg>> zs;
+  std::map<int, const float2*> ytw, ztw;
+  std::map<int, long> ytot, ztot;
+  for (int p = 0; p < n; ++p) {
+    const int i = order[p];
+    const RLEngine& e = *engines[i];
+    const Fft3& F = e.fft;
+    float2* S = a_spec.as<float2>() + soff[i];
+    {
+      const long L = col_tile_cols(F.ny), nt = (F.nxh + L - 1) / L;
+      ColSeg s{S, F.nxh, F.pitch, (long)F.ny * F.pitch, nt, ytot[F.ny]};
+      ys[F.ny].push_back(s);
+      ytot[F.ny] += (long)F.nz * nt;
+      ytw[F.ny] = F.twy.as<float2>();
+    }
+    {
+      const long L = col_tile_cols(F.nz), nt = (F.nxh + L - 1) / L;
+      ZSeg s{S, F.nxh, (long)F.ny * F.pitch, F.pitch, nt, F.ny, ztot[F.nz], e.otf.params(e.table, 1.0 / (double)F.nvox)};
+      zs[F.nz].push_back(s);
+      ztot[F.nz] += (long)F.ny * nt;
+      ztw[F.nz] = F.twz.as<float2>();
+    }
+  }
+  for (auto& kv : ys) {
+    ColGroup g;
+    g.len = kv.first;
+    g.nseg = (int)kv.second.size();
+    g.total = ytot[kv.first];
+    g.tw = ytw[kv.first];
+    g.segs.ensure(sizeof(ColSeg) * kv.second.size());
+    CDN_HIP(hipMemcpy(g.segs.p, kv.second.data(), sizeof(ColSeg) * kv.second.size(), hipMemcpyHostToDevice));
+    fft_prepare_ragged(g.len);
+    yg.push_back(std::move(g));
+  }
+  for (auto& kv : zs) {
+    ColGroup g;
+    g.len = kv.first;
+    g.nseg = (int)kv.second.size();
+    g.total = ztot[kv.first];
+    g.tw = ztw[kv.first];
+    g.segs.ensure(sizeof(ZSeg) * kv.second.size());
+    CDN_HIP(hipMemcpy(g.segs.p, kv.second.data(), sizeof(ZSeg) * kv.second.size(), hipMemcpyHostToDevice));
+    fft_prepare_ragged(g.len);
+    zg.push_back(std::move(g));
+  }
+  for (const XGroup& g : xg) fft_prepare_ragged(g.nx);
+  cap_grouped.reset();
+}
+
+int Batch::grouped_launches() const { return 3 * (int)xg.size() + 4 * (int)yg.size() + 2 * (int)zg.size() + 1; }
+
+void Batch::enqueue_engine(hipStream_t st, int i, int iters) {
   RLEngine& e = *engines[i];
   e.reset(st);
   e.iterate(iters, 0, false, st, nullptr);
 }
 
-void Batch::capture(int iters) {
-  for (auto& e : engines) e->prepare(iters);  // no allocation while capturing
-  // a graph replays the addresses it was captured with: reuse it only while
-  // no engine buffer has moved (a longer mode 0/1 run regrows lam, say)
-  std::vector<std::vector<const void*>> addr;
-  for (auto& e : engines) addr.push_back(e->device_addresses());
-  if (exec && graph_iters == iters && addr == graph_addr) return;
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  exec = nullptr;
-  graph = nullptr;
-  graph_iters = -1;
+void Batch::enqueue_streams(int iters) {
   const int S = (int)streams.size() - 1;
   hipStream_t root = streams[S];
-  CDN_HIP(hipDeviceSynchronize());
-  CDN_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
   CDN_HIP(hipEventRecord(events[0], root));
   for (int s = 0; s < S; ++s) CDN_HIP(hipStreamWaitEvent(streams[s], events[0], 0));
-  for (int i = 0; i < (int)engines.size(); ++i) enqueue(streams[i % S], i, iters);
+  for (int i = 0; i < (int)engines.size(); ++i) enqueue_engine(streams[i % S], i, iters);
   for (int s = 0; s < S; ++s) {
     CDN_HIP(hipEventRecord(events[1 + s], streams[s]));
     CDN_HIP(hipStreamWaitEvent(root, events[1 + s], 0));
   }
-  CDN_HIP(hipStreamEndCapture(root, &graph));
-  CDN_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  graph_iters = iters;
-  graph_addr = std::move(addr);
-  // capturing ran each engine's host bookkeeping of the schedule: its X_k and
-  // X_{k-1} pointers are now where a replay leaves them
-  graph_post.clear();
-  for (auto& e : engines) graph_post.emplace_back(e->cur, e->prev);
+}
+
+void Batch::enqueue_grouped(hipStream_t st, int iters) {
+  const int n = (int)engines.size();
+  const float eps = FLT_EPSILON;  // src/RL-Biggs-Andrews.cpp:274
+  float* c = a_xa.as<float>();
+  float* pv = a_xb.as<float>();
+  CDN_HIP(hipMemcpyAsync(c, a_raw.p, a_raw.bytes, hipMemcpyDeviceToDevice, st));  // X_0 = R, every stack
+  float2* spec = a_spec.as<float2>();
+  for (int k = 0; k < iters; ++k) {
+    double* lamk = g_lam.as<double>() + 4 * (size_t)n * k;
+    if (k >= 2) accel_factor_ragged(st, g_part.as<double>(), g_ranges.as<int>(), n, (double)eps, lamk);
+    const int mode = k >= 2 ? XIN_PRED : XIN_PLAIN;
+    for (XGroup& g : xg)
+      launch_x_r2c(g.nx, st, c + g.voff, pv + g.voff, a_y.as<float>() + g.voff, lamk + 4 * g.first, mode,
+                   spec + g.soff, g.rows, g.pitch, g.tw, 0, g.row_starts.as<long>(), g.count);
+    for (int f = 0; f < 2; ++f) {
+      for (ColGroup& g : yg)
+        launch_col_ragged(g.len, 0, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
+      for (ColGroup& g : zg)
+        launch_z_ragged(g.len, st, g.segs.as<ZSeg>(), g.nseg, g.total, f, g.tw, grid_cap);
+      for (ColGroup& g : yg)
+        launch_col_ragged(g.len, 1, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
+      if (f == 0)
+        for (XGroup& g : xg)
+          launch_x_lrcore(g.nx, st, spec + g.soff, a_raw.as<float>() + g.voff, g.rows, g.pitch, g.tw, eps);
+    }
+    const float* yv = k >= 2 ? a_y.as<float>() : c;
+    for (XGroup& g : xg) {
+      const int nb = launch_x_update(g.nx, st, spec + g.soff, yv + g.voff, pv + g.voff, a_g.as<float>() + g.voff,
+                                     g_part.as<double>() + 2 * g.part0, k >= 1, g.rows, g.pitch, g.tw);
+      if (nb != g.nb) fail("batch: the x-update launch shape changed under the grouped schedule");
+    }
+    std::swap(c, pv);
+  }
+  cur = c;
+  prev = pv;
+  for (int i = 0; i < n; ++i) {
+    engines[i]->cur = c + voff[i];
+    engines[i]->prev = pv + voff[i];
+  }
+}
+
+std::vector<const void*> Batch::addresses(bool grouped) const {
+  std::vector<const void*> a;
+  for (auto& e : engines) {
+    const auto d = e->device_addresses();
+    a.insert(a.end(), d.begin(), d.end());
+  }
+  if (grouped) {
+    for (const DevBuf* b : {&a_raw, &a_xa, &a_xb, &a_y, &a_g, &a_spec, &g_part, &g_lam, &g_ranges}) a.push_back(b->p);
+    for (const XGroup& g : xg) a.push_back(g.row_starts.p);
+    for (const ColGroup& g : yg) a.push_back(g.segs.p);
+    for (const ColGroup& g : zg) a.push_back(g.segs.p);
+    a.push_back(reinterpret_cast<const void*>(grid_cap));
+  }
+  return a;
+}
+
+void Batch::capture(CapturedRun& cr, int iters, bool grouped) {
+  // a graph replays the addresses it was captured with: reuse it only while
+  // none has moved (a longer mode 0/1 run regrows an engine's lam, say)
+  std::vector<const void*> addr = addresses(grouped);
+  if (cr.exec && cr.iters == iters && cr.addr == addr) return;
+  cr.reset();
+  const int S = (int)streams.size() - 1;
+  hipStream_t root = streams[S];
+  CDN_HIP(hipDeviceSynchronize());
+  CDN_HIP(hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal));
+  if (grouped)
+    enqueue_grouped(root, iters);
+  else
+    enqueue_streams(iters);
+  CDN_HIP(hipStreamEndCapture(root, &cr.graph));
+  CDN_HIP(hipGraphInstantiate(&cr.exec, cr.graph, nullptr, nullptr, 0));
+  cr.iters = iters;
+  cr.addr = std::move(addr);
+  // capturing ran the host bookkeeping of the schedule: the X_k / X_{k-1}
+  // pointers are now where a replay leaves them
+  for (auto& e : engines) cr.post.emplace_back(e->cur, e->prev);
+  cr.arena_post = {cur, prev};
 }
 
 double Batch::run(int iters, int mode) {
+  if (iters <= 0) fail("batch: iteration count must be positive");
+  if (mode < BATCH_SEQUENTIAL || mode > BATCH_GROUPED_GRAPH) fail("batch: unknown mode " + std::to_string(mode));
   const int S = (int)streams.size() - 1;
   hipStream_t root = streams[S];
-  for (auto& e : engines) e->timing = false;
-  if (mode == 2) capture(iters);
+  const bool grouped = mode == BATCH_GROUPED || mode == BATCH_GROUPED_GRAPH;
+  // everything is allocated before any capture
+  for (auto& e : engines) {
+    e->timing = false;
+    e->prepare(iters);
+  }
+  g_lam.ensure(sizeof(double) * 4 * engines.size() * (size_t)iters);
+  if (grouped)
+    for (const XGroup& g : xg)
+      if (x_update_rows_per_block(g.nx) != g.rpb) {  // the autotuner changed an x launch shape
+        build_groups();
+        break;
+      }
+  if (mode == BATCH_GRAPH) capture(cap_streams, iters, false);
+  if (mode == BATCH_GROUPED_GRAPH) capture(cap_grouped, iters, true);
   CDN_HIP(hipDeviceSynchronize());
   hipEvent_t t0, t1;
   CDN_HIP(hipEventCreate(&t0));
   CDN_HIP(hipEventCreate(&t1));
   CDN_HIP(hipEventRecord(t0, root));
-  if (mode == 0) {
-    for (int i = 0; i < (int)engines.size(); ++i) enqueue(root, i, iters);
-  } else if (mode == 1) {
-    CDN_HIP(hipEventRecord(events[0], root));
-    for (int s = 0; s < S; ++s) CDN_HIP(hipStreamWaitEvent(streams[s], events[0], 0));
-    for (int i = 0; i < (int)engines.size(); ++i) enqueue(streams[i % S], i, iters);
-    for (int s = 0; s < S; ++s) {
-      CDN_HIP(hipEventRecord(events[1 + s], streams[s]));
-      CDN_HIP(hipStreamWaitEvent(root, events[1 + s], 0));
-    }
+  if (mode == BATCH_SEQUENTIAL) {
+    for (int i = 0; i < (int)engines.size(); ++i) enqueue_engine(root, i, iters);
+  } else if (mode == BATCH_STREAMS) {
+    enqueue_streams(iters);
+  } else if (mode == BATCH_GROUPED) {
+    enqueue_grouped(root, iters);
   } else {
-    CDN_HIP(hipGraphLaunch(exec, root));
+    CapturedRun& cr = mode == BATCH_GRAPH ? cap_streams : cap_grouped;
+    CDN_HIP(hipGraphLaunch(cr.exec, root));
     for (size_t i = 0; i < engines.size(); ++i) {
-      engines[i]->cur = graph_post[i].first;
-      engines[i]->prev = graph_post[i].second;
+      engines[i]->cur = cr.post[i].first;
+      engines[i]->prev = cr.post[i].second;
+    }
+    if (mode == BATCH_GROUPED_GRAPH) {
+      cur = cr.arena_post.first;
+      prev = cr.arena_post.second;
     }
   }
   CDN_HIP(hipEventRecord(t1, root));
@@ -114,7 +358,28 @@ double Batch::run(int iters, int mode) {
   CDN_HIP(hipEventElapsedTime(&ms, t0, t1));
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
+  last_mode = mode;
+  lam_iters = iters;
   return ms;
+}
+
+std::vector<double> Batch::lambdas(int i, int iters) const {
+  if (i < 0 || i >= (int)engines.size()) fail("batch: stack index out of range");
+  if (last_mode < 0) fail("batch: no run yet");
+  if (iters > lam_iters) fail("batch: the last run had fewer iterations");
+  std::vector<double> out((size_t)iters, 0.0);
+  const bool grouped = last_mode == BATCH_GROUPED || last_mode == BATCH_GROUPED_GRAPH;
+  const size_t n = engines.size();
+  std::vector<double> h;
+  if (grouped) {
+    h.resize(4 * n * (size_t)iters);
+    CDN_HIP(hipMemcpy(h.data(), g_lam.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  } else {
+    h.resize(4 * (size_t)iters);
+    CDN_HIP(hipMemcpy(h.data(), engines[i]->lam.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  }
+  for (int k = 2; k < iters; ++k) out[k] = grouped ? h[4 * (n * k + pos[i])] : h[4 * (size_t)k];
+  return out;
 }
 
 }  // namespace cdn

@@ -46,6 +46,11 @@ bool g_first = true;
 // output_nz survives RL_cleanup in the reference and feeds Deskew_interface
 // (src/RL-Biggs-Andrews.cpp:459,741)
 unsigned g_output_nz = 0;
+// flat start of RL_interface: Y_0 filled with a value (the CLI's FlatStart,
+// src/linearDecon.cpp:1143-1145 and src/RL-Biggs-Andrews.cpp:305-316) until
+// RL_cleanup
+bool g_flat_on = false;
+float g_flat_value = 0.f;
 
 hipStream_t make_stream() {
   hipStream_t s;
@@ -280,7 +285,8 @@ int RL_interface(const unsigned short* const raw_data, int nx, int ny, int nz, f
         kscales(S, cx, ks);
         S.eng.setup((int)cx, (int)cy, (int)cz, S.otf, ks, st);
         S.eng.load_raw(X, true, st);
-        S.eng.iterate(nIters, 0, false, st, nullptr);
+        if (g_flat_on) fill(st, S.eng.y.as<float>(), S.eng.total_vox(), g_flat_value);
+        S.eng.iterate(nIters, 0, g_flat_on, st, nullptr);
         X = S.eng.result();
       }
     }
@@ -312,7 +318,17 @@ void RL_cleanup(void) {
   // src/RL-Biggs-Andrews.cpp:707-715
   g_intensity0 = 0.;
   g_first = true;
+  g_flat_on = false;
   g_s.reset();
+}
+
+int cudaDecon_set_flat_start(int enable, float value) {
+  return guard([&]() -> int {
+    if (enable && !std::isfinite(value)) fail("cudaDecon_set_flat_start: non-finite value");
+    g_flat_on = enable != 0;
+    g_flat_value = value;
+    return 1;
+  });
 }
 
 void cuda_reset(void) {
@@ -408,6 +424,36 @@ int camcor_interface(const unsigned short* const raw_data, int nx, int ny, int n
     copy_to_device(C.st, C.in.p, raw_data, sizeof(uint16_t) * n);
     camcor(C.st, C.in.as<uint16_t>(), C.cam.as<float>(), nx, ny, nz, C.out.as<uint16_t>());
     copy_to_host(C.st, result, C.out.p, sizeof(uint16_t) * n);
+    return 1;
+  });
+}
+
+void* cudaDecon_device_malloc(size_t bytes) {
+  void* p = nullptr;
+  guard([&]() -> int {
+    if (bytes == 0) fail("cudaDecon_device_malloc: zero bytes");
+    CDN_HIP(hipMalloc(&p, bytes));
+    return 1;
+  });
+  return p;
+}
+
+int cudaDecon_device_free(void* p) {
+  return guard([&]() -> int {
+    if (p) CDN_HIP(hipFree(p));
+    return 1;
+  });
+}
+
+int cudaDecon_device_copy(void* dst, const void* src, size_t bytes, int kind) {
+  return guard([&]() -> int {
+    if (!dst || !src) fail("cudaDecon_device_copy: null pointer");
+    const hipMemcpyKind k = kind == 1   ? hipMemcpyHostToDevice
+                            : kind == 2 ? hipMemcpyDeviceToHost
+                            : kind == 3 ? hipMemcpyDeviceToDevice
+                                        : hipMemcpyDefault;
+    if (kind < 1 || kind > 3) fail("cudaDecon_device_copy: kind must be 1, 2 or 3");
+    CDN_HIP(hipMemcpy(dst, src, bytes, k));
     return 1;
   });
 }
@@ -927,16 +973,59 @@ cudaDecon_batch* cudaDecon_batch_create(int n, const int* dims_zyx, const float*
                                         int otf_depth, const float* kscales, int nstreams) {
   cudaDecon_batch* out = nullptr;
   guard([&]() -> int {
-    if (!dims_zyx || !otf || !kscales) fail("cudaDecon_batch_create: null argument");
+    if (!dims_zyx || !otf || !kscales || n <= 0) fail("cudaDecon_batch_create: bad argument");
     auto h = std::make_unique<cudaDecon_batch>();
-    OtfTable T;
-    T.set_from_image(otf, otf_width, otf_height, otf_depth, 1.f, 1.f);
-    h->b.setup(n, dims_zyx, T, kscales, nstreams);
+    std::vector<OtfTable> T(1);
+    T[0].set_from_image(otf, otf_width, otf_height, otf_depth, 1.f, 1.f);
+    const std::vector<int> table_of((size_t)n, 0);
+    h->b.setup(n, dims_zyx, std::move(T), table_of.data(), kscales, nstreams);
     out = h.release();
     return 1;
   });
   return out;
 }
+
+cudaDecon_batch* cudaDecon_batch_create_tables(int n, const int* dims_zyx, int ntables, const float* otf_data,
+                                               const int* otf_whd, const int* table_of, const float* kscales,
+                                               int nstreams) {
+  cudaDecon_batch* out = nullptr;
+  guard([&]() -> int {
+    if (!dims_zyx || !otf_data || !otf_whd || !table_of || !kscales || n <= 0 || ntables <= 0)
+      fail("cudaDecon_batch_create_tables: bad argument");
+    auto h = std::make_unique<cudaDecon_batch>();
+    std::vector<OtfTable> T((size_t)ntables);
+    size_t off = 0;
+    for (int t = 0; t < ntables; ++t) {
+      const int w = otf_whd[3 * t], ht = otf_whd[3 * t + 1], d = otf_whd[3 * t + 2];
+      if (w <= 0 || ht <= 0 || d <= 0) fail("cudaDecon_batch_create_tables: OTF " + std::to_string(t) + " is empty");
+      T[t].set_from_image(otf_data + off, w, ht, d, 1.f, 1.f);
+      off += (size_t)w * ht * d;
+    }
+    h->b.setup(n, dims_zyx, std::move(T), table_of, kscales, nstreams);
+    out = h.release();
+    return 1;
+  });
+  return out;
+}
+
+int cudaDecon_batch_lambdas(cudaDecon_batch* h, int i, int iters, double* out) {
+  return guard([&]() -> int {
+    if (!h || !out || iters <= 0) fail("cudaDecon_batch_lambdas: bad argument");
+    const std::vector<double> l = h->b.lambdas(i, iters);
+    std::copy(l.begin(), l.end(), out);
+    return 1;
+  });
+}
+
+int cudaDecon_batch_set_grid_cap(cudaDecon_batch* h, long max_workgroups) {
+  return guard([&]() -> int {
+    if (!h || max_workgroups < 0) fail("cudaDecon_batch_set_grid_cap: bad argument");
+    h->b.grid_cap = max_workgroups;
+    return 1;
+  });
+}
+
+int cudaDecon_batch_launches(cudaDecon_batch* h) { return h ? h->b.grouped_launches() : 0; }
 
 int cudaDecon_batch_set_raw(cudaDecon_batch* h, int i, const float* raw) {
   return guard([&]() -> int {
@@ -973,7 +1062,7 @@ int cudaDecon_batch_set_raw_block(cudaDecon_batch* h, int i, const float* block,
 
 int cudaDecon_batch_run(cudaDecon_batch* h, int iters, int mode, double* ms) {
   return guard([&]() -> int {
-    if (!h || iters <= 0 || mode < 0 || mode > 2) fail("cudaDecon_batch_run: bad argument");
+    if (!h || iters <= 0 || mode < 0 || mode > 4) fail("cudaDecon_batch_run: bad argument");
     const double t = h->b.run(iters, mode);
     if (ms) *ms = t;
     return 1;

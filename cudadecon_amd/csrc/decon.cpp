@@ -15,31 +15,59 @@ namespace cdn {
 
 void DevBuf::ensure(size_t b) {
   if (b <= bytes && p) return;
+  if (!owned && p)
+    fail("a caller-provided device buffer of " + std::to_string(bytes) + " bytes is too small for " +
+         std::to_string(b));
   release();
   if (b == 0) return;
   CDN_HIP(hipMalloc(&p, b));
   bytes = b;
 }
 
+void DevBuf::bind(void* q, size_t b) {
+  release();
+  p = q;
+  bytes = b;
+  owned = false;
+}
+
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && owned) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+  owned = true;
 }
 
 // ---------------------------------------------------------------------------
 // Fft3
 // ---------------------------------------------------------------------------
-static void upload_twiddles(DevBuf& b, int n, hipStream_t st) {
+void* DevRegion::take(size_t b) {
+  const size_t off = (used + 255) / 256 * 256;
+  if (off > size || b > size - off)
+    fail("device region of " + std::to_string(size) + " bytes is too small (" + std::to_string(off + b) +
+         " needed)");
+  used = off + b;
+  return base + off;
+}
+
+// a buffer of b bytes: allocated, or the region's next piece
+static void obtain(DevBuf& buf, size_t b, DevRegion* region) {
+  if (region)
+    region->give(buf, b);
+  else
+    buf.ensure(b);
+}
+
+static void upload_twiddles(DevBuf& b, int n, hipStream_t st, DevRegion* region) {
   const int cnt = fft_twiddle_count(n);
   std::vector<float2> h(cnt);
   fft_twiddles(n, h.data());
-  b.ensure(sizeof(float2) * cnt);
+  obtain(b, sizeof(float2) * cnt, region);
   CDN_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(float2) * cnt, hipMemcpyHostToDevice, st));
   CDN_HIP(hipStreamSynchronize(st));
 }
 
-void Fft3::setup(int x, int y, int z, hipStream_t st) {
+void Fft3::setup(int x, int y, int z, hipStream_t st, DevRegion* region) {
   for (int n : {x, y, z}) {
     if (!fft_length_supported(n))
       fail("FFT length " + std::to_string(n) + " is not supported (2,3,5,7-smooth, at most 16384)");
@@ -49,12 +77,12 @@ void Fft3::setup(int x, int y, int z, hipStream_t st) {
   ny = y;
   nz = z;
   nxh = nx / 2 + 1;
-  pitch = (nxh + 15) / 16 * 16;
+  pitch = pitch_of(nx);
   nvox = (long)nx * ny * nz;
   nspec = (long)nz * ny * pitch;
-  upload_twiddles(twx, nx, st);
-  upload_twiddles(twy, ny, st);
-  upload_twiddles(twz, nz, st);
+  upload_twiddles(twx, nx, st, region);
+  upload_twiddles(twy, ny, st, region);
+  upload_twiddles(twz, nz, st, region);
   fft_prepare_cols(ny, nxh);
   fft_prepare_cols(nz, nxh);
 }
@@ -81,7 +109,7 @@ void Fft3::c2r(hipStream_t st, float2* S, float* out) const {
 // ---------------------------------------------------------------------------
 // OTF
 // ---------------------------------------------------------------------------
-void OtfDev::setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st) {
+void OtfDev::setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st, DevRegion* region) {
   ra = (t.nyotf == 1);
   ks[0] = kscale[0];
   ks[1] = kscale[1];
@@ -112,7 +140,7 @@ void OtfDev::setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t
         h[(size_t)r * nz + kz] = v;
       }
     }
-    tz.ensure(sizeof(float2) * h.size());
+    obtain(tz, sizeof(float2) * h.size(), region);
     CDN_HIP(hipMemcpyAsync(tz.p, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, st));
     // the two rows a radius interpolates between, side by side
     if (nr >= 2) {
@@ -122,11 +150,11 @@ void OtfDev::setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t
           const float2 a = h[(size_t)r * nz + kz], b = h[(size_t)(r + 1) * nz + kz];
           h2[(size_t)r * nz + kz] = make_float4(a.x, a.y, b.x, b.y);
         }
-      tz2.ensure(sizeof(float4) * h2.size());
+      obtain(tz2, sizeof(float4) * h2.size(), region);
       CDN_HIP(hipMemcpyAsync(tz2.p, h2.data(), sizeof(float4) * h2.size(), hipMemcpyHostToDevice, st));
     }
   } else {
-    raw.ensure(sizeof(float) * t.data.size());
+    obtain(raw, sizeof(float) * t.data.size(), region);
     CDN_HIP(hipMemcpyAsync(raw.p, t.data.data(), sizeof(float) * t.data.size(), hipMemcpyHostToDevice, st));
   }
   CDN_HIP(hipStreamSynchronize(st));
@@ -162,23 +190,36 @@ static void check_stack_rows(int nx, long stack_rows, int nstacks) {
 }
 
 void RLEngine::setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st,
-                     int nst) {
+                     int nst, const EngineViews* views, DevRegion* region) {
   if (nst < 1) fail("an engine needs at least one stack");
   check_stack_rows(nx, (long)ny * nz, nst);
-  if (!fft.matches(nx, ny, nz)) fft.setup(nx, ny, nz, st);
+  if (region || !fft.matches(nx, ny, nz)) fft.setup(nx, ny, nz, st, region);
   nstacks = nst;
   table = t;
-  otf.setup(table, kscale, nz, st);
+  otf.setup(table, kscale, nz, st, region);
   const size_t vb = sizeof(float) * (size_t)fft.nvox * nstacks;
-  raw.ensure(vb);
-  xa.ensure(vb);
-  xb.ensure(vb);
-  y.ensure(vb);
-  g.ensure(vb);
-  spec.ensure(sizeof(float2) * (size_t)fft.nspec * nstacks);
+  const size_t sb = sizeof(float2) * (size_t)fft.nspec * nstacks;
+  if (views) {
+    raw.bind(views->raw, vb);
+    xa.bind(views->xa, vb);
+    xb.bind(views->xb, vb);
+    y.bind(views->y, vb);
+    g.bind(views->g, vb);
+    spec.bind(views->spec, sb);
+  } else {
+    for (DevBuf* b : {&raw, &xa, &xb, &y, &g})
+      if (!b->owned) b->release();  // an earlier setup's views
+    if (!spec.owned) spec.release();
+    raw.ensure(vb);
+    xa.ensure(vb);
+    xb.ensure(vb);
+    y.ensure(vb);
+    g.ensure(vb);
+    spec.ensure(sb);
+  }
   nb_part = x_update_blocks(nx, (long)ny * nz);
   // room for any x-pass launch shape the autotuner may install later
-  part.ensure(sizeof(double) * 2 * (size_t)x_update_blocks_max((long)ny * nz) * nstacks);
+  obtain(part, sizeof(double) * 2 * (size_t)x_update_blocks_max((long)ny * nz) * nstacks, region);
   cur = xa.as<float>();
   prev = xb.as<float>();
 }
@@ -208,6 +249,10 @@ void RLEngine::reset(hipStream_t st) {
   CDN_HIP(hipMemcpyAsync(xa.p, raw.p, sizeof(float) * (size_t)total_vox(), hipMemcpyDeviceToDevice, st));
   cur = xa.as<float>();
   prev = xb.as<float>();
+}
+
+void RLEngine::bind_lambdas(DevRegion& region, int n) {
+  region.give(lam, sizeof(double) * 4 * (size_t)nstacks * (size_t)std::max(n, 1));
 }
 
 void RLEngine::prepare(int n) { lam.ensure(sizeof(double) * 4 * (size_t)nstacks * (size_t)std::max(n, 1)); }

@@ -10,18 +10,55 @@
 
 namespace cdn {
 
-// Owned device allocation that only grows (kept across calls; freed by the
-// owner's destructor or release()).
+// Device buffer: an owned allocation that only grows (kept across calls;
+// freed by the owner's destructor or release()), or a view of memory the
+// caller owns (bind), which is never freed here and never grows.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned) {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.owned = true;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      owned = o.owned;
+      o.p = nullptr;
+      o.bytes = 0;
+      o.owned = true;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
-  void ensure(size_t b);
+  void ensure(size_t b);         // a view must already hold b bytes
+  void bind(void* q, size_t b);  // view q[0, b)
   void release();
   template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Pieces of one caller-provided device region, 256-byte aligned, handed
+// out in order (a bump allocator; nothing is ever freed back).
+struct DevRegion {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  void* take(size_t b);
+  // bind buf to the next b bytes
+  void give(DevBuf& buf, size_t b) { buf.bind(take(b), b); }
+};
+
+// Where an engine's volumes and spectrum live when the caller provides them
+// (each nstacks stacks long): raw, the X ping-pong, Y, G, the spectrum.
+struct EngineViews {
+  float *raw, *xa, *xb, *y, *g;
+  float2* spec;
 };
 
 // 3-D R2C / C2R plan for one (nx, ny, nz) grid.
@@ -32,7 +69,10 @@ struct Fft3 {
   long nspec = 0;          // nz ny pitch (allocated complex elements)
   DevBuf twx, twy, twz;
   bool matches(int x, int y, int z) const { return nx == x && ny == y && nz == z; }
-  void setup(int x, int y, int z, hipStream_t st);
+  // row pitch (complex elements) of the half spectrum of an x length
+  static long pitch_of(int x) { return (x / 2 + 1 + 15) / 16 * 16; }
+  // twiddle tables allocated, or taken from `region` when given
+  void setup(int x, int y, int z, hipStream_t st, DevRegion* region = nullptr);
   void r2c(hipStream_t st, const float* in, float2* S) const;   // unnormalised
   void c2r(hipStream_t st, float2* S, float* out) const;        // unnormalised, consumes S
   void col(hipStream_t st, float2* S, int axis, bool inverse) const;  // axis 1 = y, 2 = z
@@ -42,7 +82,7 @@ struct OtfDev {
   int ra = 1, nz_for = -1;
   float ks[3] = {0, 0, 0};
   DevBuf raw, tz, tz2;
-  void setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st);
+  void setup(const OtfTable& t, const float kscale[3], int nz, hipStream_t st, DevRegion* region = nullptr);
   OtfParams params(const OtfTable& t, double inv_n) const;
 };
 
@@ -68,7 +108,13 @@ struct RLEngine {
   double pass_ms[PASS_COUNT] = {};
   long pass_calls[PASS_COUNT] = {};
 
-  void setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st, int nstacks = 1);
+  // views: the volumes and spectrum live there; region: every other device
+  // buffer (twiddles, OTF tables, partials) is taken from it, nothing is
+  // allocated (then call bind_lambdas before iterating)
+  void setup(int nx, int ny, int nz, const OtfTable& t, const float kscale[3], hipStream_t st, int nstacks = 1,
+             const EngineViews* views = nullptr, DevRegion* region = nullptr);
+  // the factor slots of up to n iterations, from the region
+  void bind_lambdas(DevRegion& region, int n);
   long stack_vox() const { return fft.nvox; }
   long total_vox() const { return fft.nvox * nstacks; }
   // Stage the (preprocessed) stacks: R := in, X_0 := in (all stacks, stack

@@ -6,6 +6,7 @@
 #include "fft_kernels.hpp"
 #include "jit.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -79,7 +80,11 @@ static inline unsigned grid1(long n) {
 namespace {
 std::mutex g_vmu;
 std::map<int, FftVariant> g_variants;
+thread_local int t_builtin_only = 0;  // > 0: no run-time compiled variant
 }  // namespace
+
+BuiltinShapesOnly::BuiltinShapesOnly() { ++t_builtin_only; }
+BuiltinShapesOnly::~BuiltinShapesOnly() { --t_builtin_only; }
 
 void fft_set_variant(int n, const FftVariant& v) {
   std::lock_guard<std::mutex> lk(g_vmu);
@@ -89,6 +94,7 @@ void fft_set_variant(int n, const FftVariant& v) {
     g_variants[n] = v;
 }
 bool fft_get_variant(int n, FftVariant* v) {
+  if (t_builtin_only > 0) return false;
   std::lock_guard<std::mutex> lk(g_vmu);
   auto it = g_variants.find(n);
   if (it == g_variants.end()) return false;
@@ -104,26 +110,35 @@ void fft_clear_variants() {
   hipLaunchKernelGGL(kernel<N>, dim3(grid), dim3(block), 0, st, __VA_ARGS__)
 
 void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam, int mode,
-                  float2* S, long rows, long pitch, const float2* tw, long stack_rows) {
+                  float2* S, long rows, long pitch, const float2* tw, long stack_rows, const long* row_starts,
+                  int nstk) {
   FftVariant tv;
   const bool tuned = fft_get_variant(nx, &tv);
   const XShape xs = x_shape(nx, tuned ? tv.xthreads : -1, XK_R2C);
   const unsigned grid = grid1(((rows + 1) / 2 + xs.NL - 1) / xs.NL);
   if (stack_rows <= 0) stack_rows = rows;
   if (stack_rows != rows && stack_rows % 2) fail("launch_x_r2c: rows per stack must be even");
-  void* args[] = {&X, &Xp, &Y, &lam, &mode, &S, &rows, &pitch, &stack_rows, &tw};
+  const bool ragged = row_starts != nullptr;
+  if (ragged && nstk < 1) fail("launch_x_r2c: no stacks");
+  const JitKernel jk = ragged ? JK_X_R2C_RAGGED : JK_X_R2C;
+  void* args[] = {&X, &Xp, &Y, &lam, &mode, &S, &rows, &pitch, &stack_rows, &row_starts, &nstk, &tw};
   if (tuned) {
-    jit_launch(nx, JK_X_R2C, grid, xs.THREADS, st, args, tv);
+    jit_launch(nx, jk, grid, xs.THREADS, st, args, tv);
   } else {
     switch (nx) {
-#define CDN_CASE(N)                                                                     \
-  case N:                                                                               \
-    CDN_LAUNCH(k_x_r2c, N, grid, xs.THREADS, X, Xp, Y, lam, mode, S, rows, pitch, stack_rows, tw); \
+#define CDN_CASE(N)                                                                                            \
+  case N:                                                                                                      \
+    if (ragged)                                                                                                \
+      hipLaunchKernelGGL((k_x_r2c<N, true>), dim3(grid), dim3(xs.THREADS), 0, st, X, Xp, Y, lam, mode, S, rows, \
+                         pitch, stack_rows, row_starts, nstk, tw);                                             \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_x_r2c<N, false>), dim3(grid), dim3(xs.THREADS), 0, st, X, Xp, Y, lam, mode, S,      \
+                         rows, pitch, stack_rows, row_starts, nstk, tw);                                       \
     break;
       CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
       default:
-        jit_launch(nx, JK_X_R2C, grid, xs.THREADS, st, args);
+        jit_launch(nx, jk, grid, xs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());
@@ -223,6 +238,7 @@ int launch_x_update(int nx, hipStream_t st, const float2* S, const float* Y, flo
 // bitwise the same results.  (Measured: 8% on stacks of 64 x 128 x 128; a
 // wider single tile, 129 columns for 3 built-in tiles, was slower.)
 bool fft_balanced_cols(int len, long ncols, FftVariant* v) {
+  if (t_builtin_only > 0) return false;
   const CShape d = c_shape(len);
   if (ncols <= d.L || ncols >= 2 * d.L || 4 * (ncols - d.L) > d.L) return false;
   const long budget = len >= 768 ? 160L * 1024 : 80L * 1024;
@@ -301,6 +317,81 @@ void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_st
       default:
         jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args);
     }
+  }
+  CDN_HIP(hipGetLastError());
+}
+
+long col_tile_cols(int len) { return c_shape(len).L; }
+
+void fft_prepare_ragged(int len) {
+  if (fft_length_precompiled(len)) return;
+  for (JitKernel k : {JK_X_R2C_RAGGED, JK_COL_RAGGED_F, JK_COL_RAGGED_I, JK_Z_RAGGED, JK_COL_RAGGED_FP,
+                      JK_COL_RAGGED_IP, JK_Z_RAGGED_P, JK_X_C2R, JK_X_LRCORE, JK_X_UPDATE})
+    jit_launch(len, k, 0, 1, nullptr, nullptr);
+}
+
+// grid of a launch over `total` items: one workgroup per item, or at most
+// `cap` workgroups (a multiple of 8) that walk the items
+static unsigned ragged_grid(long total, long cap) {
+  if (cap > 0 && cap < total) return grid1(std::max(8L, cap / 8 * 8));
+  return grid1(total);
+}
+
+// the kernel instance for (inverse, persistent)
+template <int N>
+static void col_ragged_n(bool inv, bool pers, unsigned grid, unsigned threads, hipStream_t st, const ColSeg* segs,
+                         int nseg, long total, const float2* tw) {
+  if (inv && pers)
+    hipLaunchKernelGGL((k_col_ragged<N, true, true>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, tw);
+  else if (inv)
+    hipLaunchKernelGGL((k_col_ragged<N, true, false>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, tw);
+  else if (pers)
+    hipLaunchKernelGGL((k_col_ragged<N, false, true>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, tw);
+  else
+    hipLaunchKernelGGL((k_col_ragged<N, false, false>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, tw);
+}
+
+void launch_col_ragged(int len, int inverse, hipStream_t st, const ColSeg* segs, int nseg, long total,
+                       const float2* tw, long grid_cap) {
+  if (total <= 0) return;
+  const CShape cs = c_shape(len);
+  const unsigned grid = ragged_grid(total, grid_cap);
+  const bool pers = grid < total;
+  void* args[] = {&segs, &nseg, &total, &tw};
+  switch (len) {
+#define CDN_CASE(N)                                                                      \
+  case N:                                                                                \
+    col_ragged_n<N>(inverse != 0, pers, grid, cs.THREADS, st, segs, nseg, total, tw); \
+    break;
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      jit_launch(len, pers ? (inverse ? JK_COL_RAGGED_IP : JK_COL_RAGGED_FP) : (inverse ? JK_COL_RAGGED_I : JK_COL_RAGGED_F),
+                 grid, cs.THREADS, st, args);
+  }
+  CDN_HIP(hipGetLastError());
+}
+
+void launch_z_ragged(int nz, hipStream_t st, const ZSeg* segs, int nseg, long total, int conj, const float2* tw,
+                     long grid_cap) {
+  if (total <= 0) return;
+  const CShape cs = c_shape(nz);
+  const unsigned grid = ragged_grid(total, grid_cap);
+  const bool pers = grid < total;
+  void* args[] = {&segs, &nseg, &total, &conj, &tw};
+  switch (nz) {
+#define CDN_CASE(N)                                                                                               \
+  case N:                                                                                                         \
+    if (pers)                                                                                                     \
+      hipLaunchKernelGGL((k_z_ragged<N, true>), dim3(grid), dim3(cs.THREADS), 0, st, segs, nseg, total, conj, tw); \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_z_ragged<N, false>), dim3(grid), dim3(cs.THREADS), 0, st, segs, nseg, total, conj,     \
+                         tw);                                                                                     \
+    break;
+    CDN_FFT_LENGTHS(CDN_CASE)
+#undef CDN_CASE
+    default:
+      jit_launch(nz, pers ? JK_Z_RAGGED_P : JK_Z_RAGGED, grid, cs.THREADS, st, args);
   }
   CDN_HIP(hipGetLastError());
 }

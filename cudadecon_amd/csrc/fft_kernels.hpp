@@ -27,9 +27,11 @@ void fft_twiddles(int n, float2* out);
 
 // R2C along x of row pairs with the prediction fused in; writes S.  Rows
 // come in stacks of stack_rows (even; 0: one stack), stack s using the
-// factor lam[4 s].
+// factor lam[4 s]; or, with row_starts (device, nstk ascending even rows,
+// the first 0), in stacks of different heights.
 void launch_x_r2c(int nx, hipStream_t st, const float* X, const float* Xp, float* Y, const double* lam,
-                  int mode, float2* S, long rows, long pitch, const float2* tw, long stack_rows = 0);
+                  int mode, float2* S, long rows, long pitch, const float2* tw, long stack_rows = 0,
+                  const long* row_starts = nullptr, int nstk = 0);
 // plain C2R along x (ignores imag of kx = 0 and kx = nx/2).
 void launch_x_c2r(int nx, hipStream_t st, const float2* S, float* out, long rows, long pitch,
                   const float2* tw);
@@ -58,6 +60,13 @@ bool fft_balanced_cols(int len, long ncols, FftVariant* v);
 // compile and load the kernels those launches will need (no GPU work)
 void fft_prepare_cols(int len, long ncols);
 bool fft_get_variant(int n, FftVariant* v);
+// While one exists on this thread, every launch uses the built-in kernels of
+// its length: no tuned or single-tile variant, whose run-time compiled code
+// would have to be loaded (column results are bitwise the same).
+struct BuiltinShapesOnly {
+  BuiltinShapesOnly();
+  ~BuiltinShapesOnly();
+};
 void fft_clear_variants();
 
 // Column FFT (forward or inverse) over `len` points at element stride
@@ -69,5 +78,18 @@ void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols,
 void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_stride, long outer_stride,
                      long ny, int conj, const OtfParams& otf, const float2* tw, int nstacks = 1,
                      long stack_stride = 0);
+
+// Column passes spanning stacks of different shapes: one launch for every
+// stack whose transform length is `len`, segs (device) one per stack with
+// items numbered by tiles of col_tile_cols(len) columns, `total` items in
+// all.  grid_cap > 0 bounds the workgroups, which then walk the items.
+long col_tile_cols(int len);
+// compile and load the run-time kernels the ragged launches of a length not
+// built in will use (no GPU work; call before any stream capture)
+void fft_prepare_ragged(int len);
+void launch_col_ragged(int len, int inverse, hipStream_t st, const ColSeg* segs, int nseg, long total,
+                       const float2* tw, long grid_cap = 0);
+void launch_z_ragged(int nz, hipStream_t st, const ZSeg* segs, int nseg, long total, int conj, const float2* tw,
+                     long grid_cap = 0);
 
 }  // namespace cdn

@@ -332,11 +332,26 @@ __device__ __forceinline__ void merge_load(float2 (&v)[(XCfg<N, K>::E)], float2*
   }
 }
 
-template <int N>
+// index of the last of n ascending starts that is <= x (starts[0] <= x)
+__device__ __forceinline__ int find_start(const long* __restrict__ starts, int n, long x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// RAGGED: the rows are those of nstk stacks of different heights, stack s
+// starting at row row_starts[s] (even); otherwise stacks of stack_rows rows.
+template <int N, bool RAGGED = false>
 __global__ void __launch_bounds__((XCfg<N, XK_R2C>::THREADS))
     k_x_r2c(const float* __restrict__ X, const float* __restrict__ Xp, float* __restrict__ Y,
             const double* __restrict__ lam, int mode, float2* __restrict__ S, long rows, long pitch,
-            long stack_rows, const float2* __restrict__ tw) {
+            long stack_rows, const long* __restrict__ row_starts, int nstk, const float2* __restrict__ tw) {
   using C = XCfg<N, XK_R2C>;
   __shared__ float2 lds[C::NL * C::LS];
   const int tid = threadIdx.x;
@@ -344,8 +359,15 @@ __global__ void __launch_bounds__((XCfg<N, XK_R2C>::THREADS))
   const long r0 = 2 * ((long)blockIdx.x * C::NL + l);
   const bool v0 = r0 < rows, v1 = r0 + 1 < rows;
   float2* line = lds + l * C::LS;
-  // the factor of this row pair's stack (stack_rows rows per stack, even)
-  const float lamf = (mode == XIN_PRED) ? (float)lam[4 * (v0 ? r0 / stack_rows : 0)] : 0.f;
+  // the factor of this row pair's stack
+  long stk = 0;
+  if (mode == XIN_PRED && v0) {
+    if constexpr (RAGGED)
+      stk = find_start(row_starts, nstk, r0);
+    else
+      stk = r0 / stack_rows;
+  }
+  const float lamf = (mode == XIN_PRED) ? (float)lam[4 * stk] : 0.f;
   float2 v[C::E];
   // all loads are issued before any store
   const float* src = (mode == XIN_SEED) ? Y : X;
@@ -567,15 +589,13 @@ __device__ __forceinline__ void col_line(float2 (&v)[CCfg<N>::E], float2* lds, i
   }
 }
 
+// tile lb (slab-major) of a column pass; lds as declared by the kernel
 template <int N, bool INV>
-__global__ void __launch_bounds__(CCfg<N>::THREADS)
-    k_col(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles,
-          const float2* __restrict__ tw) {
+__device__ __forceinline__ void col_tile(float2* lds, float2* __restrict__ S, long ncols, long line_stride,
+                                         long outer_stride, long ntiles, long lb, const float2* __restrict__ tw) {
   using C = CCfg<N>;
-  __shared__ float2 lds[C::LDS];
   const int tid = threadIdx.x;
   const int l = tid % C::L, p = tid / C::L;
-  const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
   const long outer = lb / ntiles, tile = lb - outer * ntiles;
   const long c = tile * C::L + l;
   const bool valid = c < ncols;
@@ -589,6 +609,50 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
 #pragma unroll
     for (int m = 0; m < C::E; ++m) st_stream<NTCS>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
   }
+}
+
+template <int N, bool INV>
+__global__ void __launch_bounds__(CCfg<N>::THREADS)
+    k_col(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles,
+          const float2* __restrict__ tw) {
+  __shared__ float2 lds[CCfg<N>::LDS];
+  col_tile<N, INV>(lds, S, ncols, line_stride, outer_stride, ntiles,
+                   xcd_logical_block<CCfg<N>::L>(blockIdx.x, gridDim.x), tw);
+}
+
+// segment holding work item `it`: the last whose item0 <= it (segs[0].item0 = 0)
+template <class Seg>
+__device__ __forceinline__ int find_seg(const Seg* __restrict__ segs, int n, long it) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].item0 <= it)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// The column transform of every stack whose columns have length N, stacks
+// of different shapes in one launch (ColSeg per stack).  One workgroup per
+// item; or (PERSIST) a persistent grid of gridDim.x workgroups, a multiple of
+// 8 so an item runs on the XCD the block order assumes, workgroup b taking
+// items b, b + gridDim.x, ...  (The loop keeps more values live: 112 against
+// 61 VGPRs at length 1024, so it is an option, not the default.)
+template <int N, bool INV, bool PERSIST>
+__global__ void __launch_bounds__(CCfg<N>::THREADS)
+    k_col_ragged(const ColSeg* __restrict__ segs, int nseg, long total, const float2* __restrict__ tw) {
+  __shared__ float2 lds[CCfg<N>::LDS];
+  long it = blockIdx.x;
+  do {
+    const long lb = xcd_logical_block<CCfg<N>::L>(it, total);
+    const ColSeg& g = segs[find_seg(segs, nseg, lb)];
+    col_tile<N, INV>(lds, g.S, g.ncols, g.line_stride, g.outer_stride, g.ntiles, lb - g.item0, tw);
+    if constexpr (!PERSIST) break;
+    __syncthreads();  // the next item's exchanges reuse lds
+    it += gridDim.x;
+  } while (it < total);
 }
 
 // Waves per SIMD the z filter is compiled for (1: the compiler's choice).
@@ -634,142 +698,35 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
   constexpr int PER = RS > 0 ? (RS * N + C::THREADS - 1) / C::THREADS : 1;
   __shared__ float2 lds[C::LDS];
   __shared__ float2 otf_rows[RS > 0 ? RS * N : 1];
-  const int tid = threadIdx.x;
-  const int l = tid % C::L, p = tid / C::L;
   const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
-  // slabs are (stack, ky) pairs, ky fastest
-  const long slab = lb / ntiles, tile = lb - slab * ntiles;
-  const long stack = slab / ny, outer = slab - stack * ny;
-  const long c = tile * C::L + l;
-  const bool valid = c < ncols;
+#include "zf_tile_body.inc"
+}
 
-  // table rows r_lo .. r_lo + nrows - 1 cover the tile's radii (with a row of
-  // margin on each side); their loads go out before the stack's
-  int r_lo = 0, nrows = 0;
-  float2 stage[PER];
-  if constexpr (RS > 0) {
-    if (otf.ra) {
-      const float ky = zf_ky(outer, ny, otf.kyscale);
-      const float kx0 = (float)(tile * C::L) * otf.kxscale, kx1 = (float)(tile * C::L + C::L - 1) * otf.kxscale;
-      const float kr0 = sqrtf(kx0 * kx0 + ky * ky), kr1 = sqrtf(kx1 * kx1 + ky * ky);
-      const int nr = otf.ra_tab.nr;
-      if (kr0 < (float)(nr - 1)) {
-        r_lo = max(0, (int)floorf(kr0) - 1);
-        const int r_hi = min(nr - 1, (int)floorf(fminf(kr1, (float)nr)) + 2);
-        nrows = min(RS, r_hi - r_lo + 1);
-      }
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int e = tid + k * C::THREADS;
-        stage[k] = e < nrows * N ? otf.ra_tab.tz[(long)r_lo * N + e] : make_float2(0.f, 0.f);
-      }
+// The z filter of every stack whose z length is N, stacks of different shapes
+// and OTFs in one launch (ZSeg per stack); items and PERSIST as k_col_ragged.
+template <int N, bool PERSIST>
+__global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(zf_waves(N))))
+    k_z_ragged(const ZSeg* __restrict__ segs, int nseg, long total, int conj, const float2* __restrict__ tw) {
+  using C = CCfg<N>;
+  constexpr int RS = ZRows<N>::R;
+  constexpr int PER = RS > 0 ? (RS * N + C::THREADS - 1) / C::THREADS : 1;
+  __shared__ float2 lds[C::LDS];
+  __shared__ float2 otf_rows[RS > 0 ? RS * N : 1];
+  long it = blockIdx.x;
+  do {
+    const long gi = xcd_logical_block<C::L>(it, total);
+    const ZSeg* g = segs + find_seg(segs, nseg, gi);
+    float2* __restrict__ S = g->S;
+    const long ncols = g->ncols, line_stride = g->line_stride, outer_stride = g->outer_stride;
+    const long ntiles = g->ntiles, ny = g->ny, stack_stride = 0, lb = gi - g->item0;
+    const OtfParams& otf = g->otf;
+    {
+#include "zf_tile_body.inc"
     }
-  }
-
-  float2* base = S + stack * stack_stride + outer * outer_stride + c;
-  float2 v[C::E];
-#pragma unroll
-  for (int m = 0; m < C::E; ++m)
-    v[m] = valid ? ld_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride) : make_float2(0.f, 0.f);
-  if constexpr (RS > 0) {
-    // visible after the transform's barriers
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = tid + k * C::THREADS;
-      if (e < nrows * N) otf_rows[e] = stage[k];
-    }
-  }
-  col_line<N, false>(v, lds, l, p, tw);
-
-  const float kx = (float)c * otf.kxscale;
-  const float ky = zf_ky(outer, ny, otf.kyscale);
-  const float sg = conj ? -1.f : 1.f;
-  if (otf.ra) {
-    const float kr = sqrtf(kx * kx + ky * ky);
-    const bool inside = kr < (float)(otf.ra_tab.nr - 1);
-    const int ir = inside ? (int)floorf(kr) : 0;
-    const float ar = kr - (float)ir;
-    const float4* t01 = otf.ra_tab.tz2 + (long)ir * N;
-    const int rr = ir - r_lo;
-    // one source per wavefront: the staged rows when they hold every row
-    // the wavefront's lanes interpolate between
-    bool from_lds = false;
-    if constexpr (RS > 0) from_lds = __all(!inside || (rr >= 0 && rr + 1 < nrows));
-    const float2* s0 = otf_rows + (rr >= 0 && rr < RS ? rr : 0) * N;
-    if (from_lds) {
-#pragma unroll
-      for (int m = 0; m < C::E; ++m) {
-        const int i = p + m * C::TPT;
-        float2 r = make_float2(0.f, 0.f);
-        if (inside) {
-          const float2 a = s0[i], b = s0[N + i];
-          float2 h = make_float2((1.f - ar) * a.x + ar * b.x, (1.f - ar) * a.y + ar * b.y);
-          h.y *= sg;
-          r = cscale(cmul(h, v[m]), otf.inv_n);
-        }
-        v[m] = r;
-      }
-    } else {
-#pragma unroll
-      for (int m = 0; m < C::E; ++m) {
-        const int i = p + m * C::TPT;
-        float2 r = make_float2(0.f, 0.f);
-        if (inside) {
-          const float4 ab = t01[i];
-          float2 h = make_float2((1.f - ar) * ab.x + ar * ab.z, (1.f - ar) * ab.y + ar * ab.w);
-          h.y *= sg;
-          r = cscale(cmul(h, v[m]), otf.inv_n);
-        }
-        v[m] = r;
-      }
-    }
-  } else {
-    const Otf3d& T = otf.tab3;
-    const float kyindex = ky < 0.f ? ky + (float)T.nyotf : ky;
-    // the >= 0 test only excludes |ky| beyond the table, which the reference
-    // would read out of bounds
-    const bool inxy = (kx < (float)(T.nxotf - 1)) && (kyindex < (float)T.nyotf) && (kyindex >= 0.f);
-    const int ix = inxy ? (int)floorf(kx) : 0;
-    const int iy = inxy ? (int)floorf(kyindex) : 0;
-    const int iy1 = (iy + 1) % T.nyotf;
-    const float ax = kx - (float)ix, ay = kyindex - (float)iy;
-    const long nxy = (long)T.nxotf * T.nyotf;
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) {
-      const int i = p + m * C::TPT;
-      const int kzi = (i > (N >> 1)) ? i - N : i;
-      const float kz = (float)kzi * otf.kzscale;
-      const float kzindex = kz < 0.f ? kz + (float)T.nzotf : kz;
-      float2 r = make_float2(0.f, 0.f);
-      if (inxy && kzindex < (float)T.nzotf && kzindex >= 0.f) {
-        const int iz = (int)floorf(kzindex);
-        const int iz1 = (iz + 1) % T.nzotf;
-        const float az = kzindex - (float)iz;
-        auto g = [&](int zz, int yy, int xx) { return T.raw[zz * nxy + (long)yy * T.nxotf + xx]; };
-        const float2 a000 = g(iz, iy, ix), a001 = g(iz, iy, ix + 1), a010 = g(iz, iy1, ix),
-                     a011 = g(iz, iy1, ix + 1), a100 = g(iz1, iy, ix), a101 = g(iz1, iy, ix + 1),
-                     a110 = g(iz1, iy1, ix), a111 = g(iz1, iy1, ix + 1);
-        float2 h;
-        h.x = (1 - az) * (a000.x * (1 - ay) * (1 - ax) + a001.x * (1 - ay) * ax + a010.x * ay * (1 - ax) +
-                          a011.x * ay * ax) +
-              az * (a100.x * (1 - ay) * (1 - ax) + a101.x * (1 - ay) * ax + a110.x * ay * (1 - ax) +
-                    a111.x * ay * ax);
-        h.y = (1 - az) * (a000.y * (1 - ay) * (1 - ax) + a001.y * (1 - ay) * ax + a010.y * ay * (1 - ax) +
-                          a011.y * ay * ax) +
-              az * (a100.y * (1 - ay) * (1 - ax) + a101.y * (1 - ay) * ax + a110.y * ay * (1 - ax) +
-                    a111.y * ay * ax);
-        h.y *= sg;
-        r = cscale(cmul(h, v[m]), otf.inv_n);
-      }
-      v[m] = r;
-    }
-  }
-  __syncthreads();
-  col_line<N, true>(v, lds, l, p, tw);
-  if (valid) {
-#pragma unroll
-    for (int m = 0; m < C::E; ++m) st_stream<NTZ>(base + (long)(p + m * C::TPT) * line_stride, v[m]);
-  }
+    if constexpr (!PERSIST) break;
+    __syncthreads();  // the next item's exchanges and staged rows reuse lds
+    it += gridDim.x;
+  } while (it < total);
 }
 
 }  // namespace cdn

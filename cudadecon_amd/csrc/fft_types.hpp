@@ -34,4 +34,19 @@ struct OtfParams {
   float inv_n;  // 1 / (nx ny nz): the filter's normalisation
 };
 
+// One stack's share of a column pass that spans stacks of different shapes
+// (same transform length): its spectrum, its column geometry, and the first
+// work item (tile) of the launch that belongs to it.  Items of stack s are
+// item0 .. item0 + n_outer ntiles - 1, slab-major.
+struct ColSeg {
+  float2* S;
+  long ncols, line_stride, outer_stride, ntiles, item0;
+};
+// The same for the z filter (slabs are ky rows; the stack's own OTF).
+struct ZSeg {
+  float2* S;
+  long ncols, line_stride, outer_stride, ntiles, ny, item0;
+  OtfParams otf;
+};
+
 }  // namespace cdn

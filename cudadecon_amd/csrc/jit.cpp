@@ -27,9 +27,13 @@ namespace {
 
 constexpr int kMaxJitLength = 16384;
 
-const char* const kExpr[JK_COUNT] = {"cdn::k_x_r2c<%d>",  "cdn::k_x_c2r<%d>",     "cdn::k_x_lrcore<%d>",
-                                     "cdn::k_x_update<%d>", "cdn::k_col<%d, false>", "cdn::k_col<%d, true>",
-                                     "cdn::k_z_filter<%d>"};
+const char* const kExpr[JK_COUNT] = {"cdn::k_x_r2c<%d, false>",   "cdn::k_x_c2r<%d>",
+                                     "cdn::k_x_lrcore<%d>",       "cdn::k_x_update<%d>",
+                                     "cdn::k_col<%d, false>",     "cdn::k_col<%d, true>",
+                                     "cdn::k_z_filter<%d>",       "cdn::k_x_r2c<%d, true>",
+                                     "cdn::k_col_ragged<%d, false, false>", "cdn::k_col_ragged<%d, true, false>",
+                                     "cdn::k_z_ragged<%d, false>", "cdn::k_col_ragged<%d, false, true>",
+                                     "cdn::k_col_ragged<%d, true, true>", "cdn::k_z_ragged<%d, true>"};
 const char* const kOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
                              "-Xclang", "-packed-fp32-ops"};
 

@@ -9,7 +9,23 @@
 
 namespace cdn {
 
-enum JitKernel { JK_X_R2C = 0, JK_X_C2R, JK_X_LRCORE, JK_X_UPDATE, JK_COL_F, JK_COL_I, JK_Z_FILTER, JK_COUNT };
+enum JitKernel {
+  JK_X_R2C = 0,
+  JK_X_C2R,
+  JK_X_LRCORE,
+  JK_X_UPDATE,
+  JK_COL_F,
+  JK_COL_I,
+  JK_Z_FILTER,
+  JK_X_R2C_RAGGED,
+  JK_COL_RAGGED_F,
+  JK_COL_RAGGED_I,
+  JK_Z_RAGGED,
+  JK_COL_RAGGED_FP,
+  JK_COL_RAGGED_IP,
+  JK_Z_RAGGED_P,
+  JK_COUNT
+};
 
 // Launch-shape choices for the kernels of one length (0 / -1: the built-in
 // rule): columns per y/z tile, threads per x-pass workgroup, non-temporal

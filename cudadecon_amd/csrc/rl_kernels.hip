@@ -190,11 +190,9 @@ void zblend(hipStream_t st, float* img, int nx, int ny, int nz, int nZblend) {
 // lam[0] = clamp(sum num / (sum den + eps)), lam[1..2] = the two sums.
 // ---------------------------------------------------------------------------
 static constexpr int LBS = 1024;  // threads of the acceleration-factor kernel
-// workgroup s: stack s's partials part[s nb .. s nb + nb) -> lam[4 s ..]
-__global__ void __launch_bounds__(LBS) k_lambda(const double2* __restrict__ part, int nb, double eps, double* lam) {
+// one workgroup: nb partial pairs -> lam[0..2]
+__device__ __forceinline__ void lambda_of(const double2* __restrict__ part, int nb, double eps, double* lam) {
   __shared__ double ra[LBS / 64], rb[LBS / 64];
-  part += (long)blockIdx.x * nb;
-  lam += 4 * (long)blockIdx.x;
   // 4 independent accumulators per thread keep several loads in flight
   double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
   int i = threadIdx.x;
@@ -232,9 +230,25 @@ __global__ void __launch_bounds__(LBS) k_lambda(const double2* __restrict__ part
     lam[2] = tb;
   }
 }
+// workgroup s: stack s's partials part[s nb .. s nb + nb) -> lam[4 s ..]
+__global__ void __launch_bounds__(LBS) k_lambda(const double2* __restrict__ part, int nb, double eps, double* lam) {
+  lambda_of(part + (long)blockIdx.x * nb, nb, eps, lam + 4 * (long)blockIdx.x);
+}
+// workgroup s: the ranges[s].y partials of stack s from ranges[s].x
+__global__ void __launch_bounds__(LBS) k_lambda_ragged(const double2* __restrict__ part,
+                                                       const int2* __restrict__ ranges, double eps, double* lam) {
+  const int2 r = ranges[blockIdx.x];
+  lambda_of(part + r.x, r.y, eps, lam + 4 * (long)blockIdx.x);
+}
 void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam, int nstacks) {
   hipLaunchKernelGGL(k_lambda, dim3(nstacks), dim3(LBS), 0, st, reinterpret_cast<const double2*>(part), nb, eps,
                      lam);
+  CDN_HIP(hipGetLastError());
+}
+void accel_factor_ragged(hipStream_t st, const double* part, const int* ranges, int nstacks, double eps,
+                         double* lam) {
+  hipLaunchKernelGGL(k_lambda_ragged, dim3(nstacks), dim3(LBS), 0, st, reinterpret_cast<const double2*>(part),
+                     reinterpret_cast<const int2*>(ranges), eps, lam);
   CDN_HIP(hipGetLastError());
 }
 

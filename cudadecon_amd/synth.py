@@ -8,7 +8,12 @@ be regenerated on any host.  Sizes and parameters of the named workloads:
   10 iterations, no deskew;
 * ``lightsheet_128x512x512``  128 x 512 x 512, 5000 beads, PSF 1.3 / 4.0 px, deskew 31.5 deg,
   dz 0.3 um, dx 0.104 um, then rotation, 15 iterations;
-* ``beads_256x1024x1024``  256 x 1024 x 1024 plain RL-BA (the throughput workload).
+* ``beads_256x1024x1024``  256 x 1024 x 1024 plain RL-BA (the throughput workload);
+* the 48-stack batch (``batch_shapes``, ``batch_stack``): shapes drawn with
+  seed 2 from z in {64, 96, 128, 192, 256} and y, x in {256, 384, 512, 768,
+  1024}; stack i has its own bead field (seed 1000 + i, one bead per 2e4
+  voxels) imaged through channel PSF i mod 3 (sigma_xy 1.2 / 1.5 / 1.8 px,
+  sigma_z 3 / 4 / 5 px), 10 iterations each.
 """
 from __future__ import annotations
 
@@ -119,6 +124,71 @@ CONFIGS = {
     "beads_256x1024x1024": dict(shape=(256, 1024, 1024), beads=300 * 256, sxy=1.5, sz=3.0, seed=7, iters=10, dr=0.104, dz=0.2,
                  angle=0.0, rot=0.0),
 }
+
+
+# ---------------------------------------------------------------------------
+# 48 stacks of uneven shape, three channels
+# ---------------------------------------------------------------------------
+BATCH_N = 48
+BATCH_ZS = (64, 96, 128, 192, 256)
+BATCH_XYS = (256, 384, 512, 768, 1024)
+BATCH_CHANNELS = ((1.2, 3.0), (1.5, 4.0), (1.8, 5.0))  # (sigma_xy, sigma_z) px
+BATCH_DR, BATCH_DZ = 0.104, 0.2  # um
+BATCH_BEAD_DENSITY = 1.0 / 2e4   # beads per voxel
+BATCH_ITERS = 10
+
+
+def batch_shapes(n: int = BATCH_N, seed: int = 2):
+    """(nz, ny, nx) of the n stacks: z, y, x drawn independently."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice(BATCH_ZS)), int(rng.choice(BATCH_XYS)), int(rng.choice(BATCH_XYS))) for _ in range(n)]
+
+
+def batch_channel(i: int):
+    """Channel index and (sigma_xy, sigma_z) of stack i."""
+    c = i % len(BATCH_CHANNELS)
+    return c, BATCH_CHANNELS[c]
+
+
+def blur_fft(vol: np.ndarray, sigma_xy: float, sigma_z: float) -> np.ndarray:
+    """The linear 'same' convolution of ``blur`` (separable 33^3 Gaussian,
+    zero outside the volume) computed with zero-padded float32 FFTs on all
+    host cores.  The PSF's transform is the outer product of its three 1-D
+    transforms (it is separable), so only the volume is transformed."""
+    import scipy.fft as sf
+    kxy, kz = _kernels(sigma_xy, sigma_z)
+    pad = PSF_SIZE // 2
+    nz, ny, nx = vol.shape
+    s = [sf.next_fast_len(n + 2 * pad, real=True) for n in vol.shape]
+    workers = max(1, min(16, os.cpu_count() or 1))
+    F = sf.rfftn(np.ascontiguousarray(vol, dtype=np.float32), s=s, workers=workers)
+
+    def dft(k, n, half=False):
+        h = np.zeros(n, np.float64)
+        h[:k.size] = k
+        return (np.fft.rfft(h) if half else np.fft.fft(h)).astype(np.complex64)
+
+    hz, hy, hx = dft(kz, s[0]), dft(kxy, s[1]), dft(kxy, s[2], half=True)
+    hyx = hy[:, None] * hx[None, :]
+
+    def plane(z):
+        F[z] *= hz[z] * hyx
+
+    with _pool() as p:
+        list(p.map(plane, range(s[0])))
+    full = sf.irfftn(F, s=s, workers=workers)
+    del F
+    return np.ascontiguousarray(full[pad:pad + nz, pad:pad + ny, pad:pad + nx])
+
+
+def batch_stack(i: int, shape):
+    """uint16 raw stack i of the batch (its own seeded bead field, channel
+    PSF i mod 3, +100 background, Poisson, +100 camera offset)."""
+    _, (sxy, sz) = batch_channel(i)
+    ss_truth, ss_noise = np.random.SeedSequence(1000 + i).spawn(2)
+    beads = max(1, int(round(np.prod(shape) * BATCH_BEAD_DENSITY)))
+    gt = ground_truth(tuple(shape), beads, np.random.default_rng(ss_truth), n_gauss=0)
+    return image(blur_fft(gt, sxy, sz), ss_noise)
 
 
 def make_stack(name: str, shape=None, beads=None, seed=None):

@@ -18,6 +18,8 @@
 #ifndef CUDADECON_AMD_H
 #define CUDADECON_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 #define CDN_DEFAULT(v) = v
 extern "C" {
@@ -90,6 +92,13 @@ int cudaDecon_fft_supported(int n);
 
 /* findOptimalDimension (src/RL-Biggs-Andrews.cpp:33-43); n <= 1 returned as is. */
 unsigned cudaDecon_find_optimal_dimension(unsigned n, int step);
+
+/* Flat start of the later RL_interface calls (until RL_cleanup or a call
+ * with enable = 0): the first prediction Y_0 is a volume filled with
+ * `value` instead of the preprocessed stack, as the reference CLI's
+ * --FlatStart does with the stack's median (src/linearDecon.cpp:1143-1145,
+ * src/RL-Biggs-Andrews.cpp:305-316).  0 for a non-finite value. */
+int cudaDecon_set_flat_start(int enable, float value);
 
 /* Width of the result Deskew_interface writes for these arguments (its
  * sizing rule, src/RL-Biggs-Andrews.cpp:736-744, including the session's
@@ -194,15 +203,45 @@ int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6);
 int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms);
 void cudaDecon_rl_destroy(cudaDecon_rl* h);
 
-/* Batch of independent stacks of uneven shape, each with its own RL engine.
- * dims_zyx: 3 ints (nz, ny, nx) per stack; kscales: 3 floats per stack.
- * run: `iters` iterations of every stack from its staged stack, mode 0 = one
- * stack at a time on one stream, 1 = round-robin over nstreams streams,
- * 2 = the mode-1 schedule captured once into a hipGraph and replayed;
- * *ms receives the device time of the whole batch. */
+
+/* Device memory for the entry points that take device pointers: allocate
+ * (NULL on failure), free, and copy (kind 1 host to device, 2 device to host,
+ * 3 device to device; synchronous). */
+void* cudaDecon_device_malloc(size_t bytes);
+int cudaDecon_device_free(void* p);
+int cudaDecon_device_copy(void* dst, const void* src, size_t bytes, int kind);
+
+/* Batch of independent stacks of uneven shape (time points, channels, tiles).
+ * dims_zyx: 3 ints (nz, ny, nx) per stack; kscales: 3 floats per stack (the
+ * k-scales RL_interface_init would compute for that stack's grid).
+ * run: `iters` iterations of every stack from its staged stack:
+ *   mode 0 = each stack's own engine, one stack at a time on one stream;
+ *   1 = stacks round-robin over nstreams streams; 2 = the mode-1 schedule
+ *   captured once into a hipGraph and replayed;
+ *   3 = grouped: every pass is one launch per distinct transform length,
+ *   spanning all stacks of that length whatever their other sizes (about
+ *   3 nx + 4 ny + 2 nz + 1 launches per iteration for the distinct lengths);
+ *   4 = the mode-3 schedule captured once into a hipGraph and replayed.
+ * Every mode gives each stack bitwise the result of a one-stack engine.
+ * *ms receives the device time of the whole batch.  set_raw / set_raw_block
+ * reject stacks holding NaN or Inf. */
 typedef struct cudaDecon_batch cudaDecon_batch;
 cudaDecon_batch* cudaDecon_batch_create(int n, const int* dims_zyx, const float* otf, int otf_width, int otf_height,
                                         int otf_depth, const float* kscales, int nstreams);
+/* As cudaDecon_batch_create with several OTFs (channels): ntables OTF images
+ * laid out as their TIFF files, back to back in otf_data, otf_whd holding
+ * (width, height, depth) of each; table_of[i] is stack i's table. */
+cudaDecon_batch* cudaDecon_batch_create_tables(int n, const int* dims_zyx, int ntables, const float* otf_data,
+                                               const int* otf_whd, const int* table_of, const float* kscales,
+                                               int nstreams);
+/* Acceleration factors of stack i at iterations 0 .. iters-1 of the last run
+ * (0 for k < 2). */
+int cudaDecon_batch_lambdas(cudaDecon_batch* h, int i, int iters, double* out);
+/* Grouped modes: column passes on a persistent grid of at most
+ * max_workgroups workgroups that walk the tiles (0: one workgroup per tile). */
+int cudaDecon_batch_set_grid_cap(cudaDecon_batch* h, long max_workgroups);
+/* Kernel launches per iteration of the grouped modes. */
+int cudaDecon_batch_launches(cudaDecon_batch* h);
 int cudaDecon_batch_set_raw(cudaDecon_batch* h, int i, const float* raw);
 int cudaDecon_batch_set_raw_block(cudaDecon_batch* h, int i, const float* block, int bx, int by, int bz, int ox,
                                   int oy, int oz, float background);

@@ -63,6 +63,7 @@ def lib():
             "orc_filter_f64": (None, [_dp, i, i, i, _dp, i]),
             "orc_rl_ba_f64": (None, [_dp, i, i, i, _dp, i, C.c_void_p, _dp, _dp]),
             "orc_rl_ba_f32": (None, [_fp, i, i, i, _fp, i, C.c_void_p, _fp, _dp]),
+            "orc_rl_ba_f32_timed": (None, [_fp, i, i, i, _fp, i, _fp, _dp, _dp]),
             "orc_crop_u16": (None, [_u16p, i, i, i, i, i, i, _dp]),
             "orc_apodize": (None, [_dp, i, i, i, i]),
             "orc_bgsub": (None, [_dp, l_, d]),
@@ -194,6 +195,19 @@ def rl_ba_f32(raw: np.ndarray, H: np.ndarray, n_iter: int, seed: np.ndarray | No
         sp = seed.ctypes.data
     lib().orc_rl_ba_f32(raw, nx, ny, nz, Hf, int(n_iter), sp, X, lam)
     return X, lam[:n_iter]
+
+
+def rl_ba_f32_timed(raw: np.ndarray, H: np.ndarray, n_iter: int):
+    """FP32 flavour with the wall time of each iteration; returns (X, lambdas, seconds)."""
+    raw = _c(raw, np.float32)
+    _check_H(raw.shape, H)
+    nz, ny, nx = raw.shape
+    X = np.empty_like(raw)
+    lam = np.zeros(max(n_iter, 1), np.float64)
+    sec = np.zeros(max(n_iter, 1), np.float64)
+    lib().orc_rl_ba_f32_timed(raw, nx, ny, nz, _c(H, np.complex64).view(np.float32).reshape(-1), int(n_iter), X,
+                              lam, sec)
+    return X, lam[:n_iter], sec[:n_iter]
 
 
 # --------------------------------------------------------------------------

@@ -30,6 +30,7 @@
 //   src/radialft_interface.cpp:95-518 radially averaged OTF generation
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -291,7 +292,7 @@ void filter(T* v, const Fft3<T>& F, const cx<T>* H, bool conjH, cx<T>* S) {
 // (the FlatStart hook at :305-316).  lambdas: optional, nIter entries.
 template <class T>
 void rl_ba(const T* raw, int nx, int ny, int nz, const cx<T>* H, int nIter, const T* Y0, T* X,
-           double* lambdas) {
+           double* lambdas, double* iter_seconds = nullptr) {
   const long N = (long)nx * ny * nz;
   const long M = (long)nz * ny * (nx / 2 + 1);
   Fft3<T> F(nx, ny, nz);
@@ -300,6 +301,7 @@ void rl_ba(const T* raw, int nx, int ny, int nz, const cx<T>* H, int nIter, cons
   const T eps = std::numeric_limits<float>::epsilon();
   std::memcpy(X, raw, sizeof(T) * N);
   for (int k = 0; k < nIter; ++k) {
+    const auto t0 = std::chrono::steady_clock::now();
     double lambda = 0;
     if (k > 1) {
       double num = 0, den = 0;
@@ -341,6 +343,8 @@ void rl_ba(const T* raw, int nx, int ny, int nz, const cx<T>* H, int nIter, cons
       X[i] = x > 0 ? x : T(0);
       G1[i] = X[i] - Y[i];
     }
+    if (iter_seconds)
+      iter_seconds[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 }
 
@@ -422,6 +426,11 @@ void orc_rl_ba_f64(const double* raw, int nx, int ny, int nz, const double* H, i
 void orc_rl_ba_f32(const float* raw, int nx, int ny, int nz, const float* H, int nIter,
                    const float* Y0, float* X, double* lambdas) {
   rl_ba<float>(raw, nx, ny, nz, (const cx<float>*)H, nIter, Y0, X, lambdas);
+}
+// the same with the wall time of every iteration (the timed CPU baseline)
+void orc_rl_ba_f32_timed(const float* raw, int nx, int ny, int nz, const float* H, int nIter, float* X,
+                         double* lambdas, double* iter_seconds) {
+  rl_ba<float>(raw, nx, ny, nz, (const cx<float>*)H, nIter, nullptr, X, lambdas, iter_seconds);
 }
 
 // ---------------------------------------------------------------------------

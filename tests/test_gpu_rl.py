@@ -262,6 +262,10 @@ def test_rl_interface_lightsheet_full_size_against_fp64_digest(cdn, tmp_path):
     from cudadecon_amd.synth import make_stack
     d = np.load(os.path.join(os.path.dirname(__file__), "golden", "lightsheet_128x512x512_digest.npz"))
     raw, psf, c = make_stack("lightsheet_128x512x512")
+    # the regenerated input is the one the digest was made from (a changed
+    # random stream would otherwise read as a parity failure)
+    assert raw.shape == (128, 512, 512) and raw.dtype == np.uint16
+    assert float(raw.astype(np.float64).sum()) == float(d["raw_sum"])
     tab = make_otf(psf)
     p = _write_otf(tmp_path, tab)
     cdn.rl_cleanup()
