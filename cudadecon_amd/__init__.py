@@ -130,6 +130,8 @@ SIGNATURES = {
     "cudaDecon_device_malloc": (C.c_void_p, [C.c_size_t]),
     "cudaDecon_device_free": (C.c_int, [C.c_void_p]),
     "cudaDecon_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "cudaDecon_rl_run_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int,
+                                           C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p]),
 }
 
 PASS_NAMES = ["x_r2c", "y_fwd", "z_filter", "y_inv", "x_lrcore", "x_update", "lambda"]
@@ -379,6 +381,20 @@ class DeviceBuffer:
             self.close()
         except Exception:
             pass
+
+
+def rl_run_noalloc(d_raw, d_out, shape, otf_table, kscale, n_iters, stream=None):
+    """cudaDecon_rl_run_noalloc on device buffers (DeviceBuffer or raw
+    pointers): n_iters RL-BA iterations of the stack in d_raw into d_out,
+    with no device allocation."""
+    nz, ny, nx = shape
+    img = otf_image(otf_table)
+    d, h, w = img.shape
+    flat = np.ascontiguousarray(img).reshape(-1)
+    src = d_raw.ptr if isinstance(d_raw, DeviceBuffer) else d_raw
+    dst = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
+    _check(lib().cudaDecon_rl_run_noalloc(src, dst, nx, ny, nz, flat, w, h, d, float(kscale[0]), float(kscale[1]),
+                                          float(kscale[2]), int(n_iters), stream), "cudaDecon_rl_run_noalloc")
 
 
 class RLEngine:

@@ -11,6 +11,7 @@
 #include "geometry.hpp"
 #include "hostcopy.hpp"
 #include "net.hpp"
+#include "noalloc.hpp"
 #include "rl_kernels.hpp"
 #include "tiff.hpp"
 #include "tune.hpp"
@@ -967,6 +968,21 @@ int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms) {
 }
 
 void cudaDecon_rl_destroy(cudaDecon_rl* h) { delete h; }
+
+int cudaDecon_rl_run_noalloc(const float* d_raw, float* d_out, int nx, int ny, int nz, const float* otf,
+                             int otf_width, int otf_height, int otf_depth, float kxscale, float kyscale,
+                             float kzscale, int n_iters, void* stream) {
+  return guard([&]() -> int {
+    const float ks[3] = {kxscale, kyscale, kzscale};
+    try {
+      noalloc_run(d_raw, d_out, nx, ny, nz, otf, otf_width, otf_height, otf_depth, ks, n_iters,
+                  static_cast<hipStream_t>(stream));
+    } catch (const std::exception& e) {
+      fail(std::string("cudaDecon_rl_run_noalloc: ") + e.what());
+    }
+    return 1;
+  });
+}
 
 // ---- batch of stacks ------------------------------------------------------
 cudaDecon_batch* cudaDecon_batch_create(int n, const int* dims_zyx, const float* otf, int otf_width, int otf_height,

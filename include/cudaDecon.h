@@ -203,6 +203,23 @@ int cudaDecon_rl_stats(cudaDecon_rl* h, double* out6);
 int cudaDecon_rl_copy_probe(cudaDecon_rl* h, int n, double* ms);
 void cudaDecon_rl_destroy(cudaDecon_rl* h);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid allocation): the no-allocation variant of cudaDecon_rl_create +
+ * cudaDecon_rl_set_raw + cudaDecon_rl_iterate(n_iters from k = 0) +
+ * cudaDecon_rl_get, with the same results.  d_raw: the preprocessed stack
+ * (device, nz*ny*nx floats, x fastest, read only); d_out: device result of
+ * the same size; otf and k-scales as cudaDecon_rl_create.  Every other
+ * device buffer is carved from a static array of 256 MiB that each GPU the
+ * library runs on holds until the process ends (about 20 bytes per voxel:
+ * stacks up to about 13 M voxels, e.g. 64 x 256 x 768); larger stacks and
+ * FFT lengths that are not built in are refused with an error.  Runs on
+ * `stream` (NULL: the null stream) and returns after it finished.  Calls on
+ * one GPU share the array and are serialised by the library. */
+int cudaDecon_rl_run_noalloc(const float* d_raw, float* d_out, int nx, int ny, int nz, const float* otf,
+                             int otf_width, int otf_height, int otf_depth, float kxscale, float kyscale,
+                             float kzscale, int n_iters, void* stream);
+
 
 /* Device memory for the entry points that take device pointers: allocate
  * (NULL on failure), free, and copy (kind 1 host to device, 2 device to host,
