@@ -97,6 +97,7 @@ SIGNATURES = {
     "cudaDecon_batch_lambdas": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d]),
     "cudaDecon_batch_set_grid_cap": (C.c_int, [C.c_void_p, C.c_long]),
     "cudaDecon_batch_launches": (C.c_int, [C.c_void_p]),
+    "cudaDecon_batch_set_group_streams": (C.c_int, [C.c_void_p, C.c_int]),
     "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
     "cudaDecon_rl_set_raw_u16": (C.c_int, [C.c_void_p, _u16, C.c_float]),
     "cudaDecon_rl_deskew": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
@@ -105,6 +106,9 @@ SIGNATURES = {
     "cudaDecon_rl_post_dims": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cudaDecon_rl_post_get": (C.c_int, [C.c_void_p, _f]),
     "cudaDecon_rl_post_stats": (C.c_int, [C.c_void_p, _d]),
+    "cudaDecon_rl_post_get_block": (C.c_int, [C.c_void_p, C.c_int, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                              C.c_int]),
+    "cudaDecon_rl_buffer_stats": (C.c_int, [C.c_void_p, C.c_int, _d]),
     "cudaDecon_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "cudaDecon_autotune_clear": (C.c_int, []),
     "cudaDecon_fft_variant": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -526,6 +530,21 @@ class RLEngine:
         _check(lib().cudaDecon_rl_post_get(self.h, out), "cudaDecon_rl_post_get")
         return out
 
+    def post_get_block(self, buf, shape_zyx, offset_zyx):
+        """Block of the deskew (buf 0) or affine (buf 1) result."""
+        bz, by, bx = shape_zyx
+        oz, oy, ox = offset_zyx
+        out = np.empty((bz, by, bx), np.float32)
+        _check(lib().cudaDecon_rl_post_get_block(self.h, int(buf), out, bx, by, bz, int(ox), int(oy), int(oz)),
+               "cudaDecon_rl_post_get_block")
+        return out
+
+    def buffer_stats(self, which):
+        """(sum, min, max) of an engine buffer: 0 staged stack, 1 X, 2 Y (seed), 3 G."""
+        o = np.zeros(3, np.float64)
+        _check(lib().cudaDecon_rl_buffer_stats(self.h, int(which), o), "cudaDecon_rl_buffer_stats")
+        return tuple(o)
+
     def post_stats(self):
         o = np.zeros(3, np.float64)
         _check(lib().cudaDecon_rl_post_stats(self.h, o), "cudaDecon_rl_post_stats")
@@ -601,6 +620,10 @@ class Batch:
     def set_grid_cap(self, max_workgroups):
         """Grouped modes: persistent column-pass grids of at most this many workgroups (0: one per tile)."""
         _check(lib().cudaDecon_batch_set_grid_cap(self.h, int(max_workgroups)), "cudaDecon_batch_set_grid_cap")
+
+    def set_group_streams(self, n):
+        """Grouped modes: spread the launches of one pass over up to n streams."""
+        _check(lib().cudaDecon_batch_set_group_streams(self.h, int(n)), "cudaDecon_batch_set_group_streams")
 
     def launches(self):
         """Kernel launches per iteration of the grouped modes."""

@@ -235,30 +235,58 @@ void Batch::enqueue_grouped(hipStream_t st, int iters) {
   float* pv = a_xb.as<float>();
   CDN_HIP(hipMemcpyAsync(c, a_raw.p, a_raw.bytes, hipMemcpyDeviceToDevice, st));  // X_0 = R, every stack
   float2* spec = a_spec.as<float2>();
+  // the launches of one pass (one per length) touch disjoint stacks: with
+  // group_streams > 1 they spread over that many streams, joined before the
+  // next pass, so one launch's tail overlaps the next one's work
+  const int S = std::max(1, std::min(group_streams, (int)streams.size() - 1));
+  auto pass = [&](size_t count, auto&& launch) {
+    if (S <= 1 || count <= 1) {
+      for (size_t i = 0; i < count; ++i) launch(i, st);
+      return;
+    }
+    const int ns = (int)std::min<size_t>(count, (size_t)S);
+    CDN_HIP(hipEventRecord(events[0], st));
+    for (int s = 0; s < ns; ++s) CDN_HIP(hipStreamWaitEvent(streams[s], events[0], 0));
+    for (size_t i = 0; i < count; ++i) launch(i, streams[i % ns]);
+    for (int s = 0; s < ns; ++s) {
+      CDN_HIP(hipEventRecord(events[1 + s], streams[s]));
+      CDN_HIP(hipStreamWaitEvent(st, events[1 + s], 0));
+    }
+  };
   for (int k = 0; k < iters; ++k) {
     double* lamk = g_lam.as<double>() + 4 * (size_t)n * k;
     if (k >= 2) accel_factor_ragged(st, g_part.as<double>(), g_ranges.as<int>(), n, (double)eps, lamk);
     const int mode = k >= 2 ? XIN_PRED : XIN_PLAIN;
-    for (XGroup& g : xg)
-      launch_x_r2c(g.nx, st, c + g.voff, pv + g.voff, a_y.as<float>() + g.voff, lamk + 4 * g.first, mode,
+    pass(xg.size(), [&](size_t i, hipStream_t s) {
+      XGroup& g = xg[i];
+      launch_x_r2c(g.nx, s, c + g.voff, pv + g.voff, a_y.as<float>() + g.voff, lamk + 4 * g.first, mode,
                    spec + g.soff, g.rows, g.pitch, g.tw, 0, g.row_starts.as<long>(), g.count);
+    });
     for (int f = 0; f < 2; ++f) {
-      for (ColGroup& g : yg)
-        launch_col_ragged(g.len, 0, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
-      for (ColGroup& g : zg)
-        launch_z_ragged(g.len, st, g.segs.as<ZSeg>(), g.nseg, g.total, f, g.tw, grid_cap);
-      for (ColGroup& g : yg)
-        launch_col_ragged(g.len, 1, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
+      for (int inv = 0; inv < 2; ++inv) {
+        pass(yg.size(), [&](size_t i, hipStream_t s) {
+          ColGroup& g = yg[i];
+          launch_col_ragged(g.len, inv, s, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
+        });
+        if (inv == 0)
+          pass(zg.size(), [&](size_t i, hipStream_t s) {
+            ColGroup& g = zg[i];
+            launch_z_ragged(g.len, s, g.segs.as<ZSeg>(), g.nseg, g.total, f, g.tw, grid_cap);
+          });
+      }
       if (f == 0)
-        for (XGroup& g : xg)
-          launch_x_lrcore(g.nx, st, spec + g.soff, a_raw.as<float>() + g.voff, g.rows, g.pitch, g.tw, eps);
+        pass(xg.size(), [&](size_t i, hipStream_t s) {
+          XGroup& g = xg[i];
+          launch_x_lrcore(g.nx, s, spec + g.soff, a_raw.as<float>() + g.voff, g.rows, g.pitch, g.tw, eps);
+        });
     }
     const float* yv = k >= 2 ? a_y.as<float>() : c;
-    for (XGroup& g : xg) {
-      const int nb = launch_x_update(g.nx, st, spec + g.soff, yv + g.voff, pv + g.voff, a_g.as<float>() + g.voff,
+    pass(xg.size(), [&](size_t i, hipStream_t s) {
+      XGroup& g = xg[i];
+      const int nb = launch_x_update(g.nx, s, spec + g.soff, yv + g.voff, pv + g.voff, a_g.as<float>() + g.voff,
                                      g_part.as<double>() + 2 * g.part0, k >= 1, g.rows, g.pitch, g.tw);
       if (nb != g.nb) fail("batch: the x-update launch shape changed under the grouped schedule");
-    }
+    });
     std::swap(c, pv);
   }
   cur = c;
@@ -281,6 +309,7 @@ std::vector<const void*> Batch::addresses(bool grouped) const {
     for (const ColGroup& g : yg) a.push_back(g.segs.p);
     for (const ColGroup& g : zg) a.push_back(g.segs.p);
     a.push_back(reinterpret_cast<const void*>(grid_cap));
+    a.push_back(reinterpret_cast<const void*>((long)group_streams));
   }
   return a;
 }

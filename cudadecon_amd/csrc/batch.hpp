@@ -66,6 +66,7 @@ struct Batch {
   DevBuf g_part, g_lam, g_ranges;
   int lam_iters = 0;       // iterations whose factors g_lam holds (last grouped run)
   long grid_cap = 0;       // > 0: persistent column-pass grids of at most this many workgroups
+  int group_streams = 1;   // streams the launches of one pass spread over (they touch disjoint stacks)
   int last_mode = -1;
 
   CapturedRun cap_streams, cap_grouped;

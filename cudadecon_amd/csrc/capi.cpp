@@ -911,6 +911,38 @@ int cudaDecon_rl_post_get(cudaDecon_rl* h, float* out) {
   });
 }
 
+int cudaDecon_rl_post_get_block(cudaDecon_rl* h, int buf, float* out, int bx, int by, int bz, int ox, int oy,
+                                int oz) {
+  return guard([&]() -> int {
+    if (!h || !out || buf < 0 || buf > 1 || !h->post[buf].p || h->post_cur < 0)
+      fail("cudaDecon_rl_post_get_block: no such deskew or affine result");
+    if (bx <= 0 || by <= 0 || bz <= 0 || ox < 0 || oy < 0 || oz < 0 || ox + bx > h->pnx || oy + by > h->pny ||
+        oz + bz > h->pnz)
+      fail("cudaDecon_rl_post_get_block: block outside the result");
+    const float* src = h->post[buf].as<float>();
+    for (int z = 0; z < bz; ++z)
+      CDN_HIP(hipMemcpy2DAsync(out + (size_t)z * bx * by, sizeof(float) * bx,
+                               src + ((size_t)(oz + z) * h->pny + oy) * h->pnx + ox, sizeof(float) * h->pnx,
+                               sizeof(float) * bx, by, hipMemcpyDeviceToHost, h->st));
+    CDN_HIP(hipStreamSynchronize(h->st));
+    return 1;
+  });
+}
+
+int cudaDecon_rl_buffer_stats(cudaDecon_rl* h, int which, double* out3) {
+  return guard([&]() -> int {
+    one_stack(h, "cudaDecon_rl_buffer_stats");
+    if (!h || !out3 || which < 0 || which > 3) fail("cudaDecon_rl_buffer_stats: bad argument");
+    const float* b = which == 0 ? h->eng.raw.as<float>()
+                     : which == 1 ? h->eng.result()
+                     : which == 2 ? h->eng.y.as<float>()
+                                  : h->eng.g.as<float>();
+    h->red.ensure(sizeof(double) * 3 * 1024);
+    stats(h->st, b, h->eng.fft.nvox, h->red.as<double>(), out3);
+    return 1;
+  });
+}
+
 int cudaDecon_rl_post_stats(cudaDecon_rl* h, double* out3) {
   return guard([&]() -> int {
     if (!h || !out3 || h->post_cur < 0) fail("cudaDecon_rl_post_stats: no deskew or affine result");
@@ -1037,6 +1069,14 @@ int cudaDecon_batch_set_grid_cap(cudaDecon_batch* h, long max_workgroups) {
   return guard([&]() -> int {
     if (!h || max_workgroups < 0) fail("cudaDecon_batch_set_grid_cap: bad argument");
     h->b.grid_cap = max_workgroups;
+    return 1;
+  });
+}
+
+int cudaDecon_batch_set_group_streams(cudaDecon_batch* h, int n) {
+  return guard([&]() -> int {
+    if (!h || n < 1) fail("cudaDecon_batch_set_group_streams: bad argument");
+    h->b.group_streams = n;
     return 1;
   });
 }

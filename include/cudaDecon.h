@@ -220,7 +220,6 @@ int cudaDecon_rl_run_noalloc(const float* d_raw, float* d_out, int nx, int ny, i
                              int otf_width, int otf_height, int otf_depth, float kxscale, float kyscale,
                              float kzscale, int n_iters, void* stream);
 
-
 /* Device memory for the entry points that take device pointers: allocate
  * (NULL on failure), free, and copy (kind 1 host to device, 2 device to host,
  * 3 device to device; synchronous). */
@@ -257,6 +256,10 @@ int cudaDecon_batch_lambdas(cudaDecon_batch* h, int i, int iters, double* out);
 /* Grouped modes: column passes on a persistent grid of at most
  * max_workgroups workgroups that walk the tiles (0: one workgroup per tile). */
 int cudaDecon_batch_set_grid_cap(cudaDecon_batch* h, long max_workgroups);
+/* Grouped modes: the launches of one pass (one per transform length, on
+ * disjoint stacks) spread over up to n of the batch's streams, joined before
+ * the next pass (1: all on one stream). */
+int cudaDecon_batch_set_group_streams(cudaDecon_batch* h, int n);
 /* Kernel launches per iteration of the grouped modes. */
 int cudaDecon_batch_launches(cudaDecon_batch* h);
 int cudaDecon_batch_set_raw(cudaDecon_batch* h, int i, const float* raw);
@@ -283,6 +286,13 @@ int cudaDecon_rl_post_dims(cudaDecon_rl* h, int* nx, int* ny, int* nz);
 int cudaDecon_rl_post_get(cudaDecon_rl* h, float* out);
 /* FP64 sum, min, max of the last deskew / affine result */
 int cudaDecon_rl_post_stats(cudaDecon_rl* h, double* out3);
+/* A block of the deskew (buf 0) or affine (buf 1, after cudaDecon_rl_affine
+ * of a deskew result) result, bx*by*bz floats from (ox, oy, oz). */
+int cudaDecon_rl_post_get_block(cudaDecon_rl* h, int buf, float* out, int bx, int by, int bz, int ox, int oy,
+                                int oz);
+/* FP64 sum, min, max of an engine buffer: 0 the staged stack, 1 X, 2 Y (the
+ * seed after cudaDecon_rl_set_seed or cudaDecon_rl_warm_start), 3 G. */
+int cudaDecon_rl_buffer_stats(cudaDecon_rl* h, int which, double* out3);
 
 /* Reference copy of the engine's current X (snapshot) and the relative L2
  * distance ||V - ref|| / ||ref|| (FP64 sums) of V = the current X

@@ -78,7 +78,7 @@ def _read(path):
 def _folder(tmp_path, shapes, seed0=70):
     from cudadecon_amd.synth import make_stack
     d = tmp_path / "data"
-    d.mkdir()
+    d.mkdir(parents=True)
     stacks = []
     for i, s in enumerate(shapes):
         raw, _, _ = make_stack("beads_64x128x128", shape=s, seed=seed0 + i)
@@ -193,14 +193,16 @@ def test_cli_outputs_equal_rl_interface(cdn, tmp_path):
 
 @pytest.mark.gpu
 def test_cli_y_tiles_stitch_as_the_reference(cdn, tmp_path):
-    """--tile 64 --TileOverlap 16 on a 160-row stack: 3 tiles of 64 rows
-    (overlap floor((3 64 - 160) / 2) = 16, offsets 0, 48, 96; the last tile
-    repeats the stack's last row past its end), each through RL_interface,
-    stitched without blending: rows [0, 56) from tile 0, [56, 104) from
-    tile 1, [104, 160) from tile 2 (src/linearDecon.cpp:1219-1226)."""
+    """--tile 64 --TileOverlap 8 on a 160-row stack: ceil(160 / (64 - 8)) = 3
+    tiles of 64 rows, overlap floor((3 64 - 160) / 2) = 16, offsets 0, 48, 96
+    (src/linearDecon.cpp:514-537; the last tile repeats the stack's last row
+    past its end), each through RL_interface, stitched without blending:
+    rows [0, 56) from tile 0, [56, 104) from tile 1, [104, 160) from tile 2
+    (src/linearDecon.cpp:1219-1226)."""
     d, stacks = _folder(tmp_path, [(32, 160, 64)], seed0=95)
     otf = _otf(tmp_path)
-    _cli(["-i", 3, "--tile", 64, "--TileOverlap", 16, d, "stack", otf])
+    r = _cli(["-i", 3, "--tile", 64, "--TileOverlap", 8, d, "stack", otf])
+    assert "# of tiles: 3." in r.stdout and "Overlap: 16 Y pix" in r.stdout
     got = _read(str(d / "GPUdecon" / "cell1_ch0_stack0000_decon.tif"))
     raw = stacks[0]
     tiles = []
