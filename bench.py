@@ -54,6 +54,18 @@ def pmc_traffic(kernel: str):
         return None
 
 
+def _progress(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def _leg(name, fn, *a, **kw):
+    """Run one side measurement, with a line on stderr when it ends."""
+    t0 = time.perf_counter()
+    out = fn(*a, **kw)
+    _progress(f"{name} done in {time.perf_counter() - t0:.1f} s")
+    return out
+
+
 def _batch_modes(b, vox, iters, modes=("sequential", "graph", "grouped_graph")):
     out = {}
     for mode in modes:
@@ -115,9 +127,13 @@ def batch_small_uneven(cdn, tab, n=256, iters=10, nstreams=8):
     out = {"stacks": n, "voxels": vox, "iterations": iters, "streams": nstreams,
            "grouped_launches_per_iter": b.launches(), "per_stack_launches_per_iter": 10 * n}
     out.update(_batch_modes(b, vox, iters))
+    # the grouped schedule as 8 concurrent ranges of stacks, one stream each
+    b.set_partitions(nstreams)
+    out[f"grouped_graph_{nstreams}_partitions"] = _batch_modes(b, vox, iters, ("grouped_graph",))["grouped_graph"]
     seq = out["sequential"]["ms"]
     out["speedup_graph"] = round(seq / out["graph"]["ms"], 3)
     out["speedup_grouped_graph"] = round(seq / out["grouped_graph"]["ms"], 3)
+    out["speedup_grouped_graph_partitions"] = round(seq / out[f"grouped_graph_{nstreams}_partitions"]["ms"], 3)
     b.close()
     return out
 
@@ -210,15 +226,24 @@ def warm_start_eval(cdn, iters=50):
     return res
 
 
-def pipeline_512x2048x2048(cdn):
+def pipeline_512x2048x2048(cdn, tune=True):
     """512x2048x2048 uint16 stack (bead blocks made as the light-sheet
     workload on a flat 200-count background) through camera offset
     subtraction, warm start, 10 RL-BA iterations, deskew and a small affine,
-    all resident in HBM; stage times."""
+    all resident in HBM; stage times.  With tune, the FFT launch shapes of
+    this grid are autotuned first (run-time compiled candidates, timed on
+    the grid; results bitwise unchanged) and the built-in shapes restored
+    after."""
     from cudadecon_amd import pipeline, warmstart
     from cudadecon_amd.otf import make_otf
     from cudadecon_amd.synth import make_stack
     shape = (512, 2048, 2048)
+    tuned = None
+    if tune:
+        t0 = time.perf_counter()
+        rep = cdn.autotune([shape], reps=3, use_cache=False)
+        tuned = {"seconds": round(time.perf_counter() - t0, 1),
+                 "chosen": {str(e["n"]): e["chosen"] for e in rep["lengths"]}}
     blk, psf, _ = make_stack("lightsheet_128x512x512", shape=(64, 128, 128), seed=4)
     raw = np.full(shape, 200, np.uint16)
     for off in ((100, 300, 500), (300, 1500, 900), (420, 100, 1800)):
@@ -231,8 +256,11 @@ def pipeline_512x2048x2048(cdn):
     t = pipeline.run(eng, raw, net)
     eng.close()
     net.close()
+    if tune:
+        cdn.autotune_clear()
     t["stack_zyx"] = list(shape)
     t["iteration_Gvoxel_s"] = round(np.prod(shape) * 10 / (t["iterations_ms"] * 1e-3) / 1e9, 2)
+    t["autotuned"] = tuned
     return t
 
 
@@ -251,6 +279,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batch measurements (48 uneven stacks, 256 small)")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start evaluation")
     ap.add_argument("--no-pipeline", action="store_true", help="skip the 512x2048x2048 pipeline")
+    ap.add_argument("--no-tune", action="store_true", help="the 512x2048x2048 pipeline on the built-in FFT shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this benchmark runs on exactly one GPU; --gpus must be 1 "
@@ -312,11 +341,13 @@ def main():
         e2e = {"entry": "RL_interface", "iterations": args.e2e_iters, "seconds": round(e2e_s, 4),
                "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
-    batch = None if args.no_batch else batch_48_stacks(cdn)
-    small = None if args.no_batch else batch_small_uneven(cdn, tab)
-    stacks = None if args.no_batch else stacks_256(cdn, tab)
-    warm = None if args.no_warm else warm_start_eval(cdn)
-    big = None if args.no_pipeline else pipeline_512x2048x2048(cdn)
+    _progress(f"headline: {ms_per_step:.3f} ms per iteration")
+    batch = None if args.no_batch else _leg("batch_48_stacks", batch_48_stacks, cdn)
+    small = None if args.no_batch else _leg("batch_256_small_uneven", batch_small_uneven, cdn, tab)
+    stacks = None if args.no_batch else _leg("stacks_256_of_64x128x128", stacks_256, cdn, tab)
+    warm = None if args.no_warm else _leg("warm_start", warm_start_eval, cdn)
+    big = None if args.no_pipeline else _leg("pipeline_512x2048x2048", pipeline_512x2048x2048, cdn,
+                                             tune=not args.no_tune)
 
     Rb = 4.0 * nvox
     Pb = 8.0 * nspec

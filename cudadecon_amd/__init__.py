@@ -97,7 +97,7 @@ SIGNATURES = {
     "cudaDecon_batch_lambdas": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _d]),
     "cudaDecon_batch_set_grid_cap": (C.c_int, [C.c_void_p, C.c_long]),
     "cudaDecon_batch_launches": (C.c_int, [C.c_void_p]),
-    "cudaDecon_batch_set_group_streams": (C.c_int, [C.c_void_p, C.c_int]),
+    "cudaDecon_batch_set_partitions": (C.c_int, [C.c_void_p, C.c_int]),
     "cudaDecon_rl_snapshot": (C.c_int, [C.c_void_p]),
     "cudaDecon_rl_set_raw_u16": (C.c_int, [C.c_void_p, _u16, C.c_float]),
     "cudaDecon_rl_deskew": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
@@ -111,7 +111,7 @@ SIGNATURES = {
     "cudaDecon_rl_buffer_stats": (C.c_int, [C.c_void_p, C.c_int, _d]),
     "cudaDecon_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "cudaDecon_autotune_clear": (C.c_int, []),
-    "cudaDecon_fft_variant": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cudaDecon_fft_variant": (C.c_int, [C.c_int, C.c_void_p, C.c_int]),
     "cudaDecon_wiener": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_float, C.c_float,
                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
     "cudaDecon_rl_distance": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
@@ -318,10 +318,11 @@ def autotune_clear():
 
 
 def fft_variant(n):
-    """None for the built-in shape of length n, else {cols, xthreads, col_nt_load}."""
-    v = (C.c_int * 3)()
-    tuned = lib().cudaDecon_fft_variant(int(n), C.byref(v, 0), C.byref(v, 4), C.byref(v, 8))
-    return dict(cols=v[0], xthreads=v[1], col_nt_load=v[2]) if tuned else None
+    """None for the built-in shape of length n, else {cols, xthreads, col_nt_load, xcd_remap, col_tpt, zf_waves}."""
+    v = np.zeros(6, np.int32)
+    tuned = lib().cudaDecon_fft_variant(int(n), v.ctypes.data, 6)
+    names = ("cols", "xthreads", "col_nt_load", "xcd_remap", "col_tpt", "zf_waves")
+    return dict(zip(names, map(int, v))) if tuned else None
 
 
 def wiener_filter(v, otf_table, kscale, dkxy, rcutoff, wiener, background=0.0):
@@ -621,9 +622,10 @@ class Batch:
         """Grouped modes: persistent column-pass grids of at most this many workgroups (0: one per tile)."""
         _check(lib().cudaDecon_batch_set_grid_cap(self.h, int(max_workgroups)), "cudaDecon_batch_set_grid_cap")
 
-    def set_group_streams(self, n):
-        """Grouped modes: spread the launches of one pass over up to n streams."""
-        _check(lib().cudaDecon_batch_set_group_streams(self.h, int(n)), "cudaDecon_batch_set_group_streams")
+    def set_partitions(self, n):
+        """Grouped modes: run the stacks as n ranges of about equal voxel count, each grouped on its own stream
+        (at most nstreams)."""
+        _check(lib().cudaDecon_batch_set_partitions(self.h, int(n)), "cudaDecon_batch_set_partitions")
 
     def launches(self):
         """Kernel launches per iteration of the grouped modes."""

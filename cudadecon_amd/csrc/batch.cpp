@@ -17,6 +17,11 @@
 //      About 3 nx + 4 ny + 2 nz + 1 launches per iteration (the distinct
 //      lengths), instead of 9 or 10 per stack;
 //   4  schedule 3 captured once into a hipGraph and replayed.
+// The grouped schedule can run as P partitions: the stacks, in arena order,
+// cut into P ranges of about equal voxel count, each range grouped on its
+// own and run on a stream of its own, so that different passes of
+// different partitions overlap (stacks at different passes fill HBM
+// better than all stacks in one pass at a time).
 // Every mode gives every stack bitwise the result of a one-stack engine: the
 // arithmetic per column, row and partial block is the same, and each stack's
 // partials are summed over the same blocks in the same order.
@@ -117,98 +122,119 @@ void Batch::setup(int n, const int* dims_zyx, std::vector<OtfTable> tabs, const 
 
 void Batch::build_groups() {
   const int n = (int)engines.size();
-  xg.clear();
-  yg.clear();
-  zg.clear();
-  // x passes: runs of stacks with one x length; a run ends after a stack
-  // whose rows would put a row pair or an update workgroup across the next
-  // stack (ny nz odd, or not a multiple of the workgroup's rows)
+  parts.clear();
+  // partitions: contiguous ranges of arena positions of about equal voxels
+  const int P = std::max(1, std::min(npart, n));
+  std::vector<int> cut{0};
+  {
+    long total = 0, acc = 0;
+    for (int p = 0; p < n; ++p) total += engines[order[p]]->fft.nvox;
+    for (int p = 0; p < n && (int)cut.size() < P; ++p) {
+      acc += engines[order[p]]->fft.nvox;
+      if (acc * P >= total * (long)cut.size() && p + 1 < n) cut.push_back(p + 1);
+    }
+    cut.push_back(n);
+  }
   std::vector<int> ranges(2 * (size_t)n);
   long part_total = 0;
-  for (int p = 0; p < n;) {
-    const RLEngine& e0 = *engines[order[p]];
-    XGroup g;
-    g.nx = e0.fft.nx;
-    g.first = p;
-    g.voff = voff[order[p]];
-    g.soff = soff[order[p]];
-    g.pitch = e0.fft.pitch;
-    g.tw = e0.fft.twx.as<float2>();
-    g.rpb = x_update_rows_per_block(g.nx);
-    std::vector<long> starts;
-    while (p < n && engines[order[p]]->fft.nx == g.nx) {
-      const long r = (long)engines[order[p]]->fft.ny * engines[order[p]]->fft.nz;
-      starts.push_back(g.rows);
-      ranges[2 * (size_t)p] = (int)(part_total + g.rows / g.rpb);
-      ranges[2 * (size_t)p + 1] = (int)((r + g.rpb - 1) / g.rpb);
-      g.rows += r;
-      ++p;
-      if (r % 2 != 0 || r % g.rpb != 0) break;
+  for (size_t q = 0; q + 1 < cut.size(); ++q) {
+    Partition P1;
+    P1.p0 = cut[q];
+    P1.p1 = cut[q + 1];
+    // x passes: runs of stacks with one x length; a run ends after a stack
+    // whose rows would put a row pair or an update workgroup across the
+    // next stack (ny nz odd, or not a multiple of the workgroup's rows)
+    for (int p = P1.p0; p < P1.p1;) {
+      const RLEngine& e0 = *engines[order[p]];
+      XGroup g;
+      g.nx = e0.fft.nx;
+      g.first = p;
+      g.voff = voff[order[p]];
+      g.soff = soff[order[p]];
+      g.pitch = e0.fft.pitch;
+      g.tw = e0.fft.twx.as<float2>();
+      g.rpb = x_update_rows_per_block(g.nx);
+      std::vector<long> starts;
+      while (p < P1.p1 && engines[order[p]]->fft.nx == g.nx) {
+        const long r = (long)engines[order[p]]->fft.ny * engines[order[p]]->fft.nz;
+        starts.push_back(g.rows);
+        ranges[2 * (size_t)p] = (int)(part_total + g.rows / g.rpb);
+        ranges[2 * (size_t)p + 1] = (int)((r + g.rpb - 1) / g.rpb);
+        g.rows += r;
+        ++p;
+        if (r % 2 != 0 || r % g.rpb != 0) break;
+      }
+      g.count = p - g.first;
+      g.nb = x_update_blocks(g.nx, g.rows);
+      g.part0 = part_total;
+      part_total += g.nb;
+      g.row_starts.ensure(sizeof(long) * starts.size());
+      CDN_HIP(hipMemcpy(g.row_starts.p, starts.data(), sizeof(long) * starts.size(), hipMemcpyHostToDevice));
+      P1.xg.push_back(std::move(g));
     }
-    g.count = p - g.first;
-    g.nb = x_update_blocks(g.nx, g.rows);
-    g.part0 = part_total;
-    part_total += g.nb;
-    g.row_starts.ensure(sizeof(long) * starts.size());
-    CDN_HIP(hipMemcpy(g.row_starts.p, starts.data(), sizeof(long) * starts.size(), hipMemcpyHostToDevice));
-    xg.push_back(std::move(g));
+    // y passes by y length, z filter by z length
+    std::map<int, std::vector<ColSeg>> ys;
+    std::map<int, std::vector<ZSeg>> zs;
+    std::map<int, const float2*> ytw, ztw;
+    std::map<int, long> ytot, ztot;
+    for (int p = P1.p0; p < P1.p1; ++p) {
+      const int i = order[p];
+      const RLEngine& e = *engines[i];
+      const Fft3& F = e.fft;
+      float2* S = a_spec.as<float2>() + soff[i];
+      {
+        const long L = col_tile_cols(F.ny), nt = (F.nxh + L - 1) / L;
+        ColSeg sg{S, F.nxh, F.pitch, (long)F.ny * F.pitch, nt, ytot[F.ny]};
+        ys[F.ny].push_back(sg);
+        ytot[F.ny] += (long)F.nz * nt;
+        ytw[F.ny] = F.twy.as<float2>();
+      }
+      {
+        const long L = col_tile_cols(F.nz), nt = (F.nxh + L - 1) / L;
+        ZSeg sg{S, F.nxh, (long)F.ny * F.pitch, F.pitch, nt, F.ny, ztot[F.nz],
+                e.otf.params(e.table, 1.0 / (double)F.nvox)};
+        zs[F.nz].push_back(sg);
+        ztot[F.nz] += (long)F.ny * nt;
+        ztw[F.nz] = F.twz.as<float2>();
+      }
+    }
+    for (auto& kv : ys) {
+      ColGroup g;
+      g.len = kv.first;
+      g.nseg = (int)kv.second.size();
+      g.total = ytot[kv.first];
+      g.tw = ytw[kv.first];
+      g.segs.ensure(sizeof(ColSeg) * kv.second.size());
+      CDN_HIP(hipMemcpy(g.segs.p, kv.second.data(), sizeof(ColSeg) * kv.second.size(), hipMemcpyHostToDevice));
+      fft_prepare_ragged(g.len);
+      P1.yg.push_back(std::move(g));
+    }
+    for (auto& kv : zs) {
+      ColGroup g;
+      g.len = kv.first;
+      g.nseg = (int)kv.second.size();
+      g.total = ztot[kv.first];
+      g.tw = ztw[kv.first];
+      g.segs.ensure(sizeof(ZSeg) * kv.second.size());
+      CDN_HIP(hipMemcpy(g.segs.p, kv.second.data(), sizeof(ZSeg) * kv.second.size(), hipMemcpyHostToDevice));
+      fft_prepare_ragged(g.len);
+      P1.zg.push_back(std::move(g));
+    }
+    for (const XGroup& g : P1.xg) fft_prepare_ragged(g.nx);
+    parts.push_back(std::move(P1));
   }
   g_part.ensure(sizeof(double) * 2 * (size_t)std::max(part_total, 1L));
   g_ranges.ensure(sizeof(int) * ranges.size());
   CDN_HIP(hipMemcpy(g_ranges.p, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice));
-
-  // y passes by y length, z filter by z length (stacks in arena order)
-  std::map<int, std::vector<ColSeg>> ys;
-  std::map<int, std::vector<ZSeg>> zs;
-  std::map<int, const float2*> ytw, ztw;
-  std::map<int, long> ytot, ztot;
-  for (int p = 0; p < n; ++p) {
-    const int i = order[p];
-    const RLEngine& e = *engines[i];
-    const Fft3& F = e.fft;
-    float2* S = a_spec.as<float2>() + soff[i];
-    {
-      const long L = col_tile_cols(F.ny), nt = (F.nxh + L - 1) / L;
-      ColSeg s{S, F.nxh, F.pitch, (long)F.ny * F.pitch, nt, ytot[F.ny]};
-      ys[F.ny].push_back(s);
-      ytot[F.ny] += (long)F.nz * nt;
-      ytw[F.ny] = F.twy.as<float2>();
-    }
-    {
-      const long L = col_tile_cols(F.nz), nt = (F.nxh + L - 1) / L;
-      ZSeg s{S, F.nxh, (long)F.ny * F.pitch, F.pitch, nt, F.ny, ztot[F.nz], e.otf.params(e.table, 1.0 / (double)F.nvox)};
-      zs[F.nz].push_back(s);
-      ztot[F.nz] += (long)F.ny * nt;
-      ztw[F.nz] = F.twz.as<float2>();
-    }
-  }
-  for (auto& kv : ys) {
-    ColGroup g;
-    g.len = kv.first;
-    g.nseg = (int)kv.second.size();
-    g.total = ytot[kv.first];
-    g.tw = ytw[kv.first];
-    g.segs.ensure(sizeof(ColSeg) * kv.second.size());
-    CDN_HIP(hipMemcpy(g.segs.p, kv.second.data(), sizeof(ColSeg) * kv.second.size(), hipMemcpyHostToDevice));
-    fft_prepare_ragged(g.len);
-    yg.push_back(std::move(g));
-  }
-  for (auto& kv : zs) {
-    ColGroup g;
-    g.len = kv.first;
-    g.nseg = (int)kv.second.size();
-    g.total = ztot[kv.first];
-    g.tw = ztw[kv.first];
-    g.segs.ensure(sizeof(ZSeg) * kv.second.size());
-    CDN_HIP(hipMemcpy(g.segs.p, kv.second.data(), sizeof(ZSeg) * kv.second.size(), hipMemcpyHostToDevice));
-    fft_prepare_ragged(g.len);
-    zg.push_back(std::move(g));
-  }
-  for (const XGroup& g : xg) fft_prepare_ragged(g.nx);
+  built_npart = npart;
   cap_grouped.reset();
 }
 
-int Batch::grouped_launches() const { return 3 * (int)xg.size() + 4 * (int)yg.size() + 2 * (int)zg.size() + 1; }
+int Batch::grouped_launches() const {
+  int n = 0;
+  for (const Partition& P : parts) n += 3 * (int)P.xg.size() + 4 * (int)P.yg.size() + 2 * (int)P.zg.size() + 1;
+  return n;
+}
 
 void Batch::enqueue_engine(hipStream_t st, int i, int iters) {
   RLEngine& e = *engines[i];
@@ -228,67 +254,64 @@ void Batch::enqueue_streams(int iters) {
   }
 }
 
-void Batch::enqueue_grouped(hipStream_t st, int iters) {
+void Batch::enqueue_partition(const Partition& P, hipStream_t st, int iters) {
   const int n = (int)engines.size();
   const float eps = FLT_EPSILON;  // src/RL-Biggs-Andrews.cpp:274
   float* c = a_xa.as<float>();
   float* pv = a_xb.as<float>();
-  CDN_HIP(hipMemcpyAsync(c, a_raw.p, a_raw.bytes, hipMemcpyDeviceToDevice, st));  // X_0 = R, every stack
   float2* spec = a_spec.as<float2>();
-  // the launches of one pass (one per length) touch disjoint stacks: with
-  // group_streams > 1 they spread over that many streams, joined before the
-  // next pass, so one launch's tail overlaps the next one's work
-  const int S = std::max(1, std::min(group_streams, (int)streams.size() - 1));
-  auto pass = [&](size_t count, auto&& launch) {
-    if (S <= 1 || count <= 1) {
-      for (size_t i = 0; i < count; ++i) launch(i, st);
-      return;
+  for (int k = 0; k < iters; ++k) {
+    double* lamk = g_lam.as<double>() + 4 * (size_t)n * k;
+    if (k >= 2)
+      accel_factor_ragged(st, g_part.as<double>(), g_ranges.as<int>() + 2 * P.p0, P.p1 - P.p0, (double)eps,
+                          lamk + 4 * P.p0);
+    const int mode = k >= 2 ? XIN_PRED : XIN_PLAIN;
+    for (const XGroup& g : P.xg)
+      launch_x_r2c(g.nx, st, c + g.voff, pv + g.voff, a_y.as<float>() + g.voff, lamk + 4 * g.first, mode,
+                   spec + g.soff, g.rows, g.pitch, g.tw, 0, g.row_starts.as<long>(), g.count);
+    for (int f = 0; f < 2; ++f) {
+      for (const ColGroup& g : P.yg)
+        launch_col_ragged(g.len, 0, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
+      for (const ColGroup& g : P.zg)
+        launch_z_ragged(g.len, st, g.segs.as<ZSeg>(), g.nseg, g.total, f, g.tw, grid_cap);
+      for (const ColGroup& g : P.yg)
+        launch_col_ragged(g.len, 1, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
+      if (f == 0)
+        for (const XGroup& g : P.xg)
+          launch_x_lrcore(g.nx, st, spec + g.soff, a_raw.as<float>() + g.voff, g.rows, g.pitch, g.tw, eps);
     }
-    const int ns = (int)std::min<size_t>(count, (size_t)S);
+    const float* yv = k >= 2 ? a_y.as<float>() : c;
+    for (const XGroup& g : P.xg) {
+      const int nb = launch_x_update(g.nx, st, spec + g.soff, yv + g.voff, pv + g.voff, a_g.as<float>() + g.voff,
+                                     g_part.as<double>() + 2 * g.part0, k >= 1, g.rows, g.pitch, g.tw);
+      if (nb != g.nb) fail("batch: the x-update launch shape changed under the grouped schedule");
+    }
+    std::swap(c, pv);
+  }
+}
+
+void Batch::enqueue_grouped(hipStream_t st, int iters) {
+  const int n = (int)engines.size();
+  copy_f32(st, a_raw.as<float>(), a_xa.as<float>(), a_raw.bytes / sizeof(float));  // X_0 = R, every stack
+  if (parts.size() == 1) {
+    enqueue_partition(parts[0], st, iters);
+  } else {
+    // fork: partition q on working stream q; join on `st`
+    const int S = (int)streams.size() - 1;
     CDN_HIP(hipEventRecord(events[0], st));
-    for (int s = 0; s < ns; ++s) CDN_HIP(hipStreamWaitEvent(streams[s], events[0], 0));
-    for (size_t i = 0; i < count; ++i) launch(i, streams[i % ns]);
-    for (int s = 0; s < ns; ++s) {
+    for (size_t q = 0; q < parts.size(); ++q) {
+      hipStream_t sq = streams[q % S];
+      CDN_HIP(hipStreamWaitEvent(sq, events[0], 0));
+      enqueue_partition(parts[q], sq, iters);
+    }
+    for (int s = 0; s < std::min<int>(S, (int)parts.size()); ++s) {
       CDN_HIP(hipEventRecord(events[1 + s], streams[s]));
       CDN_HIP(hipStreamWaitEvent(st, events[1 + s], 0));
     }
-  };
-  for (int k = 0; k < iters; ++k) {
-    double* lamk = g_lam.as<double>() + 4 * (size_t)n * k;
-    if (k >= 2) accel_factor_ragged(st, g_part.as<double>(), g_ranges.as<int>(), n, (double)eps, lamk);
-    const int mode = k >= 2 ? XIN_PRED : XIN_PLAIN;
-    pass(xg.size(), [&](size_t i, hipStream_t s) {
-      XGroup& g = xg[i];
-      launch_x_r2c(g.nx, s, c + g.voff, pv + g.voff, a_y.as<float>() + g.voff, lamk + 4 * g.first, mode,
-                   spec + g.soff, g.rows, g.pitch, g.tw, 0, g.row_starts.as<long>(), g.count);
-    });
-    for (int f = 0; f < 2; ++f) {
-      for (int inv = 0; inv < 2; ++inv) {
-        pass(yg.size(), [&](size_t i, hipStream_t s) {
-          ColGroup& g = yg[i];
-          launch_col_ragged(g.len, inv, s, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
-        });
-        if (inv == 0)
-          pass(zg.size(), [&](size_t i, hipStream_t s) {
-            ColGroup& g = zg[i];
-            launch_z_ragged(g.len, s, g.segs.as<ZSeg>(), g.nseg, g.total, f, g.tw, grid_cap);
-          });
-      }
-      if (f == 0)
-        pass(xg.size(), [&](size_t i, hipStream_t s) {
-          XGroup& g = xg[i];
-          launch_x_lrcore(g.nx, s, spec + g.soff, a_raw.as<float>() + g.voff, g.rows, g.pitch, g.tw, eps);
-        });
-    }
-    const float* yv = k >= 2 ? a_y.as<float>() : c;
-    pass(xg.size(), [&](size_t i, hipStream_t s) {
-      XGroup& g = xg[i];
-      const int nb = launch_x_update(g.nx, s, spec + g.soff, yv + g.voff, pv + g.voff, a_g.as<float>() + g.voff,
-                                     g_part.as<double>() + 2 * g.part0, k >= 1, g.rows, g.pitch, g.tw);
-      if (nb != g.nb) fail("batch: the x-update launch shape changed under the grouped schedule");
-    });
-    std::swap(c, pv);
   }
+  // X_k ends in xb after an odd count
+  float* c = iters % 2 ? a_xb.as<float>() : a_xa.as<float>();
+  float* pv = iters % 2 ? a_xa.as<float>() : a_xb.as<float>();
   cur = c;
   prev = pv;
   for (int i = 0; i < n; ++i) {
@@ -305,11 +328,12 @@ std::vector<const void*> Batch::addresses(bool grouped) const {
   }
   if (grouped) {
     for (const DevBuf* b : {&a_raw, &a_xa, &a_xb, &a_y, &a_g, &a_spec, &g_part, &g_lam, &g_ranges}) a.push_back(b->p);
-    for (const XGroup& g : xg) a.push_back(g.row_starts.p);
-    for (const ColGroup& g : yg) a.push_back(g.segs.p);
-    for (const ColGroup& g : zg) a.push_back(g.segs.p);
+    for (const Partition& P : parts) {
+      for (const XGroup& g : P.xg) a.push_back(g.row_starts.p);
+      for (const ColGroup& g : P.yg) a.push_back(g.segs.p);
+      for (const ColGroup& g : P.zg) a.push_back(g.segs.p);
+    }
     a.push_back(reinterpret_cast<const void*>(grid_cap));
-    a.push_back(reinterpret_cast<const void*>((long)group_streams));
   }
   return a;
 }
@@ -350,12 +374,13 @@ double Batch::run(int iters, int mode) {
     e->prepare(iters);
   }
   g_lam.ensure(sizeof(double) * 4 * engines.size() * (size_t)iters);
-  if (grouped)
-    for (const XGroup& g : xg)
-      if (x_update_rows_per_block(g.nx) != g.rpb) {  // the autotuner changed an x launch shape
-        build_groups();
-        break;
-      }
+  if (grouped) {
+    bool stale = built_npart != npart;
+    for (const Partition& P : parts)
+      for (const XGroup& g : P.xg)
+        if (x_update_rows_per_block(g.nx) != g.rpb) stale = true;  // the autotuner changed an x launch shape
+    if (stale) build_groups();
+  }
   if (mode == BATCH_GRAPH) capture(cap_streams, iters, false);
   if (mode == BATCH_GROUPED_GRAPH) capture(cap_grouped, iters, true);
   CDN_HIP(hipDeviceSynchronize());

@@ -61,12 +61,19 @@ struct Batch {
     const float2* tw = nullptr;
     DevBuf segs;
   };
-  std::vector<XGroup> xg;
-  std::vector<ColGroup> yg, zg;
+  // the grouped schedule of one partition: the stacks at arena positions
+  // p0 .. p1 - 1, run on a stream of their own
+  struct Partition {
+    int p0 = 0, p1 = 0;
+    std::vector<XGroup> xg;
+    std::vector<ColGroup> yg, zg;
+  };
+  std::vector<Partition> parts;
   DevBuf g_part, g_lam, g_ranges;
   int lam_iters = 0;       // iterations whose factors g_lam holds (last grouped run)
   long grid_cap = 0;       // > 0: persistent column-pass grids of at most this many workgroups
-  int group_streams = 1;   // streams the launches of one pass spread over (they touch disjoint stacks)
+  int npart = 1;           // partitions of the grouped schedule (concurrent, one stream each)
+  int built_npart = 0;     // partitions the current tables were built for
   int last_mode = -1;
 
   CapturedRun cap_streams, cap_grouped;
@@ -88,6 +95,7 @@ struct Batch {
   void enqueue_engine(hipStream_t st, int i, int iters);
   void enqueue_streams(int iters);
   void enqueue_grouped(hipStream_t st, int iters);
+  void enqueue_partition(const Partition& P, hipStream_t st, int iters);
   void capture(CapturedRun& c, int iters, bool grouped);
   std::vector<const void*> addresses(bool grouped) const;
 };

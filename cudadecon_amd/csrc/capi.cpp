@@ -609,12 +609,11 @@ int cudaDecon_autotune_clear(void) {
   });
 }
 
-int cudaDecon_fft_variant(int n, int* cols, int* xthreads, int* col_nt_load) {
+int cudaDecon_fft_variant(int n, int* fields, int nfields) {
   FftVariant v;
   const bool tuned = fft_get_variant(n, &v);
-  if (cols) *cols = v.cols;
-  if (xthreads) *xthreads = v.xthreads;
-  if (col_nt_load) *col_nt_load = v.col_nt_load;
+  const int f[6] = {v.cols, v.xthreads, v.col_nt_load, v.xcd_remap, v.col_tpt, v.zf_waves};
+  for (int i = 0; fields && i < nfields && i < 6; ++i) fields[i] = f[i];
   return tuned ? 1 : 0;
 }
 
@@ -1073,10 +1072,10 @@ int cudaDecon_batch_set_grid_cap(cudaDecon_batch* h, long max_workgroups) {
   });
 }
 
-int cudaDecon_batch_set_group_streams(cudaDecon_batch* h, int n) {
+int cudaDecon_batch_set_partitions(cudaDecon_batch* h, int n) {
   return guard([&]() -> int {
-    if (!h || n < 1) fail("cudaDecon_batch_set_group_streams: bad argument");
-    h->b.group_streams = n;
+    if (!h || n < 1 || n > (int)h->b.streams.size() - 1) fail("cudaDecon_batch_set_partitions: bad argument");
+    h->b.npart = n;
     return 1;
   });
 }

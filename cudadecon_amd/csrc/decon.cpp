@@ -227,10 +227,10 @@ void RLEngine::setup(int nx, int ny, int nz, const OtfTable& t, const float ksca
 void RLEngine::load_raw(const float* in, bool on_device, hipStream_t st) {
   const size_t vb = sizeof(float) * (size_t)total_vox();
   if (on_device)
-    CDN_HIP(hipMemcpyAsync(raw.p, in, vb, hipMemcpyDeviceToDevice, st));
+    copy_f32(st, in, raw.as<float>(), (size_t)total_vox());
   else
     copy_to_device(st, raw.p, in, vb);
-  CDN_HIP(hipMemcpyAsync(xa.p, raw.p, vb, hipMemcpyDeviceToDevice, st));
+  copy_f32(st, raw.as<float>(), xa.as<float>(), (size_t)total_vox());
   cur = xa.as<float>();
   prev = xb.as<float>();
 }
@@ -239,14 +239,14 @@ void RLEngine::load_stack(int s, const float* in, bool on_device, hipStream_t st
   if (s < 0 || s >= nstacks) fail("stack index out of range");
   const size_t vb = sizeof(float) * (size_t)fft.nvox, off = (size_t)fft.nvox * s;
   if (on_device)
-    CDN_HIP(hipMemcpyAsync(raw.as<float>() + off, in, vb, hipMemcpyDeviceToDevice, st));
+    copy_f32(st, in, raw.as<float>() + off, (size_t)fft.nvox);
   else
     copy_to_device(st, raw.as<float>() + off, in, vb);
-  CDN_HIP(hipMemcpyAsync(cur + off, raw.as<float>() + off, vb, hipMemcpyDeviceToDevice, st));  // its X_0
+  copy_f32(st, raw.as<float>() + off, cur + off, (size_t)fft.nvox);  // its X_0
 }
 
 void RLEngine::reset(hipStream_t st) {
-  CDN_HIP(hipMemcpyAsync(xa.p, raw.p, sizeof(float) * (size_t)total_vox(), hipMemcpyDeviceToDevice, st));
+  copy_f32(st, raw.as<float>(), xa.as<float>(), (size_t)total_vox());
   cur = xa.as<float>();
   prev = xb.as<float>();
 }

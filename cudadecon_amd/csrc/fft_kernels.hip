@@ -267,7 +267,7 @@ void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols,
                     long outer_stride, long n_outer, const float2* tw) {
   FftVariant tv;
   const bool tuned = col_variant(len, ncols, &tv);
-  const CShape cs = c_shape(len, tuned ? tv.cols : -1);
+  const CShape cs = c_shape(len, tuned ? tv.cols : -1, tuned ? tv.col_tpt : -1);
   long ntiles = (ncols + cs.L - 1) / cs.L;
   const unsigned grid = grid1(ntiles * n_outer);
   void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &tw};
@@ -297,7 +297,7 @@ void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_st
                      int conj, const OtfParams& otf, const float2* tw, int nstacks, long stack_stride) {
   FftVariant tv;
   const bool tuned = col_variant(nz, ncols, &tv);
-  const CShape cs = c_shape(nz, tuned ? tv.cols : -1);
+  const CShape cs = c_shape(nz, tuned ? tv.cols : -1, tuned ? tv.col_tpt : -1);
   long ntiles = (ncols + cs.L - 1) / cs.L;
   if (nstacks < 1) fail("launch_z_filter: no stacks");
   const unsigned grid = grid1(ntiles * ny * nstacks);

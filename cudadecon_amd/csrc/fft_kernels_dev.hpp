@@ -191,9 +191,11 @@ struct CShape {
 #define CDN_CTPT_OVERRIDE 0, 0
 #endif
 constexpr int kCTptOverride[] = {CDN_CTPT_OVERRIDE};
-constexpr CShape c_shape(int N, int cols = -1) {
+// cols, tpt > 0: a run-time compiled variant's shape (what its -D overrides
+// give the kernel)
+constexpr CShape c_shape(int N, int cols = -1, int tpt = -1) {
   CShape s{};
-  s.TPT = lookup_override(kCTptOverride, N) > 0 ? lookup_override(kCTptOverride, N) : tpt_col(N);
+  s.TPT = tpt > 0 ? tpt : lookup_override(kCTptOverride, N) > 0 ? lookup_override(kCTptOverride, N) : tpt_col(N);
   s.L = lc_for(N, s.TPT, cols);
   s.THREADS = s.TPT * s.L;
   s.E = N / s.TPT;

@@ -175,6 +175,9 @@ std::string FftVariant::defines(int n) const {
   if (cols > 0) d += " -DCDN_COLS_OVERRIDE=" + std::to_string(n) + "," + std::to_string(cols);
   if (xthreads > 0) d += " -DCDN_XTHREADS_OVERRIDE=" + std::to_string(n) + "," + std::to_string(xthreads);
   if (col_nt_load >= 0) d += " -DCDN_NT_COL_LOAD=" + std::to_string(col_nt_load);
+  if (xcd_remap >= 0) d += " -DCDN_XCD_REMAP=" + std::to_string(xcd_remap);
+  if (col_tpt > 0) d += " -DCDN_CTPT_OVERRIDE=" + std::to_string(n) + "," + std::to_string(col_tpt);
+  if (zf_waves > 0) d += " -DCDN_ZF_WAVES=" + std::to_string(zf_waves);
   return d;
 }
 

@@ -271,6 +271,28 @@ void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes) {
   CDN_HIP(hipGetLastError());
 }
 
+__global__ void __launch_bounds__(BS) k_copy_f32(const float* __restrict__ in, float* __restrict__ out, long n) {
+  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(in + i), out + i);
+}
+
+// Device-to-device copy of n floats as kernels (16-byte vectors where both
+// ends allow, scalar otherwise): a plain stream operation under graph
+// capture, at the streaming copy's rate whatever the size.
+void copy_f32(hipStream_t st, const float* in, float* out, size_t n) {
+  if (n == 0) return;
+  size_t head = 0;
+  if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) % 16 == 0) {
+    head = n / 4 * 4;
+    if (head) stream_copy(st, in, out, sizeof(float) * head);
+  }
+  if (n > head) {
+    hipLaunchKernelGGL(k_copy_f32, dim3(nblocks((long)(n - head))), dim3(BS), 0, st, in + head, out + head,
+                       (long)(n - head));
+    CDN_HIP(hipGetLastError());
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fill, and FP64 sum / min / max of a volume (test and diagnostics support)
 // ---------------------------------------------------------------------------

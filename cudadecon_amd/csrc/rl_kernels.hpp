@@ -31,6 +31,7 @@ void wiener_multiply(hipStream_t st, float2* S, int nxh, int ny, int nz, long pi
                      float kxscale, float kyscale, float dkx, float dky, float rcut, float w2, float inv_n);
 // bytes must be a multiple of 16
 void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes);
+void copy_f32(hipStream_t st, const float* in, float* out, size_t n);
 // acceleration factor of nstacks stacks, nb FP64 partial pairs each
 // (stack s: part[2 (s nb ..)]) -> lam[4 s] = factor, lam[4 s + 1..2] = sums
 void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam, int nstacks = 1);

@@ -7,13 +7,18 @@
 //              workgroup;
 //   columns    the column FFT (y) and / or the z filter with half, the
 //              default and twice the default columns per tile (within 1024
-//              threads and 160 KB of LDS), each with cached or non-temporal
-//              loads;
-// and installs the fastest combination (fft_set_variant).  Variants only
-// change how lines are grouped into workgroups, so results are bitwise
-// those of the defaults.  Candidates are compiled at run time (hipRTC),
-// several at once; decisions are cached in <cache>/tune_gfx950.txt keyed by
-// length, role and the grid timed.
+//              threads and 160 KB of LDS), and twice the columns at half the
+//              threads per line; each with cached or non-temporal loads and
+//              with the built-in or the other workgroup-to-XCD renumbering
+//              (consecutive tiles of one line on one XCD share its L2 and its
+//              translation caches); then, for a z length, the chosen shape's
+//              z filter built for 6 waves per SIMD;
+// and installs the fastest combination (fft_set_variant).  Column variants
+// only change how lines are grouped into workgroups and threads, so their
+// results are bitwise those of the defaults; an x-pass variant also regroups
+// the dot products' per-block partial sums (FP64, another summation order).  Candidates are compiled at run time (hipRTC),
+// several at once; decisions are cached in <cache>/tune_gfx950_v2.txt keyed
+// by length, role and the grid timed.
 #include "tune.hpp"
 
 #include "common.hpp"
@@ -41,17 +46,32 @@ struct Roles {
 
 long vox(const std::array<int, 3>& g) { return (long)g[0] * g[1] * g[2]; }
 
+// LDS bytes of a column / z-filter workgroup of length n, tile width L and
+// TPT threads per line (half when the exchange goes in halves)
+long col_lds_bytes(int n, int L, int tpt) {
+  const bool half = CDN_SPLIT_LDS != 0 && plan_exact(n, tpt);
+  return (long)n * L * (half ? 4 : 8) + (n == 128 ? 8L * CDN_ZF_ROWS * n : 0L);  // + the z filter's OTF rows
+}
+
 std::vector<FftVariant> col_candidates(int n) {
   const CShape d = c_shape(n);
   std::vector<FftVariant> out;
-  for (int L : {d.L / 2, d.L, d.L * 2}) {
-    if (L < 1 || L * d.TPT > 1024 || (long)n * L * 8 > 163840) continue;
-    for (int nt : {0, 1}) {
-      FftVariant v;
-      if (L != d.L) v.cols = L;
-      if (nt != 0) v.col_nt_load = nt;
-      out.push_back(v);
-    }
+  // {threads per line, columns}: the built-in threads with half, the same
+  // and twice the columns; half the threads (twice the elements each, up
+  // to 32) with twice the columns, for a wider tile within 1024 threads
+  std::vector<std::pair<int, int>> shapes{{d.TPT, d.L / 2}, {d.TPT, d.L}, {d.TPT, d.L * 2}};
+  if (d.TPT % 2 == 0 && n / (d.TPT / 2) <= 32) shapes.push_back({d.TPT / 2, d.L * 2});
+  for (auto [tpt, L] : shapes) {
+    if (L < 1 || L * tpt > 1024 || col_lds_bytes(n, L, tpt) > 163840) continue;
+    for (int nt : {0, 1})
+      for (int remap : {-1, L * 8 >= 128 ? 2 : 0}) {  // the built-in renumbering and the other one
+        FftVariant v;
+        if (L != d.L) v.cols = L;
+        if (tpt != d.TPT) v.col_tpt = tpt;
+        if (nt != 0) v.col_nt_load = nt;
+        v.xcd_remap = remap;
+        out.push_back(v);
+      }
   }
   return out;
 }
@@ -134,7 +154,9 @@ double time_role(Bench& b, int role, int reps) {
 
 std::string vstr(const FftVariant& v) {
   std::ostringstream o;
-  o << "{\"cols\": " << v.cols << ", \"xthreads\": " << v.xthreads << ", \"col_nt_load\": " << v.col_nt_load << "}";
+  o << "{\"cols\": " << v.cols << ", \"xthreads\": " << v.xthreads << ", \"col_nt_load\": " << v.col_nt_load
+    << ", \"xcd_remap\": " << v.xcd_remap << ", \"col_tpt\": " << v.col_tpt << ", \"zf_waves\": " << v.zf_waves
+    << "}";
   return o.str();
 }
 
@@ -153,7 +175,7 @@ std::map<std::string, FftVariant> load_cache(const std::string& path) {
     int n;
     std::string role, grid;
     FftVariant v;
-    if (is >> n >> role >> grid >> v.cols >> v.xthreads >> v.col_nt_load)
+    if (is >> n >> role >> grid >> v.cols >> v.xthreads >> v.col_nt_load >> v.xcd_remap >> v.col_tpt >> v.zf_waves)
       m[std::to_string(n) + ' ' + role + ' ' + grid] = v;  // the layout gkey writes
   }
   return m;
@@ -161,7 +183,9 @@ std::map<std::string, FftVariant> load_cache(const std::string& path) {
 
 void append_cache(const std::string& path, const std::string& key, const FftVariant& v) {
   std::ofstream f(path, std::ios::app);
-  if (f) f << key << ' ' << v.cols << ' ' << v.xthreads << ' ' << v.col_nt_load << '\n';
+  if (f)
+    f << key << ' ' << v.cols << ' ' << v.xthreads << ' ' << v.col_nt_load << ' ' << v.xcd_remap << ' ' << v.col_tpt
+      << ' ' << v.zf_waves << '\n';
 }
 
 }  // namespace
@@ -184,7 +208,7 @@ std::string autotune(const std::vector<std::array<int, 3>>& grids, int reps, boo
     rz.z = true;
   }
   const std::string dir = jit_cache_dir();
-  const std::string cpath = dir.empty() ? "" : dir + "/tune_gfx950.txt";
+  const std::string cpath = dir.empty() ? "" : dir + "/tune_gfx950_v2.txt";
   std::map<std::string, FftVariant> cache;
   if (use_cache && !cpath.empty()) cache = load_cache(cpath);
 
@@ -285,11 +309,28 @@ std::string autotune(const std::vector<std::array<int, 3>>& grids, int reps, boo
               bc = w;
             }
           }
-          rep << "]}";
+          rep << "]";
+          // then the z filter of the chosen shape at 6 waves per SIMD (a
+          // tighter register budget, 3 workgroups of 512 threads per CU)
+          if (r.z && bc.zf_waves == 0 && zf_waves(n) < 6) {
+            FftVariant w = bc;
+            w.zf_waves = 6;
+            fft_set_variant(n, bc);
+            const double t0 = time_role(bench(r.gz), 2, reps);
+            jit_prepare(n, w);
+            fft_set_variant(n, w);
+            const double t1 = time_role(bench(r.gz), 2, reps);
+            rep << ", \"z_waves\": [[0, " << t0 << "], [6, " << t1 << "]]";
+            if (t1 < t0 * 0.99) bc = w;
+          }
+          rep << "}";
           if (!cpath.empty()) append_cache(cpath, key, bc);
         }
         best.cols = bc.cols;
         best.col_nt_load = bc.col_nt_load;
+        best.xcd_remap = bc.xcd_remap;
+        best.col_tpt = bc.col_tpt;
+        best.zf_waves = bc.zf_waves;
       }
       fft_set_variant(n, best);
       if (!best.is_default()) jit_prepare(n, best);

@@ -148,8 +148,12 @@ int cudaDecon_wiener(float* vol, int nx, int ny, int nz, const float* otf, int o
 int cudaDecon_autotune(const int* dims_zyx, int count, int reps, int use_cache, char* report, int report_len);
 /* back to the built-in shapes for every length */
 int cudaDecon_autotune_clear(void);
-/* 1 if length n has a tuned shape (and its parameters, 0 / -1 = built-in) */
-int cudaDecon_fft_variant(int n, int* cols, int* xthreads, int* col_nt_load);
+/* 1 if length n has a tuned shape; its first nfields parameters (0 / -1 =
+ * built-in) go to fields: columns per column tile, threads per x-pass
+ * workgroup, non-temporal column loads, workgroup-to-XCD renumbering of the
+ * column passes (0 none, 1 tiles narrower than a cache line, 2 every tile),
+ * threads per column line, the z filter's minimum waves per SIMD. */
+int cudaDecon_fft_variant(int n, int* fields, int nfields);
 
 /* RL-Biggs-Andrews engine on HBM-resident state (the iteration of
  * RL_interface without pre/post-processing).  The OTF is given as its
@@ -256,10 +260,11 @@ int cudaDecon_batch_lambdas(cudaDecon_batch* h, int i, int iters, double* out);
 /* Grouped modes: column passes on a persistent grid of at most
  * max_workgroups workgroups that walk the tiles (0: one workgroup per tile). */
 int cudaDecon_batch_set_grid_cap(cudaDecon_batch* h, long max_workgroups);
-/* Grouped modes: the launches of one pass (one per transform length, on
- * disjoint stacks) spread over up to n of the batch's streams, joined before
- * the next pass (1: all on one stream). */
-int cudaDecon_batch_set_group_streams(cudaDecon_batch* h, int n);
+/* Grouped modes: cut the stacks (ordered by x length) into n ranges of about
+ * equal voxel count, each grouped on its own and run on its own stream, so
+ * that passes of different ranges overlap (1: one schedule on one stream;
+ * at most the batch's nstreams). */
+int cudaDecon_batch_set_partitions(cudaDecon_batch* h, int n);
 /* Kernel launches per iteration of the grouped modes. */
 int cudaDecon_batch_launches(cudaDecon_batch* h);
 int cudaDecon_batch_set_raw(cudaDecon_batch* h, int i, const float* raw);

@@ -148,6 +148,17 @@ def test_grouped_modes_match_single_engines(cdn, orc):
     b.run(7, "grouped")
     for i in range(len(MIXED)):
         assert np.array_equal(b.get(i), refs[i]), ("persistent", i)
+    b.set_grid_cap(0)
+    for parts in (2, 3):
+        b.set_partitions(parts)
+        for mode in ("grouped", "grouped_graph", "grouped_graph"):
+            print("partitions", parts, mode, flush=True)
+            b.run(7, mode)
+            for i in range(len(MIXED)):
+                assert np.array_equal(b.get(i), refs[i]), (parts, mode, i)
+                assert np.array_equal(b.lambdas(i, 7), lams[i]), (parts, mode, i)
+    with pytest.raises(RuntimeError):
+        b.set_partitions(4)  # more than the batch's streams
     b.close()
 
 

@@ -1,7 +1,8 @@
 """Autotuner of the FFT launch shapes (cudaDecon_autotune).
 
 Variants only regroup lines into workgroups: column-pass variants (columns
-per tile, non-temporal loads) leave every result bitwise unchanged; an
+per tile, threads per line, non-temporal loads, workgroup-to-XCD
+renumbering) leave every result bitwise unchanged; an
 x-pass variant also regroups the per-block partial sums of the acceleration
 factor's dot products (FP64 sums in another order), so RL results agree to
 FP32 rounding.  Variants are forced through the tuner's decision cache in a
@@ -39,7 +40,7 @@ def case(cdn):
 def _force(cdn, tmp_path, lines):
     d = tmp_path / "cache"
     (d / "cudadecon_amd").mkdir(parents=True)
-    (d / "cudadecon_amd" / "tune_gfx950.txt").write_text("".join(l + "\n" for l in lines))
+    (d / "cudadecon_amd" / "tune_gfx950_v2.txt").write_text("".join(l + "\n" for l in lines))
     old = os.environ.get("XDG_CACHE_HOME")
     os.environ["XDG_CACHE_HOME"] = str(d)
     try:
@@ -54,10 +55,12 @@ def _force(cdn, tmp_path, lines):
 def test_forced_column_variants_are_bitwise(cdn, case, tmp_path):
     R, tab, X0, lam0 = case
     g = "x".join(map(str, SHAPE))
-    rep = _force(cdn, tmp_path, [f"192 x {g} 0 0 -1", f"96 y {g} 8 0 1", f"32 z {g} 16 0 1"])
+    rep = _force(cdn, tmp_path, [f"192 x {g} 0 0 -1 -1 0 0", f"96 y {g} 8 0 1 0 0 0", f"32 z {g} 128 0 1 2 1 2"])
     print(rep, flush=True)
-    assert cdn.fft_variant(96) == dict(cols=8, xthreads=0, col_nt_load=1)
-    assert cdn.fft_variant(32) == dict(cols=16, xthreads=0, col_nt_load=1)
+    assert cdn.fft_variant(96) == dict(cols=8, xthreads=0, col_nt_load=1, xcd_remap=0, col_tpt=0, zf_waves=0)
+    # z length 32: one thread per line, 128 columns per tile (all 97 in one),
+    # z filter built for 2 waves per SIMD
+    assert cdn.fft_variant(32) == dict(cols=128, xthreads=0, col_nt_load=1, xcd_remap=2, col_tpt=1, zf_waves=2)
     assert cdn.fft_variant(192) is None
     X1, lam1 = _rl(cdn, R, tab)
     assert np.array_equal(X1, X0)
@@ -68,8 +71,8 @@ def test_forced_column_variants_are_bitwise(cdn, case, tmp_path):
 def test_forced_x_variant_agrees_to_rounding(cdn, case, tmp_path):
     R, tab, X0, lam0 = case
     g = "x".join(map(str, SHAPE))
-    _force(cdn, tmp_path, [f"192 x {g} 0 256 -1", f"96 y {g} 0 0 -1", f"32 z {g} 0 0 -1"])
-    assert cdn.fft_variant(192) == dict(cols=0, xthreads=256, col_nt_load=-1)
+    _force(cdn, tmp_path, [f"192 x {g} 0 256 -1 -1 0 0", f"96 y {g} 0 0 -1 -1 0 0", f"32 z {g} 0 0 -1 -1 0 0"])
+    assert cdn.fft_variant(192) == dict(cols=0, xthreads=256, col_nt_load=-1, xcd_remap=-1, col_tpt=0, zf_waves=0)
     X1, lam1 = _rl(cdn, R, tab)
     assert np.abs(X1 - X0).max() <= 1e-5 * np.abs(X0).max()
     assert np.abs(lam1 - lam0).max() <= 1e-9
