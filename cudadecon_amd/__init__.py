@@ -318,10 +318,11 @@ def autotune_clear():
 
 
 def fft_variant(n):
-    """None for the built-in shape of length n, else {cols, xthreads, col_nt_load, xcd_remap, col_tpt, zf_waves}."""
-    v = np.zeros(6, np.int32)
-    tuned = lib().cudaDecon_fft_variant(int(n), v.ctypes.data, 6)
-    names = ("cols", "xthreads", "col_nt_load", "xcd_remap", "col_tpt", "zf_waves")
+    """None for the built-in shape of length n, else its
+    {cols, xthreads, col_nt_load, xcd_remap, col_tpt, zf_waves, zf_rows}."""
+    v = np.zeros(7, np.int32)
+    tuned = lib().cudaDecon_fft_variant(int(n), v.ctypes.data, 7)
+    names = ("cols", "xthreads", "col_nt_load", "xcd_remap", "col_tpt", "zf_waves", "zf_rows")
     return dict(zip(names, map(int, v))) if tuned else None
 
 

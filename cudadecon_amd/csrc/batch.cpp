@@ -173,10 +173,15 @@ void Batch::build_groups() {
       P1.xg.push_back(std::move(g));
     }
     // y passes by y length, z filter by z length
+    // z groups by length and table kind (rotationally averaged or 3-D), so
+    // that each stack runs the z-filter kind of its one-stack engine
+    using ZKey = std::pair<int, bool>;
     std::map<int, std::vector<ColSeg>> ys;
-    std::map<int, std::vector<ZSeg>> zs;
-    std::map<int, const float2*> ytw, ztw;
-    std::map<int, long> ytot, ztot;
+    std::map<ZKey, std::vector<ZSeg>> zs;
+    std::map<int, const float2*> ytw;
+    std::map<ZKey, const float2*> ztw;
+    std::map<int, long> ytot;
+    std::map<ZKey, long> ztot;
     for (int p = P1.p0; p < P1.p1; ++p) {
       const int i = order[p];
       const RLEngine& e = *engines[i];
@@ -191,11 +196,12 @@ void Batch::build_groups() {
       }
       {
         const long L = col_tile_cols(F.nz), nt = (F.nxh + L - 1) / L;
-        ZSeg sg{S, F.nxh, (long)F.ny * F.pitch, F.pitch, nt, F.ny, ztot[F.nz],
-                e.otf.params(e.table, 1.0 / (double)F.nvox)};
-        zs[F.nz].push_back(sg);
-        ztot[F.nz] += (long)F.ny * nt;
-        ztw[F.nz] = F.twz.as<float2>();
+        const OtfParams op = e.otf.params(e.table, 1.0 / (double)F.nvox);
+        const ZKey key{F.nz, op.ra != 0};
+        ZSeg sg{S, F.nxh, (long)F.ny * F.pitch, F.pitch, nt, F.ny, ztot[key], op};
+        zs[key].push_back(sg);
+        ztot[key] += (long)F.ny * nt;
+        ztw[key] = F.twz.as<float2>();
       }
     }
     for (auto& kv : ys) {
@@ -211,7 +217,8 @@ void Batch::build_groups() {
     }
     for (auto& kv : zs) {
       ColGroup g;
-      g.len = kv.first;
+      g.len = kv.first.first;
+      g.ra = kv.first.second;
       g.nseg = (int)kv.second.size();
       g.total = ztot[kv.first];
       g.tw = ztw[kv.first];
@@ -273,7 +280,7 @@ void Batch::enqueue_partition(const Partition& P, hipStream_t st, int iters) {
       for (const ColGroup& g : P.yg)
         launch_col_ragged(g.len, 0, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
       for (const ColGroup& g : P.zg)
-        launch_z_ragged(g.len, st, g.segs.as<ZSeg>(), g.nseg, g.total, f, g.tw, grid_cap);
+        launch_z_ragged(g.len, st, g.segs.as<ZSeg>(), g.nseg, g.total, f, g.tw, grid_cap, g.ra);
       for (const ColGroup& g : P.yg)
         launch_col_ragged(g.len, 1, st, g.segs.as<ColSeg>(), g.nseg, g.total, g.tw, grid_cap);
       if (f == 0)

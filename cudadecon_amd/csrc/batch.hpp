@@ -48,7 +48,7 @@ struct Batch {
   float* cur = nullptr;          // arena X_k after the last grouped run
   float* prev = nullptr;
 
-  // grouped schedule: one launch per pass and length
+  // grouped schedule: one launch per pass and length (z filter: and table kind)
   struct XGroup {
     int nx = 0, first = 0, count = 0;  // arena positions first .. first + count - 1
     long rows = 0, voff = 0, soff = 0, pitch = 0, part0 = 0, nb = 0, rpb = 0;
@@ -57,6 +57,7 @@ struct Batch {
   };
   struct ColGroup {
     int len = 0, nseg = 0;
+    bool ra = false;  // z groups: every table rotationally averaged
     long total = 0;
     const float2* tw = nullptr;
     DevBuf segs;

@@ -612,8 +612,8 @@ int cudaDecon_autotune_clear(void) {
 int cudaDecon_fft_variant(int n, int* fields, int nfields) {
   FftVariant v;
   const bool tuned = fft_get_variant(n, &v);
-  const int f[6] = {v.cols, v.xthreads, v.col_nt_load, v.xcd_remap, v.col_tpt, v.zf_waves};
-  for (int i = 0; fields && i < nfields && i < 6; ++i) fields[i] = f[i];
+  const int f[7] = {v.cols, v.xthreads, v.col_nt_load, v.xcd_remap, v.col_tpt, v.zf_waves, v.zf_rows};
+  for (int i = 0; fields && i < nfields && i < 7; ++i) fields[i] = f[i];
   return tuned ? 1 : 0;
 }
 

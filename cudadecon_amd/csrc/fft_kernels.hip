@@ -260,7 +260,7 @@ void fft_prepare_cols(int len, long ncols) {
   FftVariant v;
   if (fft_get_variant(len, &v) || !fft_balanced_cols(len, ncols, &v)) return;
   // compile and load (grid 0: no launch) before any stream capture
-  for (JitKernel k : {JK_COL_F, JK_COL_I, JK_Z_FILTER}) jit_launch(len, k, 0, 1, nullptr, nullptr, v);
+  for (JitKernel k : {JK_COL_F, JK_COL_I, JK_Z_FILTER, JK_Z_FILTER_RA}) jit_launch(len, k, 0, 1, nullptr, nullptr, v);
 }
 
 void launch_col_fft(int len, int inverse, hipStream_t st, float2* S, long ncols, long line_stride,
@@ -303,19 +303,25 @@ void launch_z_filter(int nz, hipStream_t st, float2* S, long ncols, long line_st
   const unsigned grid = grid1(ntiles * ny * nstacks);
   OtfParams op = otf;
   void* args[] = {&S, &ncols, &line_stride, &outer_stride, &ntiles, &ny, &stack_stride, &conj, &op, &tw};
+  // the averaged-table kind for a rotationally averaged OTF
+  const JitKernel jk = op.ra ? JK_Z_FILTER_RA : JK_Z_FILTER;
   if (tuned) {
-    jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args, tv);
+    jit_launch(nz, jk, grid, cs.THREADS, st, args, tv);
   } else {
     switch (nz) {
-#define CDN_CASE(N)                                                                                      \
-  case N:                                                                                                \
-    CDN_LAUNCH(k_z_filter, N, grid, cs.THREADS, S, ncols, line_stride, outer_stride, ntiles, ny,         \
-               stack_stride, conj, op, tw);                                                              \
+#define CDN_CASE(N)                                                                                          \
+  case N:                                                                                                    \
+    if (op.ra)                                                                                               \
+      hipLaunchKernelGGL((k_z_filter<N, ZF_RA>), dim3(grid), dim3(cs.THREADS), 0, st, S, ncols, line_stride, \
+                         outer_stride, ntiles, ny, stack_stride, conj, op, tw);                              \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_z_filter<N, ZF_ANY>), dim3(grid), dim3(cs.THREADS), 0, st, S, ncols,             \
+                         line_stride, outer_stride, ntiles, ny, stack_stride, conj, op, tw);                 \
     break;
       CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
       default:
-        jit_launch(nz, JK_Z_FILTER, grid, cs.THREADS, st, args);
+        jit_launch(nz, jk, grid, cs.THREADS, st, args);
     }
   }
   CDN_HIP(hipGetLastError());
@@ -326,7 +332,8 @@ long col_tile_cols(int len) { return c_shape(len).L; }
 void fft_prepare_ragged(int len) {
   if (fft_length_precompiled(len)) return;
   for (JitKernel k : {JK_X_R2C_RAGGED, JK_COL_RAGGED_F, JK_COL_RAGGED_I, JK_Z_RAGGED, JK_COL_RAGGED_FP,
-                      JK_COL_RAGGED_IP, JK_Z_RAGGED_P, JK_X_C2R, JK_X_LRCORE, JK_X_UPDATE})
+                      JK_COL_RAGGED_IP, JK_Z_RAGGED_P, JK_Z_RAGGED_RA, JK_Z_RAGGED_P_RA, JK_X_C2R, JK_X_LRCORE,
+                      JK_X_UPDATE})
     jit_launch(len, k, 0, 1, nullptr, nullptr);
 }
 
@@ -372,26 +379,38 @@ void launch_col_ragged(int len, int inverse, hipStream_t st, const ColSeg* segs,
   CDN_HIP(hipGetLastError());
 }
 
+// the kernel instance for (persistent, kind)
+template <int N>
+static void z_ragged_n(bool pers, bool ra, unsigned grid, unsigned threads, hipStream_t st, const ZSeg* segs,
+                       int nseg, long total, int conj, const float2* tw) {
+  if (pers && ra)
+    hipLaunchKernelGGL((k_z_ragged<N, true, ZF_RA>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, conj, tw);
+  else if (pers)
+    hipLaunchKernelGGL((k_z_ragged<N, true, ZF_ANY>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, conj, tw);
+  else if (ra)
+    hipLaunchKernelGGL((k_z_ragged<N, false, ZF_RA>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, conj, tw);
+  else
+    hipLaunchKernelGGL((k_z_ragged<N, false, ZF_ANY>), dim3(grid), dim3(threads), 0, st, segs, nseg, total, conj,
+                       tw);
+}
+
 void launch_z_ragged(int nz, hipStream_t st, const ZSeg* segs, int nseg, long total, int conj, const float2* tw,
-                     long grid_cap) {
+                     long grid_cap, bool ra) {
   if (total <= 0) return;
   const CShape cs = c_shape(nz);
   const unsigned grid = ragged_grid(total, grid_cap);
   const bool pers = grid < total;
   void* args[] = {&segs, &nseg, &total, &conj, &tw};
   switch (nz) {
-#define CDN_CASE(N)                                                                                               \
-  case N:                                                                                                         \
-    if (pers)                                                                                                     \
-      hipLaunchKernelGGL((k_z_ragged<N, true>), dim3(grid), dim3(cs.THREADS), 0, st, segs, nseg, total, conj, tw); \
-    else                                                                                                          \
-      hipLaunchKernelGGL((k_z_ragged<N, false>), dim3(grid), dim3(cs.THREADS), 0, st, segs, nseg, total, conj,     \
-                         tw);                                                                                     \
+#define CDN_CASE(N)                                                                    \
+  case N:                                                                              \
+    z_ragged_n<N>(pers, ra, grid, cs.THREADS, st, segs, nseg, total, conj, tw);     \
     break;
     CDN_FFT_LENGTHS(CDN_CASE)
 #undef CDN_CASE
     default:
-      jit_launch(nz, pers ? JK_Z_RAGGED_P : JK_Z_RAGGED, grid, cs.THREADS, st, args);
+      jit_launch(nz, pers ? (ra ? JK_Z_RAGGED_P_RA : JK_Z_RAGGED_P) : (ra ? JK_Z_RAGGED_RA : JK_Z_RAGGED), grid,
+                 cs.THREADS, st, args);
   }
   CDN_HIP(hipGetLastError());
 }

@@ -89,7 +89,9 @@ long col_tile_cols(int len);
 void fft_prepare_ragged(int len);
 void launch_col_ragged(int len, int inverse, hipStream_t st, const ColSeg* segs, int nseg, long total,
                        const float2* tw, long grid_cap = 0);
+// ra: every segment's OTF is rotationally averaged (the kernel kind the
+// one-stack z filter uses for such a table)
 void launch_z_ragged(int nz, hipStream_t st, const ZSeg* segs, int nseg, long total, int conj, const float2* tw,
-                     long grid_cap = 0);
+                     long grid_cap = 0, bool ra = false);
 
 }  // namespace cdn

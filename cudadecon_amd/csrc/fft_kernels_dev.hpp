@@ -657,25 +657,36 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS)
   } while (it < total);
 }
 
+// Kinds of z filter: the OTF table chosen at run time (3-D or rotationally
+// averaged), or built for the rotationally averaged table only (the common
+// case; without the 3-D interpolation's registers).
+enum ZfKind : int { ZF_ANY = 0, ZF_RA = 1 };
+
 // Waves per SIMD the z filter is compiled for (1: the compiler's choice).
 // Length 256 (32-column tiles of 512 threads, 32 KB of LDS) at 6 waves, three
 // workgroups per CU instead of two, for 80 registers and 15 spilled dwords:
 // 4% faster at 256 x 1024 x 1024; the other lengths spill far more and
-// measured slower (64: 0.018 -> 0.023 ms at 64 x 256 x 256).
+// measured slower (64: 0.018 -> 0.023 ms at 64 x 256 x 256).  Length 512
+// (16-column tiles of 512 threads) of the averaged-table kind fits 6 waves
+// in 76 registers without spilling.
 #ifndef CDN_ZF_WAVES
 #define CDN_ZF_WAVES 0
 #endif
-constexpr int zf_waves(int N) {
+constexpr int zf_waves(int N, int kind = ZF_ANY) {
   if (CDN_ZF_WAVES > 0) return CDN_ZF_WAVES;
   const CShape c = c_shape(N);
   const bool half = CDN_SPLIT_LDS != 0 && plan_exact(N, c.TPT);
-  return (N == 256 && c.THREADS == 512 && half && N * c.L * 4 <= 32768) ? 6 : 1;
+  if (N == 256 && c.THREADS == 512 && half && N * c.L * 4 <= 32768) return 6;
+  if (N == 512 && kind == ZF_RA && c.THREADS == 512 && c.E == 16 && half) return 6;
+  return 1;
 }
 // Rows of the rotationally averaged OTF table staged in LDS per tile.  A
 // tile is one ky and L adjacent kx, so its radii span a few table rows.
 // Measured: z length 128 (128 x 512 x 810) 0.117 -> 0.104 ms; 64 no change;
-// 256 (256 x 1024 x 1024, at 6 waves with 15 spilled dwords) 0.508 -> 0.527,
-// so only 128 stages.  0 disables.
+// 256 (256 x 1024 x 1024, an OTF support ending near 70% of Nyquist): the
+// averaged-table kind 0.493 -> 0.457 ms (1 spilled dword, 44 KB of LDS),
+// the kind with both tables 0.508 -> 0.527 (15 spilled dwords); 512 slower
+// (one workgroup per CU fewer).  0 disables.
 #ifndef CDN_ZF_ROWS
 #define CDN_ZF_ROWS 6
 #endif
@@ -685,19 +696,29 @@ __device__ __forceinline__ float zf_ky(long outer, long ny, float kyscale) {
   return (float)kyi * kyscale;
 }
 
-template <int N> struct ZRows {
-  static constexpr int R = N == 128 ? CDN_ZF_ROWS : 0;
+// per-length override (n, rows; rows < 0: none), for run-time variants
+#ifndef CDN_ZF_ROWS_OVERRIDE
+#define CDN_ZF_ROWS_OVERRIDE 0, 0
+#endif
+constexpr int kZfRowsOverride[] = {CDN_ZF_ROWS_OVERRIDE};
+constexpr int zf_rows(int N, int kind = ZF_ANY) {
+  const int o = lookup_override(kZfRowsOverride, N);
+  return o > 0 ? o : o < 0 ? 0 : (N == 128 || (N == 256 && kind == ZF_RA)) ? CDN_ZF_ROWS : 0;
+}
+template <int N, int KIND> struct ZRows {
+  static constexpr int R = zf_rows(N, KIND);
 };
 
 // filter_kernel / filterConj_kernel (src/RLgpuImpl.cu:331-350) with the OTF
 // value computed on the fly as in dev_otfinterpolate (src/RLgpuImpl.cu:228-329)
-template <int N>
-__global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(zf_waves(N))))
+template <int N, int KIND = ZF_ANY>
+__global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(zf_waves(N, KIND))))
     k_z_filter(float2* __restrict__ S, long ncols, long line_stride, long outer_stride, long ntiles, long ny,
                long stack_stride, int conj, OtfParams otf, const float2* __restrict__ tw) {
   using C = CCfg<N>;
-  constexpr int RS = ZRows<N>::R;
+  constexpr int RS = ZRows<N, KIND>::R;
   constexpr int PER = RS > 0 ? (RS * N + C::THREADS - 1) / C::THREADS : 1;
+  constexpr bool ZRA = KIND == ZF_RA;
   __shared__ float2 lds[C::LDS];
   __shared__ float2 otf_rows[RS > 0 ? RS * N : 1];
   const long lb = xcd_logical_block<C::L>(blockIdx.x, gridDim.x);
@@ -706,11 +727,13 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
 
 // The z filter of every stack whose z length is N, stacks of different shapes
 // and OTFs in one launch (ZSeg per stack); items and PERSIST as k_col_ragged.
-template <int N, bool PERSIST>
-__global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(zf_waves(N))))
+// KIND as k_z_filter's (ZF_RA: every segment's table is rotationally
+// averaged), so that each stack runs the code its one-stack engine runs.
+template <int N, bool PERSIST, int KIND = ZF_ANY>
+__global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(zf_waves(N, KIND))))
     k_z_ragged(const ZSeg* __restrict__ segs, int nseg, long total, int conj, const float2* __restrict__ tw) {
   using C = CCfg<N>;
-  constexpr int RS = ZRows<N>::R;
+  constexpr int RS = ZRows<N, KIND>::R;
   constexpr int PER = RS > 0 ? (RS * N + C::THREADS - 1) / C::THREADS : 1;
   __shared__ float2 lds[C::LDS];
   __shared__ float2 otf_rows[RS > 0 ? RS * N : 1];
@@ -722,6 +745,7 @@ __global__ void __launch_bounds__(CCfg<N>::THREADS) __attribute__((amdgpu_waves_
     const long ncols = g->ncols, line_stride = g->line_stride, outer_stride = g->outer_stride;
     const long ntiles = g->ntiles, ny = g->ny, stack_stride = 0, lb = gi - g->item0;
     const OtfParams& otf = g->otf;
+    constexpr bool ZRA = KIND == ZF_RA;
     {
 #include "zf_tile_body.inc"
     }

@@ -33,7 +33,9 @@ const char* const kExpr[JK_COUNT] = {"cdn::k_x_r2c<%d, false>",   "cdn::k_x_c2r<
                                      "cdn::k_z_filter<%d>",       "cdn::k_x_r2c<%d, true>",
                                      "cdn::k_col_ragged<%d, false, false>", "cdn::k_col_ragged<%d, true, false>",
                                      "cdn::k_z_ragged<%d, false>", "cdn::k_col_ragged<%d, false, true>",
-                                     "cdn::k_col_ragged<%d, true, true>", "cdn::k_z_ragged<%d, true>"};
+                                     "cdn::k_col_ragged<%d, true, true>", "cdn::k_z_ragged<%d, true>",
+                                     "cdn::k_z_filter<%d, 1>",    "cdn::k_z_ragged<%d, false, 1>",
+                                     "cdn::k_z_ragged<%d, true, 1>"};
 const char* const kOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
                              "-Xclang", "-packed-fp32-ops"};
 
@@ -119,6 +121,7 @@ Compiled compile(int n, const std::string& defines) {
   const std::string src = "#include \"fft_kernels_dev.hpp\"\n";
   std::string key = src + std::to_string(n) + defines;
   for (const char* o : kOpts) key += o;
+  for (const char* e : kExpr) key += e;
   for (int i = 0; i < kJitHeaderCount; ++i) key += kJitHeaderText[i];
   char hex[32];
   std::snprintf(hex, sizeof hex, "%016llx", (unsigned long long)fnv1a(key));
@@ -178,6 +181,7 @@ std::string FftVariant::defines(int n) const {
   if (xcd_remap >= 0) d += " -DCDN_XCD_REMAP=" + std::to_string(xcd_remap);
   if (col_tpt > 0) d += " -DCDN_CTPT_OVERRIDE=" + std::to_string(n) + "," + std::to_string(col_tpt);
   if (zf_waves > 0) d += " -DCDN_ZF_WAVES=" + std::to_string(zf_waves);
+  if (zf_rows != 0) d += " -DCDN_ZF_ROWS_OVERRIDE=" + std::to_string(n) + "," + std::to_string(zf_rows);
   return d;
 }
 

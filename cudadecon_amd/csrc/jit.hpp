@@ -24,6 +24,9 @@ enum JitKernel {
   JK_COL_RAGGED_FP,
   JK_COL_RAGGED_IP,
   JK_Z_RAGGED_P,
+  JK_Z_FILTER_RA,
+  JK_Z_RAGGED_RA,
+  JK_Z_RAGGED_P_RA,
   JK_COUNT
 };
 
@@ -31,8 +34,9 @@ enum JitKernel {
 // rule): columns per y/z tile, threads per x-pass workgroup, non-temporal
 // loads in the column FFT, the column passes' workgroup-to-XCD renumbering
 // (0 none, 1 tiles narrower than a line, 2 every tile), threads per column
-// line, the z filter's minimum waves per SIMD (a register budget).  Picked
-// by the autotuner (tune.cpp).
+// line, the z filter's minimum waves per SIMD (a register budget), OTF table
+// rows the z filter stages in LDS per tile (-1: none).  Picked by the
+// autotuner (tune.cpp).
 struct FftVariant {
   int cols = 0;
   int xthreads = 0;
@@ -40,14 +44,16 @@ struct FftVariant {
   int xcd_remap = -1;
   int col_tpt = 0;
   int zf_waves = 0;
+  int zf_rows = 0;
   bool is_default() const {
-    return cols == 0 && xthreads == 0 && col_nt_load < 0 && xcd_remap < 0 && col_tpt == 0 && zf_waves == 0;
+    return cols == 0 && xthreads == 0 && col_nt_load < 0 && xcd_remap < 0 && col_tpt == 0 && zf_waves == 0 &&
+           zf_rows == 0;
   }
   // -D options that build this variant for length n
   std::string defines(int n) const;
   bool operator==(const FftVariant& o) const {
     return cols == o.cols && xthreads == o.xthreads && col_nt_load == o.col_nt_load && xcd_remap == o.xcd_remap &&
-           col_tpt == o.col_tpt && zf_waves == o.zf_waves;
+           col_tpt == o.col_tpt && zf_waves == o.zf_waves && zf_rows == o.zf_rows;
   }
 };
 

@@ -12,7 +12,8 @@
 //              with the built-in or the other workgroup-to-XCD renumbering
 //              (consecutive tiles of one line on one XCD share its L2 and its
 //              translation caches); then, for a z length, the chosen shape's
-//              z filter built for 6 waves per SIMD;
+//              z filter built for 6 waves per SIMD and with 4 or 6 OTF table
+//              rows staged in LDS per tile;
 // and installs the fastest combination (fft_set_variant).  Column variants
 // only change how lines are grouped into workgroups and threads, so their
 // results are bitwise those of the defaults; an x-pass variant also regroups
@@ -50,7 +51,7 @@ long vox(const std::array<int, 3>& g) { return (long)g[0] * g[1] * g[2]; }
 // TPT threads per line (half when the exchange goes in halves)
 long col_lds_bytes(int n, int L, int tpt) {
   const bool half = CDN_SPLIT_LDS != 0 && plan_exact(n, tpt);
-  return (long)n * L * (half ? 4 : 8) + (n == 128 ? 8L * CDN_ZF_ROWS * n : 0L);  // + the z filter's OTF rows
+  return (long)n * L * (half ? 4 : 8) + 8L * zf_rows(n, ZF_RA) * n;  // + the z filter's staged OTF rows
 }
 
 std::vector<FftVariant> col_candidates(int n) {
@@ -100,9 +101,13 @@ struct Bench {
   void setup(const std::array<int, 3>& g, hipStream_t s) {
     st = s;
     F.setup(g[2], g[1], g[0], st);
-    const float tab[4] = {1.f, 0.f, 1.f, 0.f};  // 2 x 1 RA table of ones
-    T.set_from_image(tab, 2, 2, 1, 1.f, 1.f);
-    const float ks[3] = {0.5f, 0.5f, 0.5f};
+    // a rotationally averaged table of 33 radii x 17 kz samples, scaled so
+    // that its support ends at 70% of the x and y Nyquist frequencies: the
+    // z filter interpolates in about 40% of the columns, as with a real OTF
+    const int nr = 33, nzo = 17;
+    const std::vector<float> tab((size_t)2 * nzo * nr, 0.5f);
+    T.set_from_image(tab.data(), 2 * nzo, nr, 1, 1.f, 1.f);
+    const float ks[3] = {(nr - 1) / (0.35f * g[2]), (nr - 1) / (0.35f * g[1]), (nzo - 1) / (0.35f * g[0])};
     D.setup(T, ks, g[0], st);
     vol.ensure(sizeof(float) * F.nvox);
     aux.ensure(sizeof(float) * F.nvox);
@@ -156,7 +161,7 @@ std::string vstr(const FftVariant& v) {
   std::ostringstream o;
   o << "{\"cols\": " << v.cols << ", \"xthreads\": " << v.xthreads << ", \"col_nt_load\": " << v.col_nt_load
     << ", \"xcd_remap\": " << v.xcd_remap << ", \"col_tpt\": " << v.col_tpt << ", \"zf_waves\": " << v.zf_waves
-    << "}";
+    << ", \"zf_rows\": " << v.zf_rows << "}";
   return o.str();
 }
 
@@ -175,7 +180,8 @@ std::map<std::string, FftVariant> load_cache(const std::string& path) {
     int n;
     std::string role, grid;
     FftVariant v;
-    if (is >> n >> role >> grid >> v.cols >> v.xthreads >> v.col_nt_load >> v.xcd_remap >> v.col_tpt >> v.zf_waves)
+    if (is >> n >> role >> grid >> v.cols >> v.xthreads >> v.col_nt_load >> v.xcd_remap >> v.col_tpt >> v.zf_waves >>
+        v.zf_rows)
       m[std::to_string(n) + ' ' + role + ' ' + grid] = v;  // the layout gkey writes
   }
   return m;
@@ -185,7 +191,28 @@ void append_cache(const std::string& path, const std::string& key, const FftVari
   std::ofstream f(path, std::ios::app);
   if (f)
     f << key << ' ' << v.cols << ' ' << v.xthreads << ' ' << v.col_nt_load << ' ' << v.xcd_remap << ' ' << v.col_tpt
-      << ' ' << v.zf_waves << '\n';
+      << ' ' << v.zf_waves << ' ' << v.zf_rows << '\n';
+}
+
+// compile (hipRTC, on several threads) every variant that is not built in
+void prepare_parallel(const std::vector<std::pair<int, FftVariant>>& todo) {
+  std::vector<std::pair<int, FftVariant>> jit;
+  for (auto& t : todo)
+    if (!t.second.is_default() || !fft_length_precompiled(t.first)) jit.push_back(t);
+  const unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  std::vector<std::string> errs(nth);
+  for (unsigned w = 0; w < nth; ++w)
+    pool.emplace_back([&, w]() {
+      try {
+        for (size_t i = w; i < jit.size(); i += nth) jit_prepare(jit[i].first, jit[i].second);
+      } catch (const std::exception& e) {
+        errs[w] = e.what();
+      }
+    });
+  for (auto& t : pool) t.join();
+  for (auto& e : errs)
+    if (!e.empty()) fail("autotune: " + e);
 }
 
 }  // namespace
@@ -220,25 +247,7 @@ std::string autotune(const std::vector<std::array<int, 3>>& grids, int reps, boo
     if (r.y || r.z)
       for (const FftVariant& v : col_candidates(n)) todo.push_back({n, v});
   }
-  {
-    std::vector<std::pair<int, FftVariant>> jit;
-    for (auto& t : todo)
-      if (!t.second.is_default() || !fft_length_precompiled(t.first)) jit.push_back(t);
-    const unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    std::vector<std::string> errs(nth);
-    for (unsigned w = 0; w < nth; ++w)
-      pool.emplace_back([&, w]() {
-        try {
-          for (size_t i = w; i < jit.size(); i += nth) jit_prepare(jit[i].first, jit[i].second);
-        } catch (const std::exception& e) {
-          errs[w] = e.what();
-        }
-      });
-    for (auto& t : pool) t.join();
-    for (auto& e : errs)
-      if (!e.empty()) fail("autotune: " + e);
-  }
+  prepare_parallel(todo);
 
   hipStream_t st;
   CDN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -310,18 +319,40 @@ std::string autotune(const std::vector<std::array<int, 3>>& grids, int reps, boo
             }
           }
           rep << "]";
-          // then the z filter of the chosen shape at 6 waves per SIMD (a
-          // tighter register budget, 3 workgroups of 512 threads per CU)
-          if (r.z && bc.zf_waves == 0 && zf_waves(n) < 6) {
-            FftVariant w = bc;
-            w.zf_waves = 6;
+          // then the z filter of the chosen shape with a tighter register
+          // budget (6 waves per SIMD: 3 workgroups of 512 threads per CU)
+          // and with 4 or 6 OTF table rows staged in LDS per tile
+          if (r.z) {
+            const CShape cz = c_shape(n, bc.cols > 0 ? bc.cols : -1, bc.col_tpt > 0 ? bc.col_tpt : -1);
+            std::vector<std::pair<int, FftVariant>> zc;
+            if (bc.zf_waves == 0 && zf_waves(n, ZF_RA) < 6 && cz.THREADS == 512 && cz.E == 16) {
+              FftVariant w = bc;
+              w.zf_waves = 6;
+              zc.push_back({n, w});
+            }
+            if (bc.zf_rows == 0 && zf_rows(n, ZF_RA) == 0)
+              for (int rows : {4, 6}) {
+                if (col_lds_bytes(n, cz.L, cz.TPT) + 8L * rows * n > 163840) continue;
+                FftVariant w = bc;
+                w.zf_rows = rows;
+                zc.push_back({n, w});
+              }
+            prepare_parallel(zc);
             fft_set_variant(n, bc);
-            const double t0 = time_role(bench(r.gz), 2, reps);
-            jit_prepare(n, w);
-            fft_set_variant(n, w);
-            const double t1 = time_role(bench(r.gz), 2, reps);
-            rep << ", \"z_waves\": [[0, " << t0 << "], [6, " << t1 << "]]";
-            if (t1 < t0 * 0.99) bc = w;
+            double tb = time_role(bench(r.gz), 2, reps);
+            rep << ", \"z_filter\": [[" << vstr(bc) << ", " << tb << "]";
+            FftVariant b2 = bc;
+            for (auto& [len, w] : zc) {
+              fft_set_variant(n, w);
+              const double t = time_role(bench(r.gz), 2, reps);
+              rep << ", [" << vstr(w) << ", " << t << "]";
+              if (t < tb * 0.99) {
+                tb = t;
+                b2 = w;
+              }
+            }
+            rep << "]";
+            bc = b2;
           }
           rep << "}";
           if (!cpath.empty()) append_cache(cpath, key, bc);
@@ -331,6 +362,7 @@ std::string autotune(const std::vector<std::array<int, 3>>& grids, int reps, boo
         best.xcd_remap = bc.xcd_remap;
         best.col_tpt = bc.col_tpt;
         best.zf_waves = bc.zf_waves;
+        best.zf_rows = bc.zf_rows;
       }
       fft_set_variant(n, best);
       if (!best.is_default()) jit_prepare(n, best);

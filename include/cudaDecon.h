@@ -152,7 +152,8 @@ int cudaDecon_autotune_clear(void);
  * built-in) go to fields: columns per column tile, threads per x-pass
  * workgroup, non-temporal column loads, workgroup-to-XCD renumbering of the
  * column passes (0 none, 1 tiles narrower than a cache line, 2 every tile),
- * threads per column line, the z filter's minimum waves per SIMD. */
+ * threads per column line, the z filter's minimum waves per SIMD, OTF table
+ * rows the z filter stages in LDS per tile (-1 none). */
 int cudaDecon_fft_variant(int n, int* fields, int nfields);
 
 /* RL-Biggs-Andrews engine on HBM-resident state (the iteration of

@@ -55,12 +55,12 @@ def _force(cdn, tmp_path, lines):
 def test_forced_column_variants_are_bitwise(cdn, case, tmp_path):
     R, tab, X0, lam0 = case
     g = "x".join(map(str, SHAPE))
-    rep = _force(cdn, tmp_path, [f"192 x {g} 0 0 -1 -1 0 0", f"96 y {g} 8 0 1 0 0 0", f"32 z {g} 128 0 1 2 1 2"])
+    rep = _force(cdn, tmp_path, [f"192 x {g} 0 0 -1 -1 0 0 0", f"96 y {g} 8 0 1 0 0 0 0", f"32 z {g} 128 0 1 2 1 2 4"])
     print(rep, flush=True)
-    assert cdn.fft_variant(96) == dict(cols=8, xthreads=0, col_nt_load=1, xcd_remap=0, col_tpt=0, zf_waves=0)
+    assert cdn.fft_variant(96) == dict(cols=8, xthreads=0, col_nt_load=1, xcd_remap=0, col_tpt=0, zf_waves=0, zf_rows=0)
     # z length 32: one thread per line, 128 columns per tile (all 97 in one),
-    # z filter built for 2 waves per SIMD
-    assert cdn.fft_variant(32) == dict(cols=128, xthreads=0, col_nt_load=1, xcd_remap=2, col_tpt=1, zf_waves=2)
+    # z filter built for 2 waves per SIMD, 4 OTF rows staged in LDS
+    assert cdn.fft_variant(32) == dict(cols=128, xthreads=0, col_nt_load=1, xcd_remap=2, col_tpt=1, zf_waves=2, zf_rows=4)
     assert cdn.fft_variant(192) is None
     X1, lam1 = _rl(cdn, R, tab)
     assert np.array_equal(X1, X0)
@@ -71,8 +71,8 @@ def test_forced_column_variants_are_bitwise(cdn, case, tmp_path):
 def test_forced_x_variant_agrees_to_rounding(cdn, case, tmp_path):
     R, tab, X0, lam0 = case
     g = "x".join(map(str, SHAPE))
-    _force(cdn, tmp_path, [f"192 x {g} 0 256 -1 -1 0 0", f"96 y {g} 0 0 -1 -1 0 0", f"32 z {g} 0 0 -1 -1 0 0"])
-    assert cdn.fft_variant(192) == dict(cols=0, xthreads=256, col_nt_load=-1, xcd_remap=-1, col_tpt=0, zf_waves=0)
+    _force(cdn, tmp_path, [f"192 x {g} 0 256 -1 -1 0 0 0", f"96 y {g} 0 0 -1 -1 0 0 0", f"32 z {g} 0 0 -1 -1 0 0 0"])
+    assert cdn.fft_variant(192) == dict(cols=0, xthreads=256, col_nt_load=-1, xcd_remap=-1, col_tpt=0, zf_waves=0, zf_rows=0)
     X1, lam1 = _rl(cdn, R, tab)
     assert np.abs(X1 - X0).max() <= 1e-5 * np.abs(X0).max()
     assert np.abs(lam1 - lam0).max() <= 1e-9
