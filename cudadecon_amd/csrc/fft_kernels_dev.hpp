@@ -129,9 +129,11 @@ enum XKind : int { XK_PLAIN = 0, XK_LR = 1, XK_R2C = 2 };
 // from 0.742 to 0.712 ms at 256 x 1024 x 1024 (the same shape made LRcore
 // slower, 0.60 -> 0.76 ms, and the update no faster, so they keep theirs);
 // 512 with 128 from 0.098-0.100 to 0.094-0.096 ms at 128 x 512 x 512 (three
-// alternating runs each; 64 threads: 0.098).  2048-point lines with 256 or
-// 512 threads measured like the 128 they keep (5.4-6.2 ms either way at
-// 512 x 2048 x 2048).
+// alternating runs each; 64 threads: 0.098).  2048-point lines keep the 128
+// of the other passes: 256 or 512 threads measured the same (5.4-6.2 ms
+// either way at 512 x 2048 x 2048), and 64 threads (32 elements each) gave
+// 4.77-5.16 ms against 4.85-5.17 in alternating processes, where the x
+// passes there vary by 7% between processes of one build.
 #ifndef CDN_R2C_TPT
 #define CDN_R2C_TPT 512, 128, 1024, 256
 #endif

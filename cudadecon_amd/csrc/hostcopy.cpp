@@ -5,7 +5,6 @@
 #include "copypool.hpp"
 
 #include <algorithm>
-#include <map>
 #include <mutex>
 
 namespace cdn {
@@ -17,34 +16,43 @@ constexpr size_t kDirect = 16UL << 20;  // smaller copies go straight through th
 struct Stage {
   std::mutex mu;
   CopyPool* pool = nullptr;
-  void* buf[2] = {nullptr, nullptr};      // pinned, portable across devices
-  std::map<int, hipEvent_t> ev[2];        // last DMA through buf[i], per device
-  // the event for buf[i] on the device of stream st (created there)
-  hipEvent_t event(int i, hipStream_t st) {
-    int dev = 0;
-    CDN_HIP(hipStreamGetDevice(st, &dev));
-    auto it = ev[i].find(dev);
-    if (it != ev[i].end()) return it->second;
-    int cur = 0;
-    CDN_HIP(hipGetDevice(&cur));
-    if (cur != dev) CDN_HIP(hipSetDevice(dev));
-    hipEvent_t e = nullptr;
-    const hipError_t rc = hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    if (cur != dev) CDN_HIP(hipSetDevice(cur));
-    CDN_HIP(rc);
-    ev[i][dev] = e;
-    return e;
-  }
+  void* buf[2] = {nullptr, nullptr};  // pinned, portable across devices
   void ensure() {
     for (int i = 0; i < 2; ++i)
       if (!buf[i]) CDN_HIP(hipHostMalloc(&buf[i], kChunk, hipHostMallocPortable));
     if (!pool) pool = new CopyPool;
   }
-  // every DMA through the staging buffers, on any device, has finished
-  void drain() {
-    for (int i = 0; i < 2; ++i)
-      for (auto& kv : ev[i]) CDN_HIP(hipEventSynchronize(kv.second));
+};
+
+// The two events of one call (the last DMA through buf[0] and buf[1]),
+// created on the device of the call's stream and destroyed when the call
+// returns, after every DMA through the staging buffers has finished.  No
+// event outlives its call, so none is ever waited on after the stream it
+// was recorded on has been destroyed (the caller may destroy it right after
+// the call).
+struct CallEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  explicit CallEvents(hipStream_t st) {
+    int dev = 0, cur = 0;
+    CDN_HIP(hipStreamGetDevice(st, &dev));
+    CDN_HIP(hipGetDevice(&cur));
+    if (cur != dev) CDN_HIP(hipSetDevice(dev));
+    hipError_t rc = hipSuccess;
+    for (int i = 0; i < 2 && rc == hipSuccess; ++i) rc = hipEventCreateWithFlags(&e[i], hipEventDisableTiming);
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (rc != hipSuccess) {
+      release();
+      CDN_HIP(rc);
+    }
   }
+  void release() {
+    for (hipEvent_t& x : e)
+      if (x) {
+        (void)hipEventDestroy(x);
+        x = nullptr;
+      }
+  }
+  ~CallEvents() { release(); }
 };
 
 // process lifetime: the pinned buffers are released when the process ends
@@ -63,17 +71,18 @@ void copy_to_device(hipStream_t st, void* dev, const void* host, size_t bytes) {
   Stage& S = stage();
   std::lock_guard<std::mutex> lock(S.mu);
   S.ensure();
-  S.drain();
+  CallEvents E(st);
   const size_t n = (bytes + kChunk - 1) / kChunk;
   for (size_t i = 0; i < n; ++i) {
     const int b = (int)(i & 1);
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
-    hipEvent_t e = S.event(b, st);
-    if (i >= 2) CDN_HIP(hipEventSynchronize(e));  // the DMA of chunk i - 2 left buf[b]
+    if (i >= 2) CDN_HIP(hipEventSynchronize(E.e[b]));  // the DMA of chunk i - 2 left buf[b]
     S.pool->copy(S.buf[b], static_cast<const char*>(host) + off, len);
     CDN_HIP(hipMemcpyAsync(static_cast<char*>(dev) + off, S.buf[b], len, hipMemcpyHostToDevice, st));
-    CDN_HIP(hipEventRecord(e, st));
+    CDN_HIP(hipEventRecord(E.e[b], st));
   }
+  // the staging buffers are free for the next call
+  for (size_t b = 0; b < 2 && b < n; ++b) CDN_HIP(hipEventSynchronize(E.e[b]));
 }
 
 void copy_to_host(hipStream_t st, void* host, const void* dev, size_t bytes) {
@@ -85,13 +94,13 @@ void copy_to_host(hipStream_t st, void* host, const void* dev, size_t bytes) {
   Stage& S = stage();
   std::lock_guard<std::mutex> lock(S.mu);
   S.ensure();
-  S.drain();
+  CallEvents E(st);
   const size_t n = (bytes + kChunk - 1) / kChunk;
   auto issue = [&](size_t i) {
     const int b = (int)(i & 1);
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
     CDN_HIP(hipMemcpyAsync(S.buf[b], static_cast<const char*>(dev) + off, len, hipMemcpyDeviceToHost, st));
-    CDN_HIP(hipEventRecord(S.event(b, st), st));
+    CDN_HIP(hipEventRecord(E.e[b], st));
   };
   issue(0);
   for (size_t i = 0; i < n; ++i) {
@@ -99,7 +108,7 @@ void copy_to_host(hipStream_t st, void* host, const void* dev, size_t bytes) {
     if (i + 1 < n) issue(i + 1);
     const int b = (int)(i & 1);
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
-    CDN_HIP(hipEventSynchronize(S.event(b, st)));
+    CDN_HIP(hipEventSynchronize(E.e[b]));
     S.pool->copy(static_cast<char*>(host) + off, S.buf[b], len);
   }
 }

@@ -13,8 +13,8 @@
 
 namespace cdn {
 
-// host -> device.  Returns once the host buffer has been read; the last DMAs
-// may still be in flight, ordered on `st` before any later work there.
+// host -> device, after the work already queued on `st`.  Returns once the
+// data is on the device.
 void copy_to_device(hipStream_t st, void* dev, const void* host, size_t bytes);
 
 // device -> host, after the work already queued on `st`.  Returns once the

@@ -37,14 +37,14 @@ def case(cdn):
     return R, tab, X0, lam0
 
 
-def _force(cdn, tmp_path, lines):
+def _force(cdn, tmp_path, lines, shape=SHAPE):
     d = tmp_path / "cache"
     (d / "cudadecon_amd").mkdir(parents=True)
     (d / "cudadecon_amd" / "tune_gfx950_v2.txt").write_text("".join(l + "\n" for l in lines))
     old = os.environ.get("XDG_CACHE_HOME")
     os.environ["XDG_CACHE_HOME"] = str(d)
     try:
-        return cdn.autotune([SHAPE], reps=1, use_cache=True)
+        return cdn.autotune([shape], reps=1, use_cache=True)
     finally:
         if old is None:
             del os.environ["XDG_CACHE_HOME"]
@@ -63,6 +63,36 @@ def test_forced_column_variants_are_bitwise(cdn, case, tmp_path):
     assert cdn.fft_variant(32) == dict(cols=128, xthreads=0, col_nt_load=1, xcd_remap=2, col_tpt=1, zf_waves=2, zf_rows=4)
     assert cdn.fft_variant(192) is None
     X1, lam1 = _rl(cdn, R, tab)
+    assert np.array_equal(X1, X0)
+    assert np.array_equal(lam1, lam0)
+    cdn.autotune_clear()
+
+
+# the wide column tiles the tuner picks at 512 x 2048 x 2048, on small grids:
+# (nz, ny, nx), the variant's cache line, the length it applies to
+WIDE = [((512, 32, 64), "512 z {g} 32 0 -1 -1 16 0 0", 512),
+        ((32, 2048, 64), "2048 y {g} 16 0 -1 2 64 0 0", 2048)]
+
+
+@pytest.mark.parametrize("shape,line,n", WIDE)
+def test_forced_wide_tiles_are_bitwise(cdn, tmp_path, shape, line, n):
+    """Twice the columns per tile at half the threads per line (32 elements
+    per thread): the 512-point z filter (rotationally averaged OTF kind) and
+    the 2048-point y transforms give the built-in shapes' results bit for
+    bit."""
+    from cudadecon_amd.otf import make_otf
+    from cudadecon_amd.synth import gaussian_psf
+    R = np.random.default_rng(4).uniform(0, 200, shape).astype(np.float32)
+    tab = make_otf(gaussian_psf(1.5, 3.0))
+    cdn.autotune_clear()
+    X0, lam0 = _rl(cdn, R, tab, n=4)
+    g = "x".join(map(str, shape))
+    roles = {shape[2]: "x", shape[1]: "y", shape[0]: "z"}
+    lines = [line.format(g=g)] + [f"{m} {r} {g} 0 0 -1 -1 0 0 0" for m, r in roles.items() if m != n]
+    _force(cdn, tmp_path, lines, shape)
+    v = cdn.fft_variant(n)
+    assert v is not None and v["col_tpt"] > 0 and v["cols"] > 0, v
+    X1, lam1 = _rl(cdn, R, tab, n=4)
     assert np.array_equal(X1, X0)
     assert np.array_equal(lam1, lam0)
     cdn.autotune_clear()
