@@ -15,13 +15,15 @@ namespace cdn {
 // caller owns (bind), which is never freed here and never grows.
 struct DevBuf {
   void* p = nullptr;
+  void* base = nullptr;  // the allocation (p may start a stagger offset into it)
   size_t bytes = 0;
   bool owned = true;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), base(o.base), bytes(o.bytes), owned(o.owned) {
     o.p = nullptr;
+    o.base = nullptr;
     o.bytes = 0;
     o.owned = true;
   }
@@ -29,9 +31,11 @@ struct DevBuf {
     if (this != &o) {
       release();
       p = o.p;
+      base = o.base;
       bytes = o.bytes;
       owned = o.owned;
       o.p = nullptr;
+      o.base = nullptr;
       o.bytes = 0;
       o.owned = true;
     }

@@ -258,6 +258,7 @@ void WarmNet::add_samples(const float* raw, const float* tgt, long count, int nz
       CDN_HIP(hipStreamSynchronize(st));
       b->release();
       b->p = np;
+      b->base = np;
       b->bytes = cap;
     }
   }
