@@ -3,8 +3,8 @@
 // Every length n of the given grids has up to three roles: x passes (n = nx),
 // y column transforms (n = ny) and the z filter (n = nz).  For each length
 // the tuner times, on the largest grid that uses it in each role:
-//   x passes   x-R2C + x-update with 64, 128 (default) or 256 threads per
-//              workgroup;
+//   x passes   x-R2C + LRcore + x-update with 64, 128 (default) or 256
+//              threads per workgroup;
 //   columns    the column FFT (y) and / or the z filter with half, the
 //              default and twice the default columns per tile (within 1024
 //              threads and 160 KB of LDS), and twice the columns at half the
@@ -30,6 +30,7 @@
 #include "rl_kernels.hpp"
 
 #include <algorithm>
+#include <cfloat>
 #include <cstdio>
 #include <fstream>
 #include <map>
@@ -123,6 +124,7 @@ struct Bench {
     if (role == 0) {
       launch_x_r2c(F.nx, st, vol.as<float>(), nullptr, nullptr, nullptr, XIN_PLAIN, S, rows, F.pitch,
                    F.twx.as<float2>());
+      launch_x_lrcore(F.nx, st, S, vol.as<float>(), rows, F.pitch, F.twx.as<float2>(), FLT_EPSILON);
       launch_x_update(F.nx, st, S, vol.as<float>(), aux.as<float>(), aux.as<float>(), part.as<double>(), 1, rows,
                       F.pitch, F.twx.as<float2>());
     } else if (role == 1) {
