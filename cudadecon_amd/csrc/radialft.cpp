@@ -1,25 +1,31 @@
-// libradialft.so: makeOTF, OTF generation from a measured PSF stack
-// (src/radialft_interface.cpp:95-518), on the host.
+// libradialft.so: makeOTF, the OTF of a measured PSF stack, computed on the
+// host (the steps of src/radialft_interface.cpp:95-518, restated here).
 //
-// Steps, each following the reference function named:
-//   determine_center_and_background (:199-255)  peak by parabolic fits, z + 0.6,
-//                                               background from rows y < yc - 20
-//   R2C of PSF - background                     own mixed-radix FFT in double
-//                                               (any length; FFTW FP32 in the
-//                                               reference), stored as complex<float>
-//   shift_center (:286-319)                     phase ramp moving the bead to the origin
-//   radialft (:321-381)                         average over rint(kr) bins, kz Hermitian
-//   cleanup (:383-411)                          zero outside the OTF support (optional)
-//   fixorigin (:413-443, 3-D :445-486)          kr = 0 by linear extrapolation
-//   rescale (:488-518)                          max |OTF| = 1
+//   locate_bead          (:199-255)  brightest voxel refined by a parabola
+//                                    along each axis, +0.6 voxel in z; the
+//                                    background is the mean of the in-focus
+//                                    plane's rows more than 20 above the bead
+//   half_spectrum                    R2C of PSF - background by a mixed-radix
+//                                    DFT in double (any length; FFTW FP32 in
+//                                    the reference), kept as complex<float>
+//   move_to_origin       (:286-319)  phase ramp that puts the bead at (0,0,0)
+//   ring_average         (:321-381)  mean over rings of rint(kr) per kz, then
+//                                    Hermitian along kz
+//   mask_support         (:383-411)  zero outside the objective's support
+//   extrapolate_origin   (:413-443)  kr = 0 from a straight-line fit over
+//                                    kr = 1..n (3-D form :445-486: the two
+//                                    diagonals of every kz plane)
+//   normalize            (:488-518)  max |OTF| = 1
+//
+// Each rounding point (float or double) is the reference's, so that OTF
+// files agree with its output to FP32 precision.
 #include "../../include/radialft.h"
 
 #include "tiff.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
-#include <cstdio>
-#include <algorithm>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -29,228 +35,280 @@ namespace {
 
 thread_local std::string g_err;
 
-typedef std::complex<double> cd;
-typedef std::complex<float> cf;
+using zd = std::complex<double>;
+using zf = std::complex<float>;
 
-// e^{-2 pi i t / n}, t < n, one table per length
-const std::vector<cd>& twiddles(int n) {
-  static std::map<int, std::vector<cd>> cache;
-  auto it = cache.find(n);
-  if (it != cache.end()) return it->second;
-  std::vector<cd> w((size_t)n);
-  for (int t = 0; t < n; ++t) {
-    const double a = -2.0 * M_PI * (double)t / (double)n;
-    w[(size_t)t] = cd(std::cos(a), std::sin(a));
-  }
-  return cache.emplace(n, std::move(w)).first->second;
-}
+[[noreturn]] void refuse(const std::string& why) { throw std::runtime_error(why); }
 
-// In-place forward DFT of any length: decimation in time over the smallest
-// prime factor p, X[k + m q] = sum_r W_n^{r (k + m q)} Y_r[k] (m = n / p).
-void fft_line(cd* x, int n) {
-  if (n <= 1) return;
-  int p = 2;
-  while ((long)p * p <= n && n % p) ++p;
-  if (n % p) p = n;
-  const int m = n / p;
-  std::vector<cd> sub((size_t)n);
-  for (int r = 0; r < p; ++r)
-    for (int j = 0; j < m; ++j) sub[(size_t)r * m + j] = x[(size_t)j * p + r];
-  for (int r = 0; r < p; ++r) fft_line(&sub[(size_t)r * m], m);
-  const std::vector<cd>& w = twiddles(n);
-  for (int q = 0; q < p; ++q)
-    for (int k = 0; k < m; ++k) {
-      const long kk = k + (long)m * q;
-      cd s = sub[(size_t)k];
-      for (int r = 1; r < p; ++r) s += sub[(size_t)r * m + k] * w[(size_t)((r * kk) % n)];
-      x[kk] = s;
+// index i of an n-point transform as a signed frequency
+int signed_freq(int i, int n) { return i > n / 2 ? i - n : i; }
+
+// ---------------------------------------------------------------------------
+// DFT of any length in double: split by the smallest prime factor p of n
+// into p interleaved subsequences of length n / p, transform those, and
+// combine X[k + q m] = sum_r w^{r (k + q m)} Y_r[k], w = e^{-2 pi i / n}.
+// ---------------------------------------------------------------------------
+class Dft {
+ public:
+  static void forward(zd* a, int n) {
+    if (n < 2) return;
+    const int p = smallest_factor(n), m = n / p;
+    std::vector<zd> parts((size_t)n);
+    for (int j = 0; j < n; ++j) parts[(size_t)(j % p) * m + j / p] = a[j];
+    for (int r = 0; r < p; ++r) forward(&parts[(size_t)r * m], m);
+    const std::vector<zd>& w = roots(n);
+    for (int k = 0; k < n; ++k) {
+      const int kk = k % m;
+      zd acc = parts[(size_t)kk];
+      for (int r = 1; r < p; ++r) acc += parts[(size_t)r * m + kk] * w[(size_t)(((long)r * k) % n)];
+      a[k] = acc;
     }
+  }
+
+ private:
+  static int smallest_factor(int n) {
+    for (int d = 2; (long)d * d <= n; ++d)
+      if (n % d == 0) return d;
+    return n;
+  }
+  static const std::vector<zd>& roots(int n) {
+    static std::map<int, std::vector<zd>> table;
+    auto it = table.find(n);
+    if (it != table.end()) return it->second;
+    std::vector<zd> w((size_t)n);
+    for (int t = 0; t < n; ++t) w[(size_t)t] = std::polar(1.0, -2.0 * M_PI * (double)t / (double)n);
+    return table.emplace(n, std::move(w)).first->second;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// The PSF and its bead
+// ---------------------------------------------------------------------------
+struct Psf {
+  int nx = 0, ny = 0, nz = 0;
+  std::vector<float> v;  // (nz, ny, nx), x fastest
+  float operator()(int z, int y, int x) const { return v[((size_t)z * ny + y) * nx + x]; }
+};
+
+struct Bead {
+  float x = 0, y = 0, z = 0;  // sub-voxel position
+  float background = 0;
+};
+
+// vertex of the parabola through (-1, lo), (0, mid), (1, hi): 0 when the
+// three are collinear or the vertex lies more than 1.5 voxels away
+float vertex(float lo, float mid, float hi) {
+  const float half_slope = 0.5 * (hi - lo);
+  const float bend = (hi + lo) - 2 * mid;
+  if (bend == 0) return 0.f;
+  const float t = -half_slope / bend;
+  return (t > 1.5 || t < -1.5) ? 0.f : t;
 }
 
-// R2C of a (nz, ny, nx) real volume: (nz, ny, nx/2+1) complex<float>
-std::vector<cf> rfft3(const std::vector<float>& v, int nx, int ny, int nz) {
-  std::vector<cd> a((size_t)nx * ny * nz);
-  for (size_t i = 0; i < a.size(); ++i) a[i] = cd(v[i], 0.0);
-  std::vector<cd> line;
-  for (long r = 0; r < (long)ny * nz; ++r) fft_line(&a[(size_t)r * nx], nx);
-  line.resize((size_t)ny);
+int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+Bead locate_bead(const Psf& p) {
+  // the first brightest voxel in z, y, x order
+  float top = 0.f;
+  int bz = 0, by = 0, bx = 0;
+  for (int z = 0; z < p.nz; ++z)
+    for (int y = 0; y < p.ny; ++y)
+      for (int x = 0; x < p.nx; ++x)
+        if (p(z, y, x) > top) {
+          top = p(z, y, x);
+          bz = z;
+          by = y;
+          bx = x;
+        }
+  Bead b;
+  b.z = bz + vertex(p(wrap(bz - 1, p.nz), by, bx), top, p(wrap(bz + 1, p.nz), by, bx));
+  b.z += 0.6;  // the reference's fixed axial offset
+  b.y = by + vertex(p(bz, wrap(by - 1, p.ny), bx), top, p(bz, wrap(by + 1, p.ny), bx));
+  b.x = bx + vertex(p(bz, by, wrap(bx - 1, p.nx)), top, p(bz, by, wrap(bx + 1, p.nx)));
+  const int plane = (int)std::floor(b.z);
+  double acc = 0;
+  for (int y = 0; y < b.y - 20; ++y)
+    for (int x = 0; x < p.nx; ++x) acc += p(plane, y, x);
+  b.background = acc / ((b.y - 20) * p.nx);
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// Half spectrum (nz, ny, nx/2+1)
+// ---------------------------------------------------------------------------
+struct Spectrum {
+  int nx = 0, nh = 0, ny = 0, nz = 0;
+  std::vector<zf> c;
+  zf& at(int z, int y, int x) { return c[((size_t)z * ny + y) * nh + x]; }
+  const zf& at(int z, int y, int x) const { return c[((size_t)z * ny + y) * nh + x]; }
+};
+
+Spectrum half_spectrum(const Psf& p, float background) {
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  std::vector<zd> w(p.v.size());
+  for (size_t i = 0; i < w.size(); ++i) w[i] = zd(p.v[i] - background, 0.0);
+  auto cell = [&](int z, int y, int x) -> zd& { return w[((size_t)z * ny + y) * nx + x]; };
+  for (long row = 0; row < (long)ny * nz; ++row) Dft::forward(&w[(size_t)row * nx], nx);
+  std::vector<zd> line((size_t)std::max(ny, nz));
   for (int z = 0; z < nz; ++z)
     for (int x = 0; x < nx; ++x) {
-      for (int y = 0; y < ny; ++y) line[(size_t)y] = a[((size_t)z * ny + y) * nx + x];
-      fft_line(line.data(), ny);
-      for (int y = 0; y < ny; ++y) a[((size_t)z * ny + y) * nx + x] = line[(size_t)y];
+      for (int y = 0; y < ny; ++y) line[(size_t)y] = cell(z, y, x);
+      Dft::forward(line.data(), ny);
+      for (int y = 0; y < ny; ++y) cell(z, y, x) = line[(size_t)y];
     }
-  line.resize((size_t)nz);
   for (int y = 0; y < ny; ++y)
     for (int x = 0; x < nx; ++x) {
-      for (int z = 0; z < nz; ++z) line[(size_t)z] = a[((size_t)z * ny + y) * nx + x];
-      fft_line(line.data(), nz);
-      for (int z = 0; z < nz; ++z) a[((size_t)z * ny + y) * nx + x] = line[(size_t)z];
+      for (int z = 0; z < nz; ++z) line[(size_t)z] = cell(z, y, x);
+      Dft::forward(line.data(), nz);
+      for (int z = 0; z < nz; ++z) cell(z, y, x) = line[(size_t)z];
     }
-  const int nh = nx / 2 + 1;
-  std::vector<cf> out((size_t)nh * ny * nz);
-  for (long r = 0; r < (long)ny * nz; ++r)
-    for (int x = 0; x < nh; ++x) out[(size_t)r * nh + x] = cf((float)a[(size_t)r * nx + x].real(),
-                                                                (float)a[(size_t)r * nx + x].imag());
-  return out;
-}
-
-float fitparabola(float a1, float a2, float a3) {
-  const float slope = 0.5 * (a3 - a1);
-  const float curve = (a3 + a1) - 2 * a2;
-  if (curve == 0) return 0.f;
-  const float peak = -slope / curve;
-  if (peak > 1.5 || peak < -1.5) return 0.f;
-  return peak;
-}
-
-void center_and_background(const std::vector<float>& s, int nx, int ny, int nz, float* xc, float* yc, float* zc,
-                           float* bg) {
-  float maxval = 0.f;
-  int mi = 0, mj = 0, mk = 0;
-  for (int k = 0; k < nz; ++k)
-    for (int i = 0; i < ny; ++i)
-      for (int j = 0; j < nx; ++j) {
-        const float v = s[((size_t)k * ny + i) * nx + j];
-        if (v > maxval) {
-          maxval = v;
-          mi = i;
-          mj = j;
-          mk = k;
-        }
+  Spectrum s;
+  s.nx = nx;
+  s.nh = nx / 2 + 1;
+  s.ny = ny;
+  s.nz = nz;
+  s.c.resize((size_t)s.nh * ny * nz);
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < s.nh; ++x) {
+        const zd& d = cell(z, y, x);
+        s.at(z, y, x) = zf((float)d.real(), (float)d.imag());
       }
-  auto at = [&](int k, int i, int j) { return s[((size_t)k * ny + i) * nx + j]; };
-  const int im = mi - 1 < 0 ? mi - 1 + ny : mi - 1, ip = mi + 1 >= ny ? mi + 1 - ny : mi + 1;
-  const int jm = mj - 1 < 0 ? mj - 1 + nx : mj - 1, jp = mj + 1 >= nx ? mj + 1 - nx : mj + 1;
-  const int km = mk - 1 < 0 ? mk - 1 + nz : mk - 1, kp = mk + 1 >= nz ? mk + 1 - nz : mk + 1;
-  *zc = mk + fitparabola(at(km, mi, mj), maxval, at(kp, mi, mj));
-  *zc += 0.6;
-  *yc = mi + fitparabola(at(mk, im, mj), maxval, at(mk, ip, mj));
-  *xc = mj + fitparabola(at(mk, mi, jm), maxval, at(mk, mi, jp));
-  double sum = 0;
-  const int infocus = (int)std::floor(*zc);
-  for (int i = 0; i < *yc - 20; ++i)
-    for (int j = 0; j < nx; ++j) sum += at(infocus, i, j);
-  *bg = sum / ((*yc - 20) * nx);
+  return s;
 }
 
-void shift_center(cf* bands, int nx, int ny, int nz, float xc, float yc, float zc) {
-  const int nh = nx / 2 + 1;
-  const float dphiz = 2 * M_PI * zc / nz, dphiy = 2 * M_PI * yc / ny, dphix = 2 * M_PI * xc / nx;
-  for (int kin = 0; kin < nz; ++kin) {
-    const int kz = kin > nz / 2 ? kin - nz : kin;
-    const float phi1 = dphiz * kz;
-    for (int iin = 0; iin < ny; ++iin) {
-      const int ky = iin > ny / 2 ? iin - ny : iin;
-      const float phi2 = dphiy * ky;
-      for (int kx = 0; kx < nh; ++kx) {
-        const float phi = phi1 + phi2 + dphix * kx;
-        cf& b = bands[((size_t)kin * ny + iin) * nh + kx];
-        b = b * cf(std::cos(phi), std::sin(phi));
+void move_to_origin(Spectrum& s, const Bead& b) {
+  const float wz = 2 * M_PI * b.z / s.nz, wy = 2 * M_PI * b.y / s.ny, wx = 2 * M_PI * b.x / s.nx;
+  for (int z = 0; z < s.nz; ++z) {
+    const float pz = wz * signed_freq(z, s.nz);
+    for (int y = 0; y < s.ny; ++y) {
+      const float py = wy * signed_freq(y, s.ny);
+      for (int x = 0; x < s.nh; ++x) {
+        const float ph = pz + py + wx * x;
+        s.at(z, y, x) *= zf(std::cos(ph), std::sin(ph));
       }
     }
   }
 }
 
-void radialft(const cf* band, int nx, int ny, int nz, cf* avg) {
-  const int nh = nx / 2 + 1, nxz = nh * nz;
-  std::vector<int> count((size_t)nxz, 0);
-  for (int kin = 0; kin < nz; ++kin)
-    for (int iin = 0; iin < ny; ++iin) {
-      const int ky = iin > ny / 2 ? iin - ny : iin;
-      for (int kx = 0; kx < nh; ++kx) {
-        const float rdist = std::sqrt((double)(kx * kx + ky * ky));
-        if (rdist < nh) {
-          const int out = (int)std::rint(rdist) * nz + kin;
-          if (out < nxz) {
-            avg[out] += band[((size_t)kin * ny + iin) * nh + kx];
-            count[(size_t)out]++;
-          }
-        }
+// ---------------------------------------------------------------------------
+// Rotationally averaged table (nr = nx/2+1 radii, nz), kz fastest
+// ---------------------------------------------------------------------------
+struct RadialOtf {
+  int nr = 0, nz = 0;
+  std::vector<zf> c;
+  zf& at(int r, int z) { return c[(size_t)r * nz + z]; }
+};
+
+RadialOtf ring_average(const Spectrum& s) {
+  RadialOtf o;
+  o.nr = s.nh;
+  o.nz = s.nz;
+  o.c.assign((size_t)o.nr * o.nz, zf(0.f, 0.f));
+  const int slots = o.nr * o.nz;
+  std::vector<int> hits((size_t)slots, 0);
+  for (int z = 0; z < s.nz; ++z)
+    for (int y = 0; y < s.ny; ++y) {
+      const int fy = signed_freq(y, s.ny);
+      for (int x = 0; x < s.nh; ++x) {
+        const float r = std::sqrt((double)(x * x + fy * fy));
+        if (!(r < o.nr)) continue;
+        const int slot = (int)std::rint(r) * o.nz + z;
+        if (slot >= slots) continue;
+        o.c[(size_t)slot] += s.at(z, y, x);
+        ++hits[(size_t)slot];
       }
     }
-  for (int i = 0; i < nxz; ++i)
-    if (count[(size_t)i] > 0) avg[i] /= (float)count[(size_t)i];
-  for (int kx = 0; kx < nh; ++kx) {
-    avg[kx * nz] = cf(avg[kx * nz].real(), 0.f);
-    for (int kz = 1; kz <= nz / 2; ++kz) {
-      const int i = kx * nz + kz, ic = kx * nz + (nz - kz);
-      avg[i] = (avg[i] + std::conj(avg[ic])) / 2.f;
-      avg[ic] = std::conj(avg[i]);
+  for (int i = 0; i < slots; ++i)
+    if (hits[(size_t)i] > 0) o.c[(size_t)i] /= (float)hits[(size_t)i];
+  // Hermitian along kz: OTF(kr, -kz) = conj OTF(kr, kz)
+  for (int r = 0; r < o.nr; ++r) {
+    o.at(r, 0) = zf(o.at(r, 0).real(), 0.f);
+    for (int kz = 1; kz <= o.nz / 2; ++kz) {
+      const zf both = (o.at(r, kz) + std::conj(o.at(r, o.nz - kz))) / 2.f;
+      o.at(r, kz) = both;
+      o.at(r, o.nz - kz) = std::conj(both);
     }
   }
+  return o;
 }
 
-void cleanup(cf* otf, int nx, int nz, float dkr, float dkz, int lamdanm, int krmax_user, float NA, float NIMM) {
-  const int ncl = nx / 2 + 1;
-  const float lamda = lamdanm * 0.001;
-  const float sinalpha = NA / NIMM;
-  const float cosalpha = std::cos(std::asin(sinalpha));
-  float krmax = 2 * NA / lamda;
-  if (krmax_user * dkr < krmax && krmax_user != 0) krmax = krmax_user * dkr;
-  for (int ix = 0; ix < ncl; ++ix) {
-    const float kr = ix * dkr;
-    if (kr <= krmax) {
-      const float beta = std::asin((NA - kr * lamda) / NIMM);
-      const float kzedge = (NIMM / lamda) * (std::cos(beta) - cosalpha);
-      const int kzstart = (int)std::rint((kzedge / dkz) + 1);
-      const int kzend = nz - kzstart;
-      for (int iz = kzstart; iz <= kzend; ++iz) otf[ix * nz + iz] = 0;
-    } else {
-      for (int iz = 0; iz < nz; ++iz) otf[ix * nz + iz] = 0;
+// Zero every (kr, kz) the objective cannot pass: kr beyond 2 NA / lambda (or
+// the caller's radius in pixels, when smaller), and per kr the kz band
+// between the two edges of the support.
+void mask_support(RadialOtf& o, float dkr, float dkz, int lambda_nm, int kr_cap, float na, float n_imm) {
+  if (!(na > 0.f && na < n_imm)) refuse("makeOTF: cleanup needs 0 < NA < the immersion index");
+  const float wl = lambda_nm * 0.001;  // micrometres
+  const float cos_max = std::cos(std::asin(na / n_imm));
+  float kr_edge = 2 * na / wl;
+  if (kr_cap != 0 && kr_cap * dkr < kr_edge) kr_edge = kr_cap * dkr;
+  for (int r = 0; r < o.nr; ++r) {
+    const float kr = r * dkr;
+    if (kr > kr_edge) {
+      for (int z = 0; z < o.nz; ++z) o.at(r, z) = 0;
+      continue;
     }
+    const float tilt = std::asin((na - kr * wl) / n_imm);
+    const float kz_edge = (n_imm / wl) * (std::cos(tilt) - cos_max);
+    const int first = std::max(0, (int)std::rint((kz_edge / dkz) + 1));
+    for (int z = first; z <= o.nz - first; ++z) o.at(r, z) = 0;
   }
 }
 
-// least-squares intercept at x = 0 of vals[1..kx2] (double, as the reference)
-cd intercept(const cd* vals, int kx2) {
-  const double mean_kx = (kx2 + 1) / 2.;
-  cd mean_val = 0, num = 0, den = 0;
-  for (int x = 1; x <= kx2; ++x) mean_val += vals[x];
-  mean_val /= (double)kx2;
-  for (int x = 1; x <= kx2; ++x) {
-    num += (x - mean_kx) * (vals[x] - mean_val);
-    den += (x - mean_kx) * (x - mean_kx);
+// least-squares line through (k, v[k]), k = 1..n, evaluated at k = 0
+zd line_at_zero(const std::vector<zd>& v, int n) {
+  const double kbar = (n + 1) / 2.;
+  zd vbar = 0;
+  for (int k = 1; k <= n; ++k) vbar += v[(size_t)k];
+  vbar /= (double)n;
+  zd cov = 0, var = 0;
+  for (int k = 1; k <= n; ++k) {
+    cov += (k - kbar) * (v[(size_t)k] - vbar);
+    var += (k - kbar) * (k - kbar);
   }
-  return mean_val - (num / den) * mean_kx;
+  return vbar - (cov / var) * kbar;
 }
 
-bool fixorigin(cf* otf, int nx, int nz, int kx2) {
-  std::vector<cd> col((size_t)kx2 + 1);
-  for (int z = 0; z < nz; ++z) {
-    for (int x = 1; x <= kx2; ++x) col[(size_t)x] = cd(otf[x * nz + z]);
-    const cd c = intercept(col.data(), kx2);
-    otf[z] = cf((float)c.real(), (float)c.imag());
-    if (z == 0 && std::abs(otf[z]) <= std::abs(otf[nz + z])) return false;
+zf to_float(const zd& d) { return zf((float)d.real(), (float)d.imag()); }
+
+// kr = 0 of every kz from kr = 1..n; false (nothing usable) when the
+// extrapolated kz = 0 value is not above its kr = 1 neighbour
+bool extrapolate_origin(RadialOtf& o, int n) {
+  std::vector<zd> col((size_t)n + 1);
+  for (int z = 0; z < o.nz; ++z) {
+    for (int k = 1; k <= n; ++k) col[(size_t)k] = zd(o.at(k, z));
+    o.at(0, z) = to_float(line_at_zero(col, n));
+    if (z == 0 && std::abs(o.at(0, 0)) <= std::abs(o.at(1, 0))) return false;
   }
   return true;
 }
 
-// 3-D variant: project the +45 and -45 degree radial lines of every kz plane
-// onto kx and extrapolate the (0, 0, 0) value only (lines stop at ky = ny - 1,
-// where the reference would read past the plane of a stack wider than tall)
-void fixorigin3d(cf* otf, int nh, int ny, int nz, int kx2) {
-  std::vector<cf> proj((size_t)nh, cf(0.f, 0.f));
-  for (int z = 0; z < nz; ++z)
-    for (int x = 1; x < nh && x < ny; ++x) {
-      proj[(size_t)x] += otf[((size_t)z * ny + x) * nh + x];
-      proj[(size_t)x] += otf[((size_t)z * ny + (ny - x)) * nh + x];
+// 3-D form: the (0, 0, 0) value only, from the sums over kz of the +45 and
+// -45 degree diagonals (ky = kx and ky = -kx) at kx = 1..n.  The diagonals
+// stop at ky = ny - 1, where the reference would read past the plane of a
+// stack wider than tall.
+void extrapolate_origin_3d(Spectrum& s, int n) {
+  std::vector<zf> diag((size_t)s.nh, zf(0.f, 0.f));
+  for (int z = 0; z < s.nz; ++z)
+    for (int k = 1; k < s.nh && k < s.ny; ++k) {
+      diag[(size_t)k] += s.at(z, k, k);
+      diag[(size_t)k] += s.at(z, s.ny - k, k);
     }
-  std::vector<cd> col((size_t)kx2 + 1);
-  for (int x = 1; x <= kx2; ++x) col[(size_t)x] = cd(proj[(size_t)x]);
-  const cd c = intercept(col.data(), kx2);
-  otf[0] = cf((float)c.real(), (float)c.imag());
+  std::vector<zd> col((size_t)n + 1);
+  for (int k = 1; k <= n; ++k) col[(size_t)k] = zd(diag[(size_t)k]);
+  s.at(0, 0, 0) = to_float(line_at_zero(col, n));
 }
 
-// RA table: multiply by 1 / max (:488-504); 3-D: divide by max (:506-518)
-void rescale(cf* otf, size_t n, bool divide) {
-  float valmax = 0;
-  for (size_t i = 0; i < n; ++i) valmax = std::max(valmax, std::abs(otf[i]));
+// max |OTF| = 1: the averaged table is multiplied by 1 / max, the 3-D one
+// divided by max (the two rounding forms of the reference)
+void normalize(std::vector<zf>& c, bool divide) {
+  float top = 0;
+  for (const zf& v : c) top = std::max(top, std::abs(v));
   if (divide) {
-    for (size_t i = 0; i < n; ++i) otf[i] = cf(otf[i].real() / valmax, otf[i].imag() / valmax);
+    for (zf& v : c) v = zf(v.real() / top, v.imag() / top);
   } else {
-    const float s = 1 / valmax;
-    for (size_t i = 0; i < n; ++i) otf[i] *= s;
+    const float inv = 1 / top;
+    for (zf& v : c) v *= inv;
   }
 }
 
@@ -265,40 +323,31 @@ int makeOTF(const char* const ifiles, const char* const ofiles, int lambdanm, fl
             bool b3Dout) {
   try {
     g_err.clear();
-    if (!ifiles || !ofiles) throw std::runtime_error("makeOTF: null file name");
+    if (!ifiles || !ofiles) refuse("makeOTF: null file name");
     cdn::TiffImage t = cdn::tiff_read(ifiles);
-    const int nx = t.width, ny = t.height, nz = t.depth;
-    if (nx < 2 || ny < 2 || nz < 2) throw std::runtime_error("makeOTF: the PSF must be at least 2x2x2");
-    const float dkr = 1 / (ny * dr);
-    const float dkz = 1 / (nz * dz);
-    float xc, yc, zc, est;
-    center_and_background(t.data, nx, ny, nz, &xc, &yc, &zc, &est);
-    if (!bUserBackground) background = est;
-    for (float& v : t.data) v -= background;
-    std::vector<cf> bands = rfft3(t.data, nx, ny, nz);
-    shift_center(bands.data(), nx, ny, nz, xc, yc, zc);
-    const int nh = nx / 2 + 1;
+    Psf psf;
+    psf.nx = t.width;
+    psf.ny = t.height;
+    psf.nz = t.depth;
+    psf.v = std::move(t.data);
+    if (psf.nx < 2 || psf.ny < 2 || psf.nz < 2) refuse("makeOTF: the PSF must be at least 2x2x2");
+    const Bead bead = locate_bead(psf);
+    Spectrum s = half_spectrum(psf, bUserBackground ? background : bead.background);
+    move_to_origin(s, bead);
+    if (interpkr >= s.nh) refuse("makeOTF: interpkr must be below nx/2+1");
     if (b3Dout) {
-      if (interpkr > 1) {
-        if (interpkr >= nh) throw std::runtime_error("makeOTF: interpkr must be below nx/2+1");
-        fixorigin3d(bands.data(), nh, ny, nz, interpkr);
-      }
-      rescale(bands.data(), bands.size(), true);
-      cdn::tiff_write_f32(ofiles, reinterpret_cast<const float*>(bands.data()), 2 * nh, ny, nz);
-    } else {
-      std::vector<cf> avg((size_t)nh * nz, cf(0.f, 0.f));
-      radialft(bands.data(), nx, ny, nz, avg.data());
-      if (bDoCleanup) cleanup(avg.data(), nx, nz, dkr, dkz, lambdanm, krmax, NA, NIMM);
-      if (interpkr > 0) {
-        if (interpkr >= nh) throw std::runtime_error("makeOTF: interpkr must be below nx/2+1");
-        while (!fixorigin(avg.data(), nx, nz, interpkr)) {
-          --interpkr;
-          if (interpkr < 4) throw std::runtime_error("#pixels < 4 used in kr=0 extrapolation");
-        }
-      }
-      rescale(avg.data(), avg.size(), false);
-      cdn::tiff_write_f32(ofiles, reinterpret_cast<const float*>(avg.data()), 2 * nz, nh, 1);
+      if (interpkr > 1) extrapolate_origin_3d(s, interpkr);
+      normalize(s.c, true);
+      cdn::tiff_write_f32(ofiles, reinterpret_cast<const float*>(s.c.data()), 2 * s.nh, s.ny, s.nz);
+      return 0;
     }
+    RadialOtf o = ring_average(s);
+    if (bDoCleanup) mask_support(o, 1 / (psf.ny * dr), 1 / (psf.nz * dz), lambdanm, krmax, NA, NIMM);
+    if (interpkr > 0)
+      while (!extrapolate_origin(o, interpkr))
+        if (--interpkr < 4) refuse("makeOTF: fewer than 4 kr samples left for the kr = 0 extrapolation (interpkr)");
+    normalize(o.c, false);
+    cdn::tiff_write_f32(ofiles, reinterpret_cast<const float*>(o.c.data()), 2 * o.nz, o.nr, 1);
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();
