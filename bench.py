@@ -222,6 +222,9 @@ def warm_start_eval(cdn, iters=50):
     res["reference"] = f"plain RL-BA after {iters} iterations (relative L2)"
     res["weights"] = {k: meta[k] for k in ("arch", "patches", "steps", "train_ms", "validation_seed_rel")
                       if k in meta}
+    res["training_data"] = ("patches of 64x128x128 cut from whole 128x1024x1024 light-sheet stacks, each stack "
+                            "deconvolved whole by 50 plain RL-BA iterations (not 4000 patches each deconvolved "
+                            "on its own): the targets carry no circular wrap at patch borders")
     res["setup_s"] = round(gen, 1)
     return res
 
