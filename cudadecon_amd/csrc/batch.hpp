@@ -96,7 +96,7 @@ struct Batch {
   void enqueue_engine(hipStream_t st, int i, int iters);
   void enqueue_streams(int iters);
   void enqueue_grouped(hipStream_t st, int iters);
-  void enqueue_partition(const Partition& P, hipStream_t st, int iters);
+  void enqueue_partition(const Partition& P, hipStream_t st, int iters, hipEvent_t mark);
   void capture(CapturedRun& c, int iters, bool grouped);
   std::vector<const void*> addresses(bool grouped) const;
 };
