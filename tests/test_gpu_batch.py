@@ -172,8 +172,8 @@ def test_batch_48_uneven_stacks(cdn, orc):
     channel PSFs; 10 iterations as one captured grouped schedule.  Three
     stacks (one per channel, the largest shape drawn among them) against
     the FP64 oracle's digest (tests/golden/make_batch_digest.py), every
-    stack bitwise against a one-stack engine in the grouped-graph and
-    per-stack graph modes."""
+    stack bitwise against a one-stack engine in the grouped-graph (one and
+    four partitions) and per-stack graph modes."""
     from cudadecon_amd import synth
     d = np.load(os.path.join(os.path.dirname(__file__), "golden", "batch_48_digest.npz"))
     shapes = synth.batch_shapes()
@@ -202,13 +202,14 @@ def test_batch_48_uneven_stacks(cdn, orc):
         eng.close()
         if i % 8 == 7:
             print(f"staged {i + 1} stacks", flush=True)
-    for mode in ("grouped_graph", "graph"):
+    for mode, parts in (("grouped_graph", 1), ("graph", 1), ("grouped_graph", 4)):
+        b.set_partitions(parts)
         ms = b.run(synth.BATCH_ITERS, mode)
-        print(f"mode {mode}: {ms:.1f} ms for 10 iterations of all 48 stacks", flush=True)
+        print(f"mode {mode}, {parts} partition(s): {ms:.1f} ms for 10 iterations of all 48 stacks", flush=True)
         for i, s in enumerate(shapes):
             X = b.get(i)
-            assert _sha(X) == single[i], (mode, i)
-            if i in digest_stacks and mode == "grouped_graph":
+            assert _sha(X) == single[i], (mode, parts, i)
+            if i in digest_stacks and mode == "grouped_graph" and parts == 1:
                 idx = d[f"s{i}_idx"]
                 err = float(np.abs(X.reshape(-1)[idx] - d[f"s{i}_val"]).max() / d[f"s{i}_xmax"])
                 lam_err = float(np.abs(b.lambdas(i, synth.BATCH_ITERS)[2:] - d[f"s{i}_lam"][2:]).max())
