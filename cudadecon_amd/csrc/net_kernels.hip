@@ -332,7 +332,8 @@ __global__ __launch_bounds__(256) void k_conv25(const bf16_t* __restrict__ in, c
       } else {
       // 64-channel layers and the fused up-add: the dz loop is not unrolled (an
       // unrolled one keeps too many fragments live: spills or occupancy 1)
-  #pragma unroll((CIN == 64 || COUT == 64 || UPADD) ? 1 : 3)
+      constexpr int kDzUnroll = (CIN == 64 || COUT == 64 || UPADD) ? 1 : 3;
+  #pragma unroll kDzUnroll
       for (int dz = -1; dz <= 1; ++dz) {
         const bf16_t* base = ring + ((z + dz + 3) % 3) * PLANE;
   #pragma unroll
