@@ -61,6 +61,10 @@ Stage& stage() {
   return *s;
 }
 
+// on a failed call: no DMA through the staging buffers may still be in
+// flight when the next call takes them (the error itself is reported)
+void drain(hipStream_t st) { (void)hipStreamSynchronize(st); }
+
 }  // namespace
 
 void copy_to_device(hipStream_t st, void* dev, const void* host, size_t bytes) {
@@ -73,16 +77,21 @@ void copy_to_device(hipStream_t st, void* dev, const void* host, size_t bytes) {
   S.ensure();
   CallEvents E(st);
   const size_t n = (bytes + kChunk - 1) / kChunk;
-  for (size_t i = 0; i < n; ++i) {
-    const int b = (int)(i & 1);
-    const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
-    if (i >= 2) CDN_HIP(hipEventSynchronize(E.e[b]));  // the DMA of chunk i - 2 left buf[b]
-    S.pool->copy(S.buf[b], static_cast<const char*>(host) + off, len);
-    CDN_HIP(hipMemcpyAsync(static_cast<char*>(dev) + off, S.buf[b], len, hipMemcpyHostToDevice, st));
-    CDN_HIP(hipEventRecord(E.e[b], st));
+  try {
+    for (size_t i = 0; i < n; ++i) {
+      const int b = (int)(i & 1);
+      const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+      if (i >= 2) CDN_HIP(hipEventSynchronize(E.e[b]));  // the DMA of chunk i - 2 left buf[b]
+      S.pool->copy(S.buf[b], static_cast<const char*>(host) + off, len);
+      CDN_HIP(hipMemcpyAsync(static_cast<char*>(dev) + off, S.buf[b], len, hipMemcpyHostToDevice, st));
+      CDN_HIP(hipEventRecord(E.e[b], st));
+    }
+    // the staging buffers are free for the next call
+    for (size_t b = 0; b < 2 && b < n; ++b) CDN_HIP(hipEventSynchronize(E.e[b]));
+  } catch (...) {
+    drain(st);
+    throw;
   }
-  // the staging buffers are free for the next call
-  for (size_t b = 0; b < 2 && b < n; ++b) CDN_HIP(hipEventSynchronize(E.e[b]));
 }
 
 void copy_to_host(hipStream_t st, void* host, const void* dev, size_t bytes) {
@@ -102,14 +111,19 @@ void copy_to_host(hipStream_t st, void* host, const void* dev, size_t bytes) {
     CDN_HIP(hipMemcpyAsync(S.buf[b], static_cast<const char*>(dev) + off, len, hipMemcpyDeviceToHost, st));
     CDN_HIP(hipEventRecord(E.e[b], st));
   };
-  issue(0);
-  for (size_t i = 0; i < n; ++i) {
-    // buf[(i + 1) & 1] was emptied by the host copy of chunk i - 1
-    if (i + 1 < n) issue(i + 1);
-    const int b = (int)(i & 1);
-    const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
-    CDN_HIP(hipEventSynchronize(E.e[b]));
-    S.pool->copy(static_cast<char*>(host) + off, S.buf[b], len);
+  try {
+    issue(0);
+    for (size_t i = 0; i < n; ++i) {
+      // buf[(i + 1) & 1] was emptied by the host copy of chunk i - 1
+      if (i + 1 < n) issue(i + 1);
+      const int b = (int)(i & 1);
+      const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+      CDN_HIP(hipEventSynchronize(E.e[b]));
+      S.pool->copy(static_cast<char*>(host) + off, S.buf[b], len);
+    }
+  } catch (...) {
+    drain(st);
+    throw;
   }
 }
 

@@ -254,8 +254,13 @@ void WarmNet::add_samples(const float* raw, const float* tgt, long count, int nz
       void* np = nullptr;
       const size_t cap = std::max(sizeof(float) * (old + add), b->bytes * 2);
       CDN_HIP(hipMalloc(&np, cap));
-      if (old) CDN_HIP(hipMemcpyAsync(np, b->p, sizeof(float) * old, hipMemcpyDeviceToDevice, st));
-      CDN_HIP(hipStreamSynchronize(st));
+      try {
+        if (old) CDN_HIP(hipMemcpyAsync(np, b->p, sizeof(float) * old, hipMemcpyDeviceToDevice, st));
+        CDN_HIP(hipStreamSynchronize(st));
+      } catch (...) {
+        (void)hipFree(np);
+        throw;
+      }
       b->release();
       b->p = np;
       b->base = np;
