@@ -243,6 +243,12 @@ int Batch::grouped_launches() const {
   return n;
 }
 
+void Batch::set_partitions(int n) {
+  if (n < 1 || n > (int)streams.size() - 1) fail("batch: partitions must be 1 .. " + std::to_string(streams.size() - 1));
+  npart = n;
+  if (built_npart != npart) build_groups();
+}
+
 void Batch::enqueue_engine(hipStream_t st, int i, int iters) {
   RLEngine& e = *engines[i];
   e.reset(st);

@@ -90,6 +90,9 @@ struct Batch {
   std::vector<double> lambdas(int i, int iters) const;
   // kernel launches per iteration in a grouped run
   int grouped_launches() const;
+  // partitions of the grouped schedule (1 <= n <= working streams); the
+  // schedule is rebuilt at once
+  void set_partitions(int n);
 
  private:
   void build_groups();

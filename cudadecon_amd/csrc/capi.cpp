@@ -1074,8 +1074,8 @@ int cudaDecon_batch_set_grid_cap(cudaDecon_batch* h, long max_workgroups) {
 
 int cudaDecon_batch_set_partitions(cudaDecon_batch* h, int n) {
   return guard([&]() -> int {
-    if (!h || n < 1 || n > (int)h->b.streams.size() - 1) fail("cudaDecon_batch_set_partitions: bad argument");
-    h->b.npart = n;
+    if (!h) fail("cudaDecon_batch_set_partitions: bad argument");
+    h->b.set_partitions(n);
     return 1;
   });
 }

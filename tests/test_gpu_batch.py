@@ -149,8 +149,12 @@ def test_grouped_modes_match_single_engines(cdn, orc):
     for i in range(len(MIXED)):
         assert np.array_equal(b.get(i), refs[i]), ("persistent", i)
     b.set_grid_cap(0)
+    one = b.launches()
     for parts in (2, 3):
         b.set_partitions(parts)
+        # the schedule is rebuilt at once: each partition launches its own
+        # passes (and its own factor launch)
+        assert b.launches() >= one + parts - 1, (parts, b.launches(), one)
         for mode in ("grouped", "grouped_graph", "grouped_graph"):
             print("partitions", parts, mode, flush=True)
             b.run(7, mode)
