@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <filesystem>
 #include <future>
 #include <iostream>
@@ -489,6 +490,35 @@ void stitch(Stack& dst, const Stack& t, int tile_index, int ntiles, int overlap,
     }
 }
 
+// The thread that writes one file's results while the next file is
+// processed.  An error while writing is reported by the next wait(); the
+// thread is joined on every way out of run(), errors included.
+struct Saver {
+  std::thread t;
+  std::exception_ptr err;
+  template <class F> void start(F&& f) {
+    wait();
+    t = std::thread([this, f = std::forward<F>(f)]() mutable {
+      try {
+        f();
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+  }
+  void wait() {
+    if (t.joinable()) t.join();
+    if (err) {
+      std::exception_ptr e = err;
+      err = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+  ~Saver() {
+    if (t.joinable()) t.join();
+  }
+};
+
 int run(int argc, char** argv) {
   Options o;
   if (parse(argc, argv, o)) return 0;
@@ -510,7 +540,7 @@ int run(int argc, char** argv) {
 
   const auto t_start = std::chrono::steady_clock::now();
   std::future<Stack> next = std::async(std::launch::async, load_stack, files[(size_t)o.skip]);
-  std::thread saver;
+  Saver saver;
   int last_nx = 0, last_ny = 0, last_nz = 0;
   for (size_t f = (size_t)o.skip; f < files.size(); ++f) {
     std::cout << "\nLoading raw_image: " << f + 1 << " out of " << files.size() << ".\n" << files[f] << std::endl;
@@ -563,13 +593,13 @@ int run(int argc, char** argv) {
       if (out.have_decon) cr(out.decon);
       if (out.have_deskewed) cr(out.deskewed);
     }
-    if (saver.joinable()) saver.join();  // the previous file's writes
+    saver.wait();  // the previous file's writes
     // voxel sizes (src/linearDecon.cpp:1014-1022): dz of the deskewed grid,
     // dr after a rotation
     const float dz_desk = desk ? o.dzdata * std::fabs(std::sin(o.deskew * (float)M_PI / 180.f)) : o.dzdata;
     const float dz_out = rot ? o.drdata : dz_desk;
     const std::string in = files[f];
-    saver = std::thread([o, in, out = std::move(out), dz_desk, dz_out, mips, rmips, dir]() {
+    saver.start([o, in, out = std::move(out), dz_desk, dz_out, mips, rmips, dir]() {
       const char* ax[3] = {"_MIP_x", "_MIP_y", "_MIP_z"};
       if (out.have_deskewed) {
         save(out_path(dir.string(), "Deskewed", in, "_deskewed"), out.deskewed, o.uint16, o.drdata, dz_desk);
@@ -589,7 +619,7 @@ int run(int argc, char** argv) {
     std::printf(">>>file_finished\n");
     std::fflush(stdout);
   }
-  if (saver.joinable()) saver.join();
+  saver.wait();
   RL_cleanup();
   const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   const size_t n = files.size() - (size_t)o.skip;

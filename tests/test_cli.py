@@ -190,6 +190,12 @@ def test_cli_outputs_equal_rl_interface(cdn, tmp_path):
     mipx = _read(str(d2 / "Deskewed" / "MIPs" / "cell1_ch0_stack0000_MIP_x.tif"))[0]
     assert np.array_equal(mipx, ref_desk.max(axis=2))
 
+    print("cli: a result that cannot be written ends the run with an error", flush=True)
+    d3, _ = _folder(tmp_path / "c", [(32, 64, 64), (32, 64, 64)], seed0=7)
+    (d3 / "GPUdecon" / "cell1_ch0_stack0000_decon.tif").mkdir(parents=True)  # a folder where the file goes
+    r = _cli(["-i", 2, d3, "stack", otf], check=False)
+    assert r.returncode == 1 and "!!Error occurred" in r.stderr, (r.returncode, r.stdout, r.stderr)
+
 
 @pytest.mark.gpu
 def test_cli_y_tiles_stitch_as_the_reference(cdn, tmp_path):
