@@ -259,11 +259,90 @@ __device__ __forceinline__ void stockham_stage(float2* line, int p, const float2
   __syncthreads();
 }
 
+// Two consecutive stages (radices R1 then R2, the first at span LP) in one
+// pass over LDS.  Super-butterfly j < N/(R1 R2) loads the R1 R2 elements
+// j + u N/(R1 R2); they are the inputs of the first stage's butterflies
+// j + m N/(R1 R2) (m < R2, twiddle index j mod LP), whose outputs are the
+// inputs of the second stage's butterflies (j - j mod LP) R1 + j mod LP + q LP
+// (q < R1, twiddle index j mod LP + q LP).  The same operations in the same
+// order as the two stages run one after the other, with one LDS round trip
+// and two barriers fewer; results agree to FP32 rounding (the compiler may
+// pick other products to fuse into multiply-adds in the longer block).
+template <int N, int R1, int R2, int LP, int TPT, int SE, bool INV>
+__device__ __forceinline__ void stockham_stage2(float2* line, int p, const float2* __restrict__ tw1,
+                                                const float2* __restrict__ tw2) {
+  constexpr int B = N / (R1 * R2);
+  constexpr int U = (B + TPT - 1) / TPT;
+  constexpr bool EXACT = (B % TPT) == 0;
+  float2 v[U][R1 * R2];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = p + u * TPT;
+    if (EXACT || j < B) {
+      const int k1 = j % LP;
+      float2 y[R2][R1];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        float2 x[R1];
+#pragma unroll
+        for (int t = 0; t < R1; ++t) x[t] = line[(j + (m + t * R2) * B) * SE];
+        if constexpr (LP > 1) {
+          const float2* w = tw1 + k1 * (R1 - 1);
+#pragma unroll
+          for (int t = 1; t < R1; ++t) x[t] = INV ? cmulc(x[t], w[t - 1]) : cmul(x[t], w[t - 1]);
+        }
+        Dft<R1, INV>::run(x);
+#pragma unroll
+        for (int q = 0; q < R1; ++q) y[m][q] = x[q];
+      }
+#pragma unroll
+      for (int q = 0; q < R1; ++q) {
+        float2 z[R2];
+#pragma unroll
+        for (int t = 0; t < R2; ++t) z[t] = y[t][q];
+        const float2* w = tw2 + (k1 + q * LP) * (R2 - 1);
+#pragma unroll
+        for (int t = 1; t < R2; ++t) z[t] = INV ? cmulc(z[t], w[t - 1]) : cmul(z[t], w[t - 1]);
+        Dft<R2, INV>::run(z);
+#pragma unroll
+        for (int q2 = 0; q2 < R2; ++q2) v[u][q * R2 + q2] = z[q2];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = p + u * TPT;
+    if (EXACT || j < B) {
+      const int k1 = j % LP;
+      const int base = (j - k1) * R1 * R2 + k1;
+#pragma unroll
+      for (int q = 0; q < R1; ++q)
+#pragma unroll
+        for (int q2 = 0; q2 < R2; ++q2) line[(base + q * LP + q2 * LP * R1) * SE] = v[u][q * R2 + q2];
+    }
+  }
+  __syncthreads();
+}
+
+// Stage pairs whose radix product is at most this run as one pass
+// (stockham_stage2); 0 runs every stage on its own.  810-point lines
+// (stages 2, 3, 3, 3, 3, 5 -> (2, 3), (3, 3), (3, 5)): the LRcore pass
+// 0.283 -> 0.208 ms at 128 x 512 x 810 with the same launch shape; a limit
+// of 9 ((2, 3), (3, 3), 3, 5) measured 0.208 against 0.200-0.206 for 16.
+// Powers of two pair nothing: every pair of their stages holds a radix 16.
+#ifndef CDN_FUSE_MAX
+#define CDN_FUSE_MAX 16
+#endif
+
 template <int N, int TPT, int SE, bool INV, int S = 0>
 __device__ __forceinline__ void fft_lds(float2* line, int p, const float2* __restrict__ tw) {
   constexpr Plan P = make_plan(N);
   static_assert(P.ok, "FFT length must be 2,3,5,7-smooth");
-  if constexpr (S < P.ns) {
+  if constexpr (S + 1 < P.ns && P.r[S] * P.r[S + 1] <= CDN_FUSE_MAX) {
+    stockham_stage2<N, P.r[S], P.r[S + 1], P.lp[S], TPT, SE, INV>(line, p, tw + P.twoff[S], tw + P.twoff[S + 1]);
+    fft_lds<N, TPT, SE, INV, S + 2>(line, p, tw);
+  } else if constexpr (S < P.ns) {
     stockham_stage<N, P.r[S], P.lp[S], TPT, SE, INV>(line, p, tw + P.twoff[S]);
     fft_lds<N, TPT, SE, INV, S + 1>(line, p, tw);
   }

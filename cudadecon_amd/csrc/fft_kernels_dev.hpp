@@ -100,13 +100,19 @@ constexpr int kXTptOverride[] = {CDN_XTPT_OVERRIDE};
 // Rule 3 adds: lines whose plan still leaves more than 16 elements per
 // thread (radix lcm 20 to 30: 768, 810) held them in up to 256 registers,
 // one or two waves per SIMD; they take the smallest thread count with at
-// most 6 elements per thread and the all-LDS transform.  The LRcore pass
-// (two transforms per line) does so only above 24 elements per thread, to
-// at most 15: at 24 (384, 768) the register-resident transform measured
-// faster (x_lrcore 0.102 vs 0.120 ms at 64 x 768 x 768).  At 128 x 512 x 810
-// the x passes went from 0.25 / 0.47 / 0.39 ms (R2C / LRcore / update) to
-// 0.18 / 0.26 / 0.24; at 64 x 768 x 768 R2C and update from 0.117 / 0.132
-// to 0.106 / 0.125.
+// most CDN_XEMAX elements per thread and the all-LDS transform.  The LRcore
+// pass (two transforms per line) does so only above 24 elements per thread,
+// to at most CDN_XEMAX_LR: at 24 (384, 768) the register-resident transform
+// measured faster (x_lrcore 0.102 vs 0.120 ms at 64 x 768 x 768).  At
+// 128 x 512 x 810 the x passes went from 0.25 / 0.47 / 0.39 ms (R2C / LRcore
+// / update) to 0.18 / 0.26 / 0.24 (caps 6 and 15); at 64 x 768 x 768 R2C and
+// update from 0.117 / 0.132 to 0.106 / 0.125.  With the all-LDS transform's
+// stage pairs fused (CDN_FUSE_MAX), caps 5 and 9 (810: 162 threads per line
+// for R2C and the update, 90 for LRcore, against 135 and 54) measured
+// 0.172 / 0.200 / 0.223 ms against 0.181 / 0.196 / 0.234 for 6 and 9,
+// 0.171 / 0.221 / 0.220 for 5 and 5, 0.172 / 0.210 / 0.222 for 5 and 15;
+// 768 was the same under all of them (iteration 1.389-1.394 ms at
+// 128 x 768 x 768).
 #ifndef CDN_XTPT_RULE
 #define CDN_XTPT_RULE 3
 #endif
@@ -138,13 +144,21 @@ enum XKind : int { XK_PLAIN = 0, XK_LR = 1, XK_R2C = 2 };
 #define CDN_R2C_TPT 512, 128, 1024, 256
 #endif
 constexpr int kR2cTpt[] = {CDN_R2C_TPT};
+// Rule 3's element caps (elements per thread) for the plain / R2C passes and
+// for the LRcore pass.
+#ifndef CDN_XEMAX
+#define CDN_XEMAX 5
+#endif
+#ifndef CDN_XEMAX_LR
+#define CDN_XEMAX_LR 9
+#endif
 constexpr int tpt_x(int N, int kind = XK_PLAIN) {
   if (lookup_override(kXTptOverride, N) > 0) return lookup_override(kXTptOverride, N);
   if (kind == XK_R2C && lookup_override(kR2cTpt, N) > 0) return lookup_override(kR2cTpt, N);
   const bool lr = kind == XK_LR;
   const int t = tpt_x_short(N, lr);
   if (CDN_XTPT_RULE < 3 || N / t <= (lr ? 24 : 16)) return t;
-  const int emax = lr ? 15 : 6;
+  const int emax = lr ? CDN_XEMAX_LR : CDN_XEMAX;
   for (int d = t; d <= N && d <= 1024; ++d)
     if (N % d == 0 && N / d <= emax) return d;
   return t;
@@ -468,7 +482,9 @@ __global__ void __launch_bounds__((XCfg<N, XK_LR>::THREADS))
 }
 
 // Deterministic workgroup sum of two doubles; out[0], out[1] written by
-// thread 0.  Wavefront shuffles only when every wavefront is full.
+// thread 0.  Wavefront shuffles only over full wavefronts: with a partial
+// last wavefront the first one gathers the entries (workgroups of more than
+// 64 threads) or thread 0 sums them alone (64 or fewer).
 template <int THREADS>
 __device__ __forceinline__ void block_sum2(double a, double b, double (*red)[THREADS], double* out) {
   const int tid = threadIdx.x;
@@ -488,6 +504,28 @@ __device__ __forceinline__ void block_sum2(double a, double b, double (*red)[THR
       }
       out[0] = sa;
       out[1] = sb;
+    }
+  } else if constexpr (THREADS > 64) {
+    // a partial last wavefront: lane t of the first (full) wavefront sums
+    // entries t, t + 64, ... in that order, then a wavefront sum
+    red[0][tid] = a;
+    red[1][tid] = b;
+    __syncthreads();
+    if (tid < 64) {
+      double sa = 0.0, sb = 0.0;
+#pragma unroll
+      for (int w = 0; w < (THREADS + 63) / 64; ++w) {
+        if (tid + 64 * w < THREADS) {
+          sa += red[0][tid + 64 * w];
+          sb += red[1][tid + 64 * w];
+        }
+      }
+      sa = wave_sum(sa);
+      sb = wave_sum(sb);
+      if (tid == 0) {
+        out[0] = sa;
+        out[1] = sb;
+      }
     }
   } else {
     red[0][tid] = a;
