@@ -173,6 +173,41 @@ def stacks_256(cdn, tab, iters=10, nstreams=8):
     return out
 
 
+def iteration_128x512x810(cdn, iters=20):
+    """Plain RL-BA iterations on the deskewed grid of the 128x512x512
+    light-sheet workload (128 x 512 x 810: x lines of 2 * 3^4 * 5 points,
+    the all-LDS transform), inputs resident; per-pass HIP-event times."""
+    from cudadecon_amd import synth, warmstart
+    from cudadecon_amd.otf import make_otf
+    shape = (128, 512, 810)
+    c = synth.CONFIGS["lightsheet_128x512x512"]
+    tab = make_otf(synth.gaussian_psf(c["sxy"], c["sz"]))
+    # deskewed plane spacing dz sin(31.5 deg) = 0.15675 um
+    ks = warmstart.kscale_for(shape, tab, dr=c["dr"], dz=0.15675)
+    eng = cdn.RLEngine(shape, tab, ks)
+    blk = np.random.default_rng(0).uniform(0, 100, (32, 64, 64)).astype(np.float32)
+    eng.set_raw_block(blk, (8, 8, 8), 20.0)
+    eng.iterate(3)
+    eng.timing(1)
+    eng.iterate(iters)
+    ms, calls = eng.timing(0)
+    _, t = eng.iterate(iters)
+    eng.close()
+    nz, ny, nx = shape
+    R, P = 4.0 * nz * ny * nx, 8.0 * nz * ny * (nx // 2 + 1)
+    it = t / iters
+    pb = pass_bytes(R, P)
+    per = {k: ms[k] / max(calls[k], 1) for k in ms}
+    return {
+        "shape_zyx": list(shape), "iterations": iters, "ms_per_iteration": round(it, 4),
+        "Gvoxel_per_s": round(nz * ny * nx / (it * 1e-3) / 1e9, 2),
+        "GBps": round((8 * R + 16 * P) / (it * 1e-3) / 1e9, 1),
+        "frac_peak": round((8 * R + 16 * P) / (it * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "pass_ms": {k: round(v, 4) for k, v in per.items()},
+        "pass_GBps": {k: round(pb[k] / (per[k] * 1e-3) / 1e9, 1) for k in per if pb.get(k)},
+    }
+
+
 def cpu_baseline(R_full: np.ndarray, tab, ks, iters: int, slab: int, warm: int = 2):
     """FP32 oracle RL-BA (OpenMP, the oracle's own FFT) on the stack (its
     first `slab` z-planes when slab > 0): `warm` + `iters` iterations, the
@@ -348,6 +383,7 @@ def main():
                "note": "uint16 host input, preprocessing, iterations, FP32 host result (PCIe-inclusive)"}
 
     _progress(f"headline: {ms_per_step:.3f} ms per iteration")
+    deskewed = _leg("iteration_128x512x810", iteration_128x512x810, cdn)
     batch = None if args.no_batch else _leg("batch_48_stacks", batch_48_stacks, cdn)
     small = None if args.no_batch else _leg("batch_256_small_uneven", batch_small_uneven, cdn, tab)
     stacks = None if args.no_batch else _leg("stacks_256_of_64x128x128", stacks_256, cdn, tab)
@@ -415,6 +451,7 @@ def main():
         },
         "passes": passes,
         "rl_interface_e2e": e2e,
+        "iteration_128x512x810": deskewed,
         "batch_48_stacks": batch,
         "batch_256_small_uneven": small,
         "stacks_256_of_64x128x128": stacks,
