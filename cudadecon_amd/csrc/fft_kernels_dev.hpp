@@ -165,13 +165,21 @@ constexpr int tpt_x(int N, int kind = XK_PLAIN) {
 }
 
 // columns per tile for y/z passes: fill the LDS budget, at most 64 columns
-// and 1024 threads, power of two (cols > 0 forces a width)
+// (32 above length 64) and 1024 threads, power of two (cols > 0 forces a
+// width).  Lengths 96 and 128 with 32 instead of 64 columns (smaller
+// workgroups fill the register-limited wave slots: the length-128 z filter
+// runs 5 waves per SIMD, which 512-thread workgroups leave at 4), ms per
+// launch: z filter 0.062 -> 0.050 at 128 x 384 x 512, 0.110 -> 0.102 at
+// 128 x 512 x 810, 0.043 -> 0.038-0.040 at 96 x 384 x 384; 16 stacks of
+// 96 x 128 x 256 0.974 -> 0.922 ms per iteration, 64 of 64 x 128 x 128
+// 1.38-1.39 -> 1.36; no length-128 y pass measured slower.
 constexpr int lc_for(int N, int TPT, int cols = -1) {
   if (cols < 0) cols = lookup_override(kColsOverride, N);
   if (cols > 0) return cols;
   const int budget = (N >= 768) ? 131072 : 65536;
   int L = budget / (N * 8);
   if (L > 64) L = 64;
+  if (N > 64 && L > 32) L = 32;
   while (L > 1 && L * TPT > 1024) L /= 2;
   int p = 1;
   while (p * 2 <= L) p *= 2;
