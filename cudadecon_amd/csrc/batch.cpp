@@ -13,8 +13,8 @@
 //      table gives each row pair its factor); the y passes and the z filter
 //      take a table of per-stack segments (spectrum, column geometry, OTF) and
 //      find each workgroup's stack by a search over it; the factors of all
-//      stacks come from one launch over per-stack ranges of the partials.
-//      About 3 nx + 4 ny + 2 nz + 1 launches per iteration (the distinct
+//      stacks come from two launches over per-stack ranges of the partials.
+//      About 3 nx + 4 ny + 2 nz + 2 launches per iteration (the distinct
 //      lengths), instead of 9 or 10 per stack;
 //   4  schedule 3 captured once into a hipGraph and replayed.
 // The grouped schedule can run as P partitions: the stacks, in arena order,
@@ -239,7 +239,7 @@ void Batch::build_groups() {
 
 int Batch::grouped_launches() const {
   int n = 0;
-  for (const Partition& P : parts) n += 3 * (int)P.xg.size() + 4 * (int)P.yg.size() + 2 * (int)P.zg.size() + 1;
+  for (const Partition& P : parts) n += 3 * (int)P.xg.size() + 4 * (int)P.yg.size() + 2 * (int)P.zg.size() + 2;
   return n;
 }
 

@@ -189,14 +189,27 @@ void zblend(hipStream_t st, float* img, int nx, int ny, int nz, int nZblend) {
 // [0, 1] at src/RL-Biggs-Andrews.cpp:297-298), kept on the device:
 // lam[0] = clamp(sum num / (sum den + eps)), lam[1..2] = the two sums.
 // ---------------------------------------------------------------------------
-static constexpr int LBS = 1024;  // threads of the acceleration-factor kernel
-// one workgroup: nb partial pairs -> lam[0..2]
-__device__ __forceinline__ void lambda_of(const double2* __restrict__ part, int nb, double eps, double* lam) {
+// Two levels, each summed in a fixed order (deterministic): workgroup g of
+// a stack with n partial pairs sums the chunk [g C, min(n, (g + 1) C)),
+// C = ceil(n / LCH), and writes the sums over the chunk's first pair (which
+// it has read); one wavefront per stack then sums the chunk sums.  The
+// chunks depend only on n, so a stack gets the same factor in any engine.
+// At 256 x 1024 x 1024 (65536 pairs) both launches take 0.0103 ms per
+// iteration against 0.0140 for one 1024-thread workgroup over all pairs,
+// which left the rest of the GPU idle between two passes.
+static constexpr int LCH = 64;   // chunks per stack: one wavefront sums them
+static constexpr int LBS = 256;  // threads of a chunk workgroup
+__device__ __forceinline__ int lambda_chunk(int n) { return n > LCH ? (n + LCH - 1) / LCH : 1; }
+
+__device__ __forceinline__ void lambda_chunk_sum(double2* __restrict__ part, int n) {
   __shared__ double ra[LBS / 64], rb[LBS / 64];
+  const int C = lambda_chunk(n);
+  const int lo = blockIdx.x * C, hi = min(n, lo + C);
+  if (lo >= n) return;  // the whole workgroup
   // 4 independent accumulators per thread keep several loads in flight
   double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
-  int i = threadIdx.x;
-  for (; i + 3 * LBS < nb; i += 4 * LBS) {
+  int i = lo + threadIdx.x;
+  for (; i + 3 * LBS < hi; i += 4 * LBS) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const double2 v = part[i + u * LBS];
@@ -204,51 +217,76 @@ __device__ __forceinline__ void lambda_of(const double2* __restrict__ part, int 
       b[u] += v.y;
     }
   }
-  for (; i < nb; i += LBS) {
+  for (; i < hi; i += LBS) {
     const double2 v = part[i];
     a[0] += v.x;
     b[0] += v.y;
   }
-  double sa = wave_sum((a[0] + a[1]) + (a[2] + a[3]));
-  double sb = wave_sum((b[0] + b[1]) + (b[2] + b[3]));
+  const double sa = wave_sum((a[0] + a[1]) + (a[2] + a[3]));
+  const double sb = wave_sum((b[0] + b[1]) + (b[2] + b[3]));
   if ((threadIdx.x & 63) == 0) {
     ra[threadIdx.x >> 6] = sa;
     rb[threadIdx.x >> 6] = sb;
   }
-  __syncthreads();
+  __syncthreads();  // every read of the chunk precedes the write over its first pair
   if (threadIdx.x == 0) {
     double ta = 0, tb = 0;
+#pragma unroll
     for (int w = 0; w < LBS / 64; ++w) {
       ta += ra[w];
       tb += rb[w];
     }
-    double l = ta / (tb + eps);
-    l = l < 1.0 ? l : 1.0;
-    l = l > 0.0 ? l : 0.0;
-    lam[0] = l;
+    part[lo] = make_double2(ta, tb);
+  }
+}
+
+// one wavefront: the chunk sums of a stack's n pairs -> lam[0..2]
+__device__ __forceinline__ void lambda_final(const double2* __restrict__ part, int n, double eps, double* lam) {
+  const int C = lambda_chunk(n), G = (n + C - 1) / C, l = threadIdx.x;
+  double2 v = make_double2(0.0, 0.0);
+  if (l < G) v = part[l * C];
+  const double ta = wave_sum(v.x), tb = wave_sum(v.y);
+  if (l == 0) {
+    double f = ta / (tb + eps);
+    f = f < 1.0 ? f : 1.0;
+    f = f > 0.0 ? f : 0.0;
+    lam[0] = f;
     lam[1] = ta;
     lam[2] = tb;
   }
 }
-// workgroup s: stack s's partials part[s nb .. s nb + nb) -> lam[4 s ..]
-__global__ void __launch_bounds__(LBS) k_lambda(const double2* __restrict__ part, int nb, double eps, double* lam) {
-  lambda_of(part + (long)blockIdx.x * nb, nb, eps, lam + 4 * (long)blockIdx.x);
+
+// stack s = blockIdx.y: part[s nb .. s nb + nb)
+__global__ void __launch_bounds__(LBS) k_lambda_chunks(double2* __restrict__ part, int nb) {
+  lambda_chunk_sum(part + (long)blockIdx.y * nb, nb);
 }
-// workgroup s: the ranges[s].y partials of stack s from ranges[s].x
-__global__ void __launch_bounds__(LBS) k_lambda_ragged(const double2* __restrict__ part,
-                                                       const int2* __restrict__ ranges, double eps, double* lam) {
+__global__ void __launch_bounds__(64) k_lambda(const double2* __restrict__ part, int nb, double eps, double* lam) {
+  lambda_final(part + (long)blockIdx.x * nb, nb, eps, lam + 4 * (long)blockIdx.x);
+}
+// stack s: the ranges[s].y pairs from ranges[s].x
+__global__ void __launch_bounds__(LBS) k_lambda_chunks_ragged(double2* __restrict__ part,
+                                                              const int2* __restrict__ ranges) {
+  const int2 r = ranges[blockIdx.y];
+  lambda_chunk_sum(part + r.x, r.y);
+}
+__global__ void __launch_bounds__(64) k_lambda_ragged(const double2* __restrict__ part,
+                                                      const int2* __restrict__ ranges, double eps, double* lam) {
   const int2 r = ranges[blockIdx.x];
-  lambda_of(part + r.x, r.y, eps, lam + 4 * (long)blockIdx.x);
+  lambda_final(part + r.x, r.y, eps, lam + 4 * (long)blockIdx.x);
 }
-void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam, int nstacks) {
-  hipLaunchKernelGGL(k_lambda, dim3(nstacks), dim3(LBS), 0, st, reinterpret_cast<const double2*>(part), nb, eps,
-                     lam);
+void accel_factor(hipStream_t st, double* part, int nb, double eps, double* lam, int nstacks) {
+  double2* p = reinterpret_cast<double2*>(part);
+  hipLaunchKernelGGL(k_lambda_chunks, dim3(LCH, nstacks), dim3(LBS), 0, st, p, nb);
+  CDN_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_lambda, dim3(nstacks), dim3(64), 0, st, p, nb, eps, lam);
   CDN_HIP(hipGetLastError());
 }
-void accel_factor_ragged(hipStream_t st, const double* part, const int* ranges, int nstacks, double eps,
-                         double* lam) {
-  hipLaunchKernelGGL(k_lambda_ragged, dim3(nstacks), dim3(LBS), 0, st, reinterpret_cast<const double2*>(part),
-                     reinterpret_cast<const int2*>(ranges), eps, lam);
+void accel_factor_ragged(hipStream_t st, double* part, const int* ranges, int nstacks, double eps, double* lam) {
+  double2* p = reinterpret_cast<double2*>(part);
+  const int2* r = reinterpret_cast<const int2*>(ranges);
+  hipLaunchKernelGGL(k_lambda_chunks_ragged, dim3(LCH, nstacks), dim3(LBS), 0, st, p, r);
+  CDN_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_lambda_ragged, dim3(nstacks), dim3(64), 0, st, p, r, eps, lam);
   CDN_HIP(hipGetLastError());
 }
 

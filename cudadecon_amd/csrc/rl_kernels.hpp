@@ -33,12 +33,12 @@ void wiener_multiply(hipStream_t st, float2* S, int nxh, int ny, int nz, long pi
 void stream_copy(hipStream_t st, const void* in, void* out, size_t bytes);
 void copy_f32(hipStream_t st, const float* in, float* out, size_t n);
 // acceleration factor of nstacks stacks, nb FP64 partial pairs each
-// (stack s: part[2 (s nb ..)]) -> lam[4 s] = factor, lam[4 s + 1..2] = sums
-void accel_factor(hipStream_t st, const double* part, int nb, double eps, double* lam, int nstacks = 1);
+// (stack s: part[2 (s nb ..)]) -> lam[4 s] = factor, lam[4 s + 1..2] = sums.
+// The partials are used up: chunk sums are written over some of them.
+void accel_factor(hipStream_t st, double* part, int nb, double eps, double* lam, int nstacks = 1);
 // the same for stacks of different sizes: stack s sums the ranges[2 s + 1]
 // partial pairs from pair ranges[2 s] (ranges: device, 2 ints per stack)
-void accel_factor_ragged(hipStream_t st, const double* part, const int* ranges, int nstacks, double eps,
-                         double* lam);
+void accel_factor_ragged(hipStream_t st, double* part, const int* ranges, int nstacks, double eps, double* lam);
 void deskew(hipStream_t st, const float* in, int nx, int ny, int nz, double factor, float* out, int nxOut,
             int extraShift, float padVal);
 void rotate(hipStream_t st, const float* in, int nx, int ny, int nz, const float* rot, float* out, int nx_out,

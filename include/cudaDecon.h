@@ -241,7 +241,7 @@ int cudaDecon_device_copy(void* dst, const void* src, size_t bytes, int kind);
  *   captured once into a hipGraph and replayed;
  *   3 = grouped: every pass is one launch per distinct transform length,
  *   spanning all stacks of that length whatever their other sizes (about
- *   3 nx + 4 ny + 2 nz + 1 launches per iteration for the distinct lengths);
+ *   3 nx + 4 ny + 2 nz + 2 launches per iteration for the distinct lengths);
  *   4 = the mode-3 schedule captured once into a hipGraph and replayed.
  * Every mode gives each stack bitwise the result of a one-stack engine.
  * *ms receives the device time of the whole batch.  set_raw / set_raw_block

@@ -134,7 +134,7 @@ def test_grouped_modes_match_single_engines(cdn, orc):
     b = cdn.Batch(MIXED, tabs, ks, nstreams=3, table_of=table_of)
     # distinct nx, ny, nz: 5, 4, 4 (an x group splits after a stack whose rows
     # do not fill whole update workgroups)
-    assert 3 * 5 + 4 * 4 + 2 * 4 + 1 <= b.launches() <= 3 * 7 + 4 * 4 + 2 * 4 + 1 < 9 * len(MIXED)
+    assert 3 * 5 + 4 * 4 + 2 * 4 + 2 <= b.launches() <= 3 * 7 + 4 * 4 + 2 * 4 + 2 < 9 * len(MIXED)
     for i, R in enumerate(raws):
         b.set_raw(i, R)
     for mode in ("grouped", "grouped_graph", "sequential", "grouped_graph", "graph", "grouped"):
@@ -153,7 +153,7 @@ def test_grouped_modes_match_single_engines(cdn, orc):
     for parts in (2, 3):
         b.set_partitions(parts)
         # the schedule is rebuilt at once: each partition launches its own
-        # passes (and its own factor launch)
+        # passes (and its own two factor launches)
         assert b.launches() >= one + parts - 1, (parts, b.launches(), one)
         for mode in ("grouped", "grouped_graph", "grouped_graph"):
             print("partitions", parts, mode, flush=True)
