@@ -199,7 +199,12 @@ constexpr XShape x_shape(int N, int xthreads = -1, int kind = XK_PLAIN) {
   s.E = N / s.TPT;
   s.SHUFFLE = plan_exact(N, s.TPT) && s.TPT <= 64 && (64 % s.TPT) == 0 && (s.E % 2) == 0;
   const int ov = xthreads >= 0 ? xthreads : lookup_override(kXThreadsOverride, N);
-  const int xt = ov > 0 ? ov : 128;
+  // LRcore lines whose thread count neither divides nor is a multiple of 64
+  // straddle wavefronts: two lines per workgroup leave fewer lanes idle
+  // (810: 90 threads per line, 0.200-0.203 -> 0.187-0.190 ms at
+  // 128 x 512 x 810 in three alternations; five lines 0.217-0.220)
+  const bool straddle = kind == XK_LR && s.TPT % 64 != 0 && 64 % s.TPT != 0;
+  const int xt = ov > 0 ? ov : straddle ? 256 : 128;
   const int nl0 = (xt / s.TPT) < 1 ? 1 : (xt / s.TPT);
   s.NL = (nl0 * N * 8 > 65536) ? ((65536 / (N * 8)) < 1 ? 1 : 65536 / (N * 8)) : nl0;
   s.PAD = s.TPT < 64 ? s.TPT : 0;  // spreads lines over LDS banks
