@@ -30,6 +30,7 @@
 #include "rl_kernels.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cstdio>
 #include <fstream>
@@ -85,8 +86,13 @@ std::vector<FftVariant> x_candidates(int n) {
     if (xs.THREADS > 1024 || xs.THREADS < 64) continue;
     FftVariant v;
     if (t != 128) v.xthreads = t;
+    // distinct when any x pass (update, LRcore, R2C) gets another workgroup
+    auto threads = [&](int xt) {
+      return std::array<int, 3>{x_shape(n, xt, XK_PLAIN).THREADS, x_shape(n, xt, XK_LR).THREADS,
+                                x_shape(n, xt, XK_R2C).THREADS};
+    };
     bool dup = false;
-    for (const FftVariant& o : out) dup |= x_shape(n, o.xthreads).THREADS == xs.THREADS;
+    for (const FftVariant& o : out) dup |= threads(o.xthreads) == threads(v.xthreads);
     if (!dup) out.push_back(v);
   }
   return out;
