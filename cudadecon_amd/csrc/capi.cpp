@@ -833,8 +833,8 @@ int cudaDecon_rl_set_raw_u16(cudaDecon_rl* h, const unsigned short* raw, float o
     const Fft3& F = h->eng.fft;
     h->u16.ensure(sizeof(uint16_t) * (size_t)F.nvox);
     copy_to_device(h->st, h->u16.p, raw, sizeof(uint16_t) * (size_t)F.nvox);
-    u16_to_f32_crop(h->st, h->u16.as<uint16_t>(), F.nx, F.ny, F.nz, h->eng.raw.as<float>(), F.nx, F.ny, F.nz);
-    bgsub(h->st, h->eng.raw.as<float>(), F.nvox, offset);  // src/RLgpuImpl.cu:124-132 with bg = offset
+    // src/RLgpuImpl.cu:124-132 with bg = offset
+    u16_to_f32_bgsub(h->st, h->u16.as<uint16_t>(), F.nx, F.ny, F.nz, h->eng.raw.as<float>(), offset);
     h->eng.reset(h->st);
     CDN_HIP(hipStreamSynchronize(h->st));
     h->k_next = 0;

@@ -22,19 +22,36 @@ static inline unsigned nblocks(long n, int per_thread = 1) {
 // ---------------------------------------------------------------------------
 // uint16 -> float with corner crop (src/RL-Biggs-Andrews.cpp:674-677)
 // ---------------------------------------------------------------------------
+// One workgroup per output row (y, z), its threads walking x: no index
+// division per element.  SUB: max(v - bg, 0) on the way (k_bgsub's
+// arithmetic, one pass instead of two).
+template <bool SUB>
 __global__ void k_u16_crop(const uint16_t* __restrict__ in, int nx, int ny, float* __restrict__ out, int onx, int ony,
-                           long n_out) {
-  for (long i = (long)blockIdx.x * BS + threadIdx.x; i < n_out; i += (long)gridDim.x * BS) {
-    const long x = i % onx;
-    const long r = i / onx;
-    const long y = r % ony, z = r / ony;
-    out[i] = (float)in[(z * ny + y) * nx + x];
+                           float bg) {
+  const int y = blockIdx.x, z = blockIdx.y;
+  const uint16_t* src = in + ((long)z * ny + y) * nx;
+  float* dst = out + ((long)z * ony + y) * onx;
+  for (int x = threadIdx.x; x < onx; x += blockDim.x) {
+    const float v = (float)src[x];
+    if constexpr (SUB) {
+      const float d = v - bg;
+      dst[x] = d > 0.f ? d : 0.f;
+    } else {
+      dst[x] = v;
+    }
   }
 }
+static unsigned u16_row_threads(int onx) { return onx >= 256 ? 256u : (unsigned)((onx + 63) / 64 * 64); }
 void u16_to_f32_crop(hipStream_t st, const uint16_t* in, int nx, int ny, int nz, float* out, int onx, int ony,
                      int onz) {
-  const long n = (long)onx * ony * onz;
-  hipLaunchKernelGGL(k_u16_crop, dim3(nblocks(n)), dim3(BS), 0, st, in, nx, ny, out, onx, ony, n);
+  if ((long)onx * ony * onz == 0) return;
+  hipLaunchKernelGGL(k_u16_crop<false>, dim3(ony, onz), dim3(u16_row_threads(onx)), 0, st, in, nx, ny, out, onx, ony,
+                     0.f);
+  CDN_HIP(hipGetLastError());
+}
+void u16_to_f32_bgsub(hipStream_t st, const uint16_t* in, int nx, int ny, int nz, float* out, float bg) {
+  if ((long)nx * ny * nz == 0) return;
+  hipLaunchKernelGGL(k_u16_crop<true>, dim3(ny, nz), dim3(u16_row_threads(nx)), 0, st, in, nx, ny, out, nx, ny, bg);
   CDN_HIP(hipGetLastError());
 }
 

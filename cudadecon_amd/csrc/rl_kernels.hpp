@@ -11,6 +11,9 @@ void u16_to_f32_crop(hipStream_t st, const uint16_t* in, int nx, int ny, int nz,
                      int onz);
 void apodize(hipStream_t st, float* img, int nx, int ny, int nz, int napodize);
 void bgsub(hipStream_t st, float* img, long n, float bg);
+// uint16 nz x ny x nx -> FP32 max(v - bg, 0) (u16_to_f32_crop without a
+// crop, then bgsub, in one pass)
+void u16_to_f32_bgsub(hipStream_t st, const uint16_t* in, int nx, int ny, int nz, float* out, float bg);
 void rescale(hipStream_t st, float* img, long n, float s);
 // d_part: >= 2048 doubles, d_acc: >= 4 doubles, d_mean: 1 float.  Synchronises.
 double mean_above_background(hipStream_t st, const float* img, long n, double* d_part, double* d_acc, float* d_mean);
